@@ -1,0 +1,177 @@
+/*
+ * lgi.h — C ABI of the MI355X-native LuminaGI reference tracer + denoiser (liblgi.so).
+ *
+ * Plain C: POD structs, pointers and sizes; no C++ or torch types cross this line.
+ * Every entry point returns an lgi_status; nothing calls exit(). Use one context
+ * per host thread. Device work is ordered on the context's HIP stream (the null
+ * stream unless lgi_create/lgi_set_stream was given one).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   LoadSceneJSON            include/scene.h:58,  src/scene_loader.cpp:26-138
+ *   BuildBVH                 include/bvh.h:26,    src/bvh.cpp:164-211
+ *   RenderSceneCUDA          src/main.cpp:19-24,  src/pathtracer.cu:581-757
+ *   RenderSceneCUDASorted    src/main.cpp:25-27,  src/pathtracer.cu:761-909
+ *   overwrite_camera         src/main.cpp:29-59
+ *   read_batch_file          src/main.cpp:61-91
+ *   tonemap_kernel           src/pathtracer.cu:413-429
+ *   gbuffer_kernel           src/pathtracer.cu:366-411
+ *   UNet layers (PyTorch)    ml/train.py:37-75 (conv3x3/1x1, pool, upsample, concat)
+ */
+#ifndef LGI_H
+#define LGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGI_API __attribute__((visibility("default")))
+
+typedef enum lgi_status {
+    LGI_OK              = 0,
+    LGI_ERR_INVALID_ARG = 1,  /* null pointer, size out of range, bad flag */
+    LGI_ERR_IO          = 2,  /* file cannot be opened / written */
+    LGI_ERR_PARSE       = 3,  /* malformed JSON or missing key (reference throws, we return) */
+    LGI_ERR_VERSION     = 4,  /* scene.json "version" != 1 */
+    LGI_ERR_HIP         = 5,  /* HIP runtime error (message in lgi_last_error) */
+    LGI_ERR_NO_SCENE    = 6,  /* render/build before a scene was set */
+    LGI_ERR_NO_BVH      = 7,  /* render before lgi_bvh_build */
+    LGI_ERR_UNSUPPORTED = 8   /* feature not built (e.g. textured materials) */
+} lgi_status;
+
+typedef struct lgi_ctx lgi_ctx;
+
+/* ---- POD mirrors of the reference data model (include/scene.h:6-56, include/bvh.h:8-16) ---- */
+
+typedef struct lgi_camera {          /* CameraDesc, include/scene.h:6-16 */
+    float   pos[3];
+    float   camera_to_world[16];     /* rows 0/1/2 = forward/left/up (src/pathtracer.cu:656-659) */
+    float   fov_y_deg;
+    float   aspect;
+    float   near_z;
+    float   far_z;
+    int32_t width;
+    int32_t height;
+} lgi_camera;
+
+typedef struct lgi_dir_light {       /* DirectionalLight, 28 B */
+    float dir[3];                    /* travel direction; direction to light = -dir */
+    float color[3];
+    float intensity;
+} lgi_dir_light;
+
+typedef struct lgi_point_light {     /* PointLight, 32 B */
+    float pos[3];
+    float color[3];
+    float intensity;
+    float radius;
+} lgi_point_light;
+
+typedef struct lgi_material {        /* Material without the texture path */
+    float albedo[3];
+    float emissive[3];
+} lgi_material;
+
+typedef struct lgi_triangle {        /* Triangle, 100 B, byte-identical layout */
+    float   v[3][3];
+    float   n[3][3];
+    float   uv[3][2];
+    int32_t material;
+} lgi_triangle;
+
+typedef struct lgi_bvh_node {        /* BvhNode, 40 B, byte-identical layout */
+    float   bb_min[3];
+    float   bb_max[3];
+    int32_t left;                    /* -1 for a leaf */
+    int32_t right;                   /* -1 for a leaf */
+    int32_t first_tri;               /* leaf: first triangle in the permuted order; internal: -1 */
+    int32_t tri_count;               /* 0 internal, >0 leaf */
+} lgi_bvh_node;
+
+typedef struct lgi_batch_job {       /* BatchJob, src/main.cpp:63-69 */
+    char    out_png[512];
+    int32_t spp;
+    int32_t bounces;
+    float   cam_pos[3];
+    float   cam_tgt[3];
+} lgi_batch_job;
+
+/* ---- render flags ---- */
+#define LGI_RNG_PCG32          0u   /* PCG32 per (pixel, sample) stream (default) */
+#define LGI_RNG_XORSHIFT_REF   1u   /* reference xorshift32 chunked seeding (src/pathtracer.cu:349) */
+#define LGI_RENDER_SORTED      2u   /* wavefront path: radix-sort rays between bounces */
+#define LGI_RENDER_NO_RR       4u   /* disable Russian roulette (reference sorted path semantics) */
+
+/* ---- context ---- */
+LGI_API lgi_status  lgi_create(lgi_ctx** out, int device, void* hip_stream);
+LGI_API void        lgi_destroy(lgi_ctx* ctx);
+LGI_API const char* lgi_last_error(const lgi_ctx* ctx);
+LGI_API lgi_status  lgi_set_stream(lgi_ctx* ctx, void* hip_stream);
+LGI_API const char* lgi_version(void);
+
+/* ---- scene (LoadSceneJSON, include/scene.h:58) ---- */
+LGI_API lgi_status lgi_scene_load_json(lgi_ctx* ctx, const char* path, int verbose);
+LGI_API lgi_status lgi_scene_save_json(lgi_ctx* ctx, const char* path);
+LGI_API lgi_status lgi_scene_set(lgi_ctx* ctx, const lgi_camera* cam, const lgi_dir_light* sun,
+                                 const lgi_dir_light* dirs, int32_t n_dirs,
+                                 const lgi_point_light* points, int32_t n_points,
+                                 const lgi_material* mats, int32_t n_mats,
+                                 const lgi_triangle* tris, int64_t n_tris);
+/* Procedural scenes: kind 1 = ~65k-triangle test scene, kind 2 = 62 meshes / ~1.05M triangles. */
+LGI_API lgi_status lgi_scene_generate(lgi_ctx* ctx, int32_t kind, uint64_t seed,
+                                      int32_t width, int32_t height);
+LGI_API lgi_status lgi_scene_counts(const lgi_ctx* ctx, int64_t* n_tris, int32_t* n_mats,
+                                    int32_t* n_points, int32_t* n_dirs);
+LGI_API lgi_status lgi_scene_get(const lgi_ctx* ctx, lgi_camera* cam, lgi_dir_light* sun,
+                                 lgi_point_light* points, lgi_material* mats,
+                                 lgi_triangle* tris /* current order (permuted after a build) */);
+LGI_API lgi_status lgi_scene_set_camera(lgi_ctx* ctx, const lgi_camera* cam);
+/* overwrite_camera, src/main.cpp:29-59 (no-op when |target-pos| < 1e-8). */
+LGI_API lgi_status lgi_camera_look_at(lgi_camera* cam, const float pos[3], const float target[3]);
+
+/* ---- BVH (BuildBVH, include/bvh.h:26) ---- */
+/* GPU LBVH. Like BuildBVH it permutes the context's triangles into Morton order. */
+LGI_API lgi_status lgi_bvh_build(lgi_ctx* ctx);
+LGI_API lgi_status lgi_bvh_node_count(const lgi_ctx* ctx, int64_t* n_nodes);
+/* nodes: n_nodes entries in BFS order (root 0); tri_perm[i] = original index of the
+ * triangle now at position i; keys: sorted 30-bit Morton codes. Any output may be null. */
+LGI_API lgi_status lgi_bvh_get(lgi_ctx* ctx, lgi_bvh_node* nodes, int32_t* tri_perm, uint32_t* keys);
+/* Milliseconds of the last GPU build (device time, HIP events). */
+LGI_API lgi_status lgi_bvh_last_build_ms(const lgi_ctx* ctx, float* ms);
+
+/* ---- rendering (RenderSceneCUDA / RenderSceneCUDASorted) ---- */
+/* Host-buffer render of the scene camera (or *cam when non-null): rgb = W*H*3 u8
+ * sqrt-tonemapped; hdr (optional) = W*H*3 f32 mean radiance; alb/nrm/wp (all three or
+ * none) = G-buffer encodings. */
+LGI_API lgi_status lgi_render(lgi_ctx* ctx, const lgi_camera* cam, int32_t spp, int32_t bounces,
+                              uint32_t flags, uint8_t* rgb, float* hdr,
+                              uint8_t* alb, uint8_t* nrm, uint8_t* wp);
+/* Device-buffer render: adds spp samples (sample indices [sample0, sample0+spp)) into
+ * d_sum (W*H*3 f32) with Kahan compensation d_comp (W*H*3 f32) and per-pixel sum of
+ * squared luminance d_sumsq (W*H f32, may be null). Buffers are caller-owned device memory. */
+LGI_API lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam, int32_t spp,
+                                         int32_t bounces, uint32_t flags, int32_t sample0,
+                                         float* d_sum, float* d_comp, float* d_sumsq);
+/* Primary-hit triangle ids (original scene index, -1 on miss) and t at pixel centres. */
+LGI_API lgi_status lgi_primary_hits(lgi_ctx* ctx, const lgi_camera* cam, int32_t* d_tri, float* d_t);
+/* Device tonemap: u8 = (u8)(sqrt(clamp(sum*inv_spp,0,1))*255). */
+LGI_API lgi_status lgi_tonemap(lgi_ctx* ctx, const float* d_sum, int32_t n_px, float inv_spp,
+                               uint8_t* d_rgb);
+/* Device G-buffer (albedo, normal, worldpos encodings) for camera cam. */
+LGI_API lgi_status lgi_gbuffer(lgi_ctx* ctx, const lgi_camera* cam, uint8_t* d_alb,
+                               uint8_t* d_nrm, uint8_t* d_wp);
+/* Kernel time of the last render call (ms, HIP events). */
+LGI_API lgi_status lgi_last_render_ms(const lgi_ctx* ctx, float* ms);
+
+/* ---- batch (read_batch_file, src/main.cpp:61-91) ---- */
+LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs);
+
+/* ---- image output (stbi_write_png call sites, src/main.cpp:176,214) ---- */
+LGI_API lgi_status lgi_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LGI_H */

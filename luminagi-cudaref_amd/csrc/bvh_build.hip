@@ -1,0 +1,585 @@
+// bvh_build.hip — GPU LBVH for gfx950, bit-exact to the reference CPU builder
+// (src/bvh.cpp:11-211): same 30-bit Morton codes, same split positions, same leaf rule
+// (<= 4 triangles, or a run of identical codes), same BFS node numbering.
+//
+// Pipeline (all on the context stream):
+//   1. scene AABB (block min/max reduction)                       src/bvh.cpp:170-178
+//   2. centroid Morton codes                                       src/bvh.cpp:180-192
+//   3. stable LSD radix sort (key, triangle) — 4 x 8-bit passes    src/bvh.cpp:194-195
+//   4. Karras binary radix tree over (key, index)-augmented codes; its split positions
+//      are the reference find_split positions (src/bvh.cpp:55-65, 84-90)
+//   5. reference nodes = root + children of "splittable" Karras nodes (count > 4 and
+//      first key != last key), i.e. the reference leaf rule (src/bvh.cpp:92)
+//   6. BFS numbering (src/bvh.cpp:124-162): sort reference nodes by (depth, first index);
+//      within a level, left-to-right order is range order
+//   7. bottom-up AABB refit with per-node atomic arrival counters; leaf boxes from the
+//      triangle vertices in order, parent box = left box then strict-union right box
+//      (src/bvh.cpp:96-103, 115-117)
+//   8. traversal layout: TNode (child boxes in the parent), TriV (v0, e1, e2), TriS.
+#include <algorithm>
+#include <cstring>
+
+#include "lgi_internal.h"
+#include "lgi_math.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kItems = 8;
+constexpr int kTile = kBlock * kItems;   // keys per radix-sort block
+constexpr int kDepthShift = 26;          // BFS key = depth << 26 | first
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ scan (exclusive, u32)
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        uint32_t t = s_wave[w];
+        if (w < wave) wpre += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return wpre + x - v;
+}
+
+// In-place exclusive scan of one kTile chunk per block; block totals to sums.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* data, int n, uint32_t* sums) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t acc = 0;
+    for (int k = 0; k < kItems; ++k) {
+        int i = base + k;
+        v[k] = (i < n) ? data[i] : 0u;
+        acc += v[k];
+    }
+    uint32_t total;
+    uint32_t pre = block_exclusive_scan(acc, s_wave, total);
+    for (int k = 0; k < kItems; ++k) {
+        int i = base + k;
+        if (i < n) data[i] = pre;
+        pre += v[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Single-block exclusive scan of an arbitrary-length array (carry across chunks).
+__global__ __launch_bounds__(kBlock) void k_scan_single(uint32_t* data, int n, uint32_t* total_out) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    uint32_t carry = 0;
+    for (int off = 0; off < n; off += kTile) {
+        const int base = off + threadIdx.x * kItems;
+        uint32_t v[kItems];
+        uint32_t acc = 0;
+        for (int k = 0; k < kItems; ++k) {
+            int i = base + k;
+            v[k] = (i < n) ? data[i] : 0u;
+            acc += v[k];
+        }
+        uint32_t total;
+        uint32_t pre = block_exclusive_scan(acc, s_wave, total) + carry;
+        for (int k = 0; k < kItems; ++k) {
+            int i = base + k;
+            if (i < n) data[i] = pre;
+            pre += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* data, int n, const uint32_t* sums) {
+    const uint32_t add = sums[blockIdx.x];
+    const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    for (int k = 0; k < kItems; ++k) {
+        int i = base + k;
+        if (i < n) data[i] += add;
+    }
+}
+
+// ------------------------------------------------------------------ radix sort
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int n, int shift,
+                                                       uint32_t* counts, int nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+    for (int j = 0; j < kItems; ++j) {
+        int i = base + j * kBlock + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: within a tile, element order is (round j, thread); ranks among equal
+// digits come from a 64-lane match built from 8 ballots plus per-wave digit counts.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int32_t* vin,
+                                                          uint32_t* kout, int32_t* vout, int n, int shift,
+                                                          const uint32_t* offs, int nb) {
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_wc[kBlock / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    s_base[tid] = offs[(size_t)tid * nb + blockIdx.x];
+    for (int w = 0; w < kBlock / 64; ++w) s_wc[w][tid] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kTile;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j = 0; j < kItems; ++j) {
+        const int i = base + j * kBlock + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0xFFFFFFFFu;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = ~0ull;
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if ((__ffsll((long long)peers) - 1) == lane) s_wc[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int w = 0; w < wave; ++w) pre += s_wc[w][d];
+        if (valid) {
+            const uint32_t pos = s_base[d] + pre + rank;
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < kBlock / 64; ++w) { add += s_wc[w][tid]; s_wc[w][tid] = 0; }
+        s_base[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ scene bbox + Morton
+__global__ __launch_bounds__(kBlock) void k_bbox_partial(const lgi_triangle* tris, int n, float* partial) {
+    __shared__ float s[6][kBlock];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const lgi_triangle& t = tris[i];
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) {
+                float p = t.v[c][k];
+                mn[k] = fminf(mn[k], p);
+                mx[k] = fmaxf(mx[k], p);
+            }
+    }
+    for (int k = 0; k < 3; ++k) { s[k][threadIdx.x] = mn[k]; s[3 + k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// out = sceneMin[3], ext[3] with ext <= 0 replaced by 1 (src/bvh.cpp:175-178)
+__global__ void k_bbox_final(const float* partial, int nb, float* out) {
+    if (threadIdx.x != 0) return;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fminf(mn[k], partial[b * 6 + k]);
+            mx[k] = fmaxf(mx[k], partial[b * 6 + 3 + k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+        float e = mx[k] - mn[k];
+        out[k] = mn[k];
+        out[3 + k] = (e <= 0.0f) ? 1.0f : e;
+    }
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quant10(float f) {
+    f = f * 1024.0f;
+    if (f < 0.0f) f = 0.0f;
+    if (f > 1023.0f) f = 1023.0f;
+    return (uint32_t)f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const lgi_triangle* tris, int n, const float* bb,
+                                                   uint32_t* keys, int32_t* vals) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const lgi_triangle& t = tris[i];
+    const float third = 1.0f / 3.0f;
+    float cx = (t.v[0][0] + t.v[1][0] + t.v[2][0]) * third;
+    float cy = (t.v[0][1] + t.v[1][1] + t.v[2][1]) * third;
+    float cz = (t.v[0][2] + t.v[1][2] + t.v[2][2]) * third;
+    uint32_t x = quant10(__fdiv_rn(cx - bb[0], bb[3]));
+    uint32_t y = quant10(__fdiv_rn(cy - bb[1], bb[4]));
+    uint32_t z = quant10(__fdiv_rn(cz - bb[2], bb[5]));
+    keys[i] = (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+    vals[i] = i;
+}
+
+// Sorted vertex array (9 floats per triangle) in Morton order.
+__global__ __launch_bounds__(kBlock) void k_gather_verts(const lgi_triangle* tris, const int32_t* perm, int n,
+                                                         float* sv) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const lgi_triangle& t = tris[perm[i]];
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) sv[(size_t)i * 9 + c * 3 + k] = t.v[c][k];
+}
+
+// ------------------------------------------------------------------ Karras tree
+__device__ __forceinline__ int delta(const uint32_t* k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const uint32_t a = k[i], b = k[j];
+    if (a != b) return __clz(a ^ b);
+    return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// Internal node i (0..n-2). Node ids: internal i, leaf k -> n-1+k.
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, int32_t* child, int32_t* parent,
+                                                   int32_t* rfirst, int32_t* rlast, uint8_t* splittable) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    const int first = min(i, j), last = max(i, j);
+    const int left = (first == gamma) ? (n - 1 + gamma) : gamma;
+    const int right = (last == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    child[2 * i] = left;
+    child[2 * i + 1] = right;
+    parent[left] = i;
+    parent[right] = i;
+    rfirst[i] = first;
+    rlast[i] = last;
+    splittable[i] = (last - first + 1 > 4) && (keys[first] != keys[last]);
+    if (i == 0) parent[0] = -1;
+}
+
+// Reference nodes: the root and every child of a splittable node. Emits (BFS key, id).
+__global__ __launch_bounds__(kBlock) void k_ref_nodes(int n, const int32_t* parent, const int32_t* rfirst,
+                                                      const uint8_t* splittable, uint32_t* out_keys,
+                                                      int32_t* out_ids, uint32_t* counter) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    if (x >= 2 * n - 1) return;
+    bool ref;
+    if (x == 0) ref = true;
+    else {
+        const int p = parent[x];
+        ref = (p >= 0) && splittable[p];
+    }
+    if (!ref) return;
+    const int first = (x < n - 1) ? rfirst[x] : x - (n - 1);
+    int depth = 0;
+    for (int y = x; y != 0; y = parent[y]) ++depth;
+    const uint32_t slot = atomicAdd(counter, 1u);
+    out_keys[slot] = ((uint32_t)depth << kDepthShift) | (uint32_t)first;
+    out_ids[slot] = x;
+}
+
+__global__ void k_init_single(int32_t* parent, int32_t* ids, int32_t* new_index) {
+    parent[0] = -1;
+    ids[0] = 0;
+    new_index[0] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_new_index(const int32_t* bfs_ids, int m, int32_t* new_index) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) new_index[bfs_ids[p]] = p;
+}
+
+__device__ __forceinline__ void fit3(float* mn, float* mx, const float* p) {
+    for (int k = 0; k < 3; ++k) {
+        if (p[k] < mn[k]) mn[k] = p[k];
+        if (p[k] > mx[k]) mx[k] = p[k];
+    }
+}
+
+// Leaf boxes, then walk up: the second child to arrive at a parent builds its box.
+__global__ __launch_bounds__(kBlock) void k_leaves_refit(int n, int m, const int32_t* bfs_ids,
+                                                         const int32_t* new_index, const int32_t* parent,
+                                                         const int32_t* child, const int32_t* rfirst,
+                                                         const int32_t* rlast, const uint8_t* splittable,
+                                                         const float* sv, uint32_t* arrivals,
+                                                         lgi_bvh_node* nodes) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int x = bfs_ids[p];
+    const bool leaf = (x >= n - 1) || !splittable[x];
+    if (!leaf) return;
+    const int first = (x < n - 1) ? rfirst[x] : x - (n - 1);
+    const int last = (x < n - 1) ? rlast[x] : first;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = first; i <= last; ++i)
+        for (int c = 0; c < 3; ++c) fit3(mn, mx, sv + (size_t)i * 9 + c * 3);
+    lgi_bvh_node nd;
+    for (int k = 0; k < 3; ++k) { nd.bb_min[k] = mn[k]; nd.bb_max[k] = mx[k]; }
+    nd.left = -1; nd.right = -1; nd.first_tri = first; nd.tri_count = last - first + 1;
+    nodes[p] = nd;
+    int y = x;
+    for (;;) {
+        const int py = parent[y];
+        if (py < 0) break;
+        __threadfence();
+        if (atomicAdd(&arrivals[py], 1u) == 0u) break;
+        __threadfence();
+        const int cl = child[2 * py], cr = child[2 * py + 1];
+        const int il = new_index[cl], ir = new_index[cr];
+        const lgi_bvh_node L = nodes[il];
+        const lgi_bvh_node R = nodes[ir];
+        lgi_bvh_node o;
+        for (int k = 0; k < 3; ++k) {
+            o.bb_min[k] = L.bb_min[k];
+            o.bb_max[k] = L.bb_max[k];
+            if (R.bb_min[k] < o.bb_min[k]) o.bb_min[k] = R.bb_min[k];
+            if (R.bb_max[k] > o.bb_max[k]) o.bb_max[k] = R.bb_max[k];
+        }
+        o.left = il; o.right = ir; o.first_tri = -1; o.tri_count = 0;
+        nodes[new_index[py]] = o;
+        y = py;
+    }
+}
+
+// ------------------------------------------------------------------ traversal layout
+__global__ __launch_bounds__(kBlock) void k_internal_flags(const lgi_bvh_node* nodes, int m, uint32_t* flags,
+                                                           uint8_t* leaf_end) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const lgi_bvh_node nd = nodes[p];
+    flags[p] = nd.tri_count == 0 ? 1u : 0u;
+    if (nd.tri_count > 0) leaf_end[nd.first_tri + nd.tri_count - 1] = 1;
+}
+
+__device__ __forceinline__ int child_ref(const lgi_bvh_node& c, const uint32_t* tidx, int ci) {
+    return c.tri_count == 0 ? (int)tidx[ci] : ~c.first_tri;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tnodes(const lgi_bvh_node* nodes, int m, const uint32_t* tidx,
+                                                   TNode* out) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const lgi_bvh_node nd = nodes[p];
+    if (nd.tri_count != 0) return;
+    const lgi_bvh_node a = nodes[nd.left], b = nodes[nd.right];
+    TNode t;
+    t.b0xy = make_float4(a.bb_min[0], a.bb_max[0], a.bb_min[1], a.bb_max[1]);
+    t.b1xy = make_float4(b.bb_min[0], b.bb_max[0], b.bb_min[1], b.bb_max[1]);
+    t.bz = make_float4(a.bb_min[2], a.bb_max[2], b.bb_min[2], b.bb_max[2]);
+    t.ch = make_int4(child_ref(a, tidx, nd.left), child_ref(b, tidx, nd.right), 0, 0);
+    out[tidx[p]] = t;
+}
+
+// Root-is-leaf case: one TNode whose two children are both the root leaf.
+__global__ void k_tnode_root_leaf(const lgi_bvh_node* nodes, TNode* out) {
+    const lgi_bvh_node a = nodes[0];
+    TNode t;
+    t.b0xy = make_float4(a.bb_min[0], a.bb_max[0], a.bb_min[1], a.bb_max[1]);
+    t.b1xy = t.b0xy;
+    t.bz = make_float4(a.bb_min[2], a.bb_max[2], a.bb_min[2], a.bb_max[2]);
+    t.ch = make_int4(~a.first_tri, ~a.first_tri, 0, 0);
+    out[0] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_layout(const lgi_triangle* tris, const int32_t* perm, int n,
+                                                       const float* sv, const uint8_t* leaf_end, TriV* triv,
+                                                       TriS* tris_s) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float* v = sv + (size_t)i * 9;
+    TriV t;
+    // e1/e2 exactly as ray_tri forms them (src/pathtracer.cu:86-88)
+    t.v0 = make_float4(v[0], v[1], v[2], __int_as_float(leaf_end[i] ? 1 : 0));
+    t.e1 = make_float4(v[3] - v[0], v[4] - v[1], v[5] - v[2], __int_as_float(perm[i]));
+    t.e2 = make_float4(v[6] - v[0], v[7] - v[1], v[8] - v[2], 0.0f);
+    triv[i] = t;
+    const lgi_triangle& src = tris[perm[i]];
+    TriS s;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 3; ++k) s.n[c * 3 + k] = src.n[c][k];
+    s.mat = src.material;
+    tris_s[i] = s;
+}
+
+struct SortBufs {
+    uint32_t* k[2];
+    int32_t* v[2];
+    uint32_t* counts;
+    uint32_t* sums;
+};
+
+// Stable LSD radix sort of (keys, vals) over `bits` low bits (a multiple of 16, so the
+// even number of ping-pong passes leaves the result in buffer 0).
+void radix_sort(hipStream_t st, SortBufs& b, int n, int bits) {
+    if (n <= 1) return;
+    const int nb = ceil_div(n, kTile);
+    const int ncount = 256 * nb;
+    const int nsb = ceil_div(ncount, kTile);
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, st, b.k[cur], n, shift, b.counts, nb);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, b.sums, nsb, (uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, st, b.k[cur], b.v[cur], b.k[cur ^ 1],
+                           b.v[cur ^ 1], n, shift, b.counts, nb);
+        cur ^= 1;
+    }
+}
+
+template <class T>
+lgi_status dev_alloc(lgi_ctx* ctx, T** p, size_t count) {
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    LGI_HIP(ctx, hipMalloc((void**)p, count * sizeof(T)));
+    return LGI_OK;
+}
+
+}  // namespace
+
+void lgi_free_dev_scene(DevScene& d) {
+    void* ptrs[] = {d.tri_in, d.nodes, d.tnodes, d.triv, d.tris, d.mat_albedo, d.points, d.perm, d.keys};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    d = DevScene();
+}
+
+lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
+    const int64_t n64 = ctx->dev.n_tris;
+    if (n64 <= 0) return lgi_set_err(ctx, LGI_ERR_NO_SCENE, "scene has no triangles");
+    if (n64 >= (int64_t(1) << kDepthShift))
+        return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "too many triangles for the GPU LBVH (max 2^26 - 1)");
+    const int n = (int)n64;
+    hipStream_t st = ctx->stream;
+    DevScene& d = ctx->dev;
+    lgi_status s;
+    if ((s = dev_alloc(ctx, &d.perm, n)) != LGI_OK) return s;
+    if ((s = dev_alloc(ctx, &d.keys, n)) != LGI_OK) return s;
+    if ((s = dev_alloc(ctx, &d.triv, n)) != LGI_OK) return s;
+    if ((s = dev_alloc(ctx, &d.tris, n)) != LGI_OK) return s;
+    if ((s = dev_alloc(ctx, &d.nodes, 2 * (size_t)n)) != LGI_OK) return s;
+
+    // scratch
+    const int nb_sort = ceil_div(n, kTile);
+    const int npart = std::min(1024, ceil_div(n, kBlock));
+    auto S = [&](int slot, size_t bytes) { return ctx->scratch_get(slot, bytes); };
+    float* partial = (float*)S(1, (size_t)npart * 6 * 4);
+    float* bb = (float*)S(2, 6 * 4 + 64);
+    uint32_t* counter = (uint32_t*)((char*)bb + 32);
+    SortBufs sb;
+    const size_t m_cap = 2 * (size_t)n;  // reference nodes <= 2n-1
+    sb.k[0] = (uint32_t*)S(3, m_cap * 4);
+    sb.k[1] = (uint32_t*)S(4, m_cap * 4);
+    sb.v[0] = (int32_t*)S(5, m_cap * 4);
+    sb.v[1] = (int32_t*)S(6, m_cap * 4);
+    sb.counts = (uint32_t*)S(7, (size_t)256 * ceil_div(m_cap, kTile) * 4 + 64);
+    sb.sums = (uint32_t*)S(8, (size_t)(ceil_div(256 * (size_t)ceil_div(m_cap, kTile), kTile) + 16) * 4);
+    float* sv = (float*)S(9, (size_t)n * 9 * 4);
+    int32_t* child = (int32_t*)S(10, (size_t)2 * n * 4);
+    int32_t* parent = (int32_t*)S(11, (size_t)2 * n * 4);
+    int32_t* rfirst = (int32_t*)S(12, (size_t)n * 4);
+    int32_t* rlast = (int32_t*)S(13, (size_t)n * 4);
+    uint8_t* splittable = (uint8_t*)S(14, (size_t)n + 64);
+    int32_t* new_index = (int32_t*)S(15, (size_t)2 * n * 4);
+    uint32_t* arrivals = (uint32_t*)S(16, (size_t)n * 4);
+    uint32_t* tflags = (uint32_t*)S(17, m_cap * 4 + 64);
+    uint8_t* leaf_end = (uint8_t*)S(18, (size_t)n + 64);
+    uint32_t* tsums = (uint32_t*)S(19, (size_t)(ceil_div(m_cap, kTile) + 16) * 4);
+    for (void* p : {(void*)partial, (void*)sb.k[0], (void*)sv, (void*)child, (void*)new_index, (void*)arrivals,
+                    (void*)tflags, (void*)leaf_end, (void*)tsums})
+        if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "scratch allocation failed");
+    (void)nb_sort;
+
+    LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(k_bbox_partial, dim3(npart), dim3(kBlock), 0, st, d.tri_in, n, partial);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, partial, npart, bb);
+    hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, n, bb, sb.k[0], sb.v[0]);
+    radix_sort(st, sb, n, 32);
+    LGI_HIP(ctx, hipMemcpyAsync(d.keys, sb.k[0], (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    LGI_HIP(ctx, hipMemcpyAsync(d.perm, sb.v[0], (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_gather_verts, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, d.perm, n, sv);
+
+    int64_t m = 1;
+    if (n == 1) {
+        // single triangle: one leaf; reuse the refit kernel with a trivial tree
+        hipLaunchKernelGGL(k_init_single, dim3(1), dim3(1), 0, st, parent, sb.v[0], new_index);
+        hipLaunchKernelGGL(k_leaves_refit, dim3(1), dim3(kBlock), 0, st, n, 1, sb.v[0], new_index, parent, child,
+                           rfirst, rlast, splittable, sv, arrivals, d.nodes);
+    } else {
+        hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, kBlock)), dim3(kBlock), 0, st, d.keys, n, child, parent,
+                           rfirst, rlast, splittable);
+        LGI_HIP(ctx, hipMemsetAsync(counter, 0, 4, st));
+        hipLaunchKernelGGL(k_ref_nodes, dim3(ceil_div(2 * n - 1, kBlock)), dim3(kBlock), 0, st, n, parent, rfirst,
+                           splittable, sb.k[0], sb.v[0], counter);
+        uint32_t hm = 0;
+        LGI_HIP(ctx, hipMemcpyAsync(&hm, counter, 4, hipMemcpyDeviceToHost, st));
+        LGI_HIP(ctx, hipStreamSynchronize(st));
+        m = hm;
+        radix_sort(st, sb, (int)m, 32);
+        hipLaunchKernelGGL(k_new_index, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, sb.v[0], (int)m, new_index);
+        LGI_HIP(ctx, hipMemsetAsync(arrivals, 0, (size_t)n * 4, st));
+        hipLaunchKernelGGL(k_leaves_refit, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, n, (int)m, sb.v[0],
+                           new_index, parent, child, rfirst, rlast, splittable, sv, arrivals, d.nodes);
+    }
+    // traversal layout
+    LGI_HIP(ctx, hipMemsetAsync(leaf_end, 0, (size_t)n, st));
+    hipLaunchKernelGGL(k_internal_flags, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, d.nodes, (int)m, tflags,
+                       leaf_end);
+    const int nsb = ceil_div(m, kTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nsb), dim3(kBlock), 0, st, tflags, (int)m, tsums);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, tsums, nsb, counter);
+    hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(kBlock), 0, st, tflags, (int)m, tsums);
+    uint32_t n_int = 0;
+    LGI_HIP(ctx, hipMemcpyAsync(&n_int, counter, 4, hipMemcpyDeviceToHost, st));
+    LGI_HIP(ctx, hipStreamSynchronize(st));
+    d.root_is_leaf = (n_int == 0);
+    const int64_t n_t = std::max<int64_t>(1, n_int);
+    if ((s = dev_alloc(ctx, &d.tnodes, n_t)) != LGI_OK) return s;
+    if (n_int == 0)
+        hipLaunchKernelGGL(k_tnode_root_leaf, dim3(1), dim3(1), 0, st, d.nodes, d.tnodes);
+    else
+        hipLaunchKernelGGL(k_tnodes, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, d.nodes, (int)m, tflags,
+                           d.tnodes);
+    hipLaunchKernelGGL(k_tri_layout, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, d.perm, n, sv,
+                       leaf_end, d.triv, d.tris);
+    LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));
+    LGI_HIP(ctx, hipGetLastError());
+    LGI_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    LGI_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_build_ms = ms;
+    d.n_nodes = m;
+    d.n_tnodes = n_t;
+    ctx->bvh_ready = true;
+    return LGI_OK;
+}

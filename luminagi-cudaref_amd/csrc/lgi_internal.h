@@ -1,0 +1,107 @@
+// lgi_internal.h — host-side context and device data layouts shared by the HIP
+// translation units of liblgi.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lgi.h"
+#include "scene_host.h"
+
+// ------------------------------------------------------------------ device layouts
+// Traversal node: both children's boxes live in the parent (64 B, one 64-B line), so a
+// wave fetches one node per step and tests two boxes. Child refs: >= 0 internal node
+// index; < 0 leaf starting at triangle ~ref (leaf triangles end at the TriV end flag).
+struct alignas(16) TNode {
+    float4 b0xy;   // child0 lo.x, hi.x, lo.y, hi.y
+    float4 b1xy;   // child1 lo.x, hi.x, lo.y, hi.y
+    float4 bz;     // child0 lo.z, hi.z, child1 lo.z, hi.z
+    int4 ch;       // child0 ref, child1 ref, unused, unused
+};
+static_assert(sizeof(TNode) == 64, "TNode is one 64-byte line");
+
+// Intersection-only triangle record in leaf order: v0, e1 = v1 - v0, e2 = v2 - v0 (the
+// exact FP32 differences ray_tri forms, src/pathtracer.cu:86-88). v0.w holds the
+// end-of-leaf flag bit, e1.w the original scene index (as int bits).
+struct alignas(16) TriV {
+    float4 v0;
+    float4 e1;
+    float4 e2;
+};
+static_assert(sizeof(TriV) == 48, "TriV");
+
+// Shading record in leaf order: vertex normals and material index.
+struct TriS {
+    float n[9];
+    int32_t mat;
+};
+static_assert(sizeof(TriS) == 40, "TriS");
+
+struct DevLights {
+    float sun_dir[3];
+    float sun_color[3];
+    float sun_intensity;
+    int32_t n_points;
+    const lgi_point_light* points;   // device array
+};
+
+struct DevCamera {
+    float pos[3], fwd[3], left[3], up[3];
+    float tan_half_fov, aspect;
+    int32_t width, height;
+};
+
+// Device-resident scene and acceleration structure (kept across renders and batch jobs).
+struct DevScene {
+    int64_t n_tris = 0;
+    int64_t n_nodes = 0;        // BvhNode count (reference format)
+    int64_t n_tnodes = 0;       // traversal nodes
+    lgi_triangle* tri_in = nullptr;   // original order, AoS 100 B
+    lgi_bvh_node* nodes = nullptr;    // BFS order, reference format
+    TNode* tnodes = nullptr;
+    TriV* triv = nullptr;
+    TriS* tris = nullptr;
+    float* mat_albedo = nullptr;      // 3 per material
+    lgi_point_light* points = nullptr;
+    int32_t n_points = 0;
+    int32_t* perm = nullptr;          // sorted position -> original index
+    uint32_t* keys = nullptr;         // sorted Morton keys
+    int32_t root_is_leaf = 0;
+    int32_t n_mats = 0;
+};
+
+struct lgi_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    HostScene scene;             // host copy (original order until a build permutes it)
+    bool gpu_ready = false;      // device selected, events created
+    bool uploaded = false;       // dev mirrors `scene`
+    DevScene dev;
+    bool bvh_ready = false;
+    float last_build_ms = 0.f;
+    float last_render_ms = 0.f;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // scratch buffers (grow-only)
+    std::vector<std::pair<void*, size_t>> scratch;
+    void* scratch_get(int slot, size_t bytes);
+};
+
+// ------------------------------------------------------------------ helpers
+lgi_status lgi_set_err(lgi_ctx* ctx, lgi_status st, const std::string& msg);
+
+#define LGI_HIP(ctx, call)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return lgi_set_err((ctx), LGI_ERR_HIP,                                        \
+                               std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+// Kernel-side entry points implemented in the .hip files.
+lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx);
+void lgi_free_dev_scene(DevScene& d);
+DevCamera lgi_make_dev_camera(const lgi_camera& c);
+lgi_status lgi_upload_lights(lgi_ctx* ctx, DevLights* out);

@@ -1,0 +1,320 @@
+// trace.hip — path-tracing kernels for gfx950 (wave64).
+//
+// Semantics follow the reference renderer (src/pathtracer.cu:61-429): Moller-Trumbore with
+// the (1e-6, bestT) acceptance window, slab test with the `lo > tmax` cull, sun + point-light
+// NEE with shadow rays, cosine bounces, Russian roulette from bounce 1, sqrt tonemap,
+// G-buffer encodings. Every floating-point expression keeps the reference operation order
+// and the file is built with -ffp-contract=off, so FP32 results are bit-identical to the CPU
+// oracle; closest hits can differ only where two triangles tie in distance, because the
+// traversal visits the nearer child first.
+//
+// Layout: TNode (both child boxes in the parent, 64 B), TriV (v0,e1,e2 = 48 B) in leaf order,
+// TriS (normals + material) read once per closest hit. One lane per pixel, 8x8 pixel tile
+// per wave, 4 waves per block. The traversal stack lives in LDS as [depth][thread], so lanes
+// at different depths still hit distinct banks; the tree depth is at most 30 internal levels
+// (one per Morton bit), so 32 entries never overflow.
+#include "lgi_internal.h"
+#include "lgi_math.h"
+#include "trace_kernels.h"
+
+using namespace lgi;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStack = 32;
+
+__device__ __forceinline__ bool slab(const Ray& r, float lox, float hix, float loy, float hiy, float loz,
+                                     float hiz, float tmax, float& tenter) {
+    // src/pathtracer.cu:61-80, same expression tree
+    float t0 = (lox - r.o.x) * r.inv.x;
+    float t1 = (hix - r.o.x) * r.inv.x;
+    float lo = fminf(t0, t1), hi = fmaxf(t0, t1);
+    t0 = (loy - r.o.y) * r.inv.y;
+    t1 = (hiy - r.o.y) * r.inv.y;
+    lo = fmaxf(lo, fminf(t0, t1));
+    hi = fminf(hi, fmaxf(t0, t1));
+    t0 = (loz - r.o.z) * r.inv.z;
+    t1 = (hiz - r.o.z) * r.inv.z;
+    lo = fmaxf(lo, fminf(t0, t1));
+    hi = fminf(hi, fmaxf(t0, t1));
+    tenter = fmaxf(lo, 0.0f);
+    return !(hi < tenter || lo > tmax);
+}
+
+// Moller-Trumbore on the precomputed edges (src/pathtracer.cu:82-107).
+__device__ __forceinline__ bool tri_hit(const Ray& r, const TriV& t, float tmax, float& outT, float& outU,
+                                        float& outV) {
+    const float kEps = 1e-6f;
+    V3 v0 = v3(t.v0.x, t.v0.y, t.v0.z);
+    V3 e1 = v3(t.e1.x, t.e1.y, t.e1.z);
+    V3 e2 = v3(t.e2.x, t.e2.y, t.e2.z);
+    V3 p = cross(r.d, e2);
+    float det = dot(e1, p);
+    if (det > -kEps && det < kEps) return false;
+    float invDet = __fdiv_rn(1.0f, det);
+    V3 s = sub(r.o, v0);
+    float u = dot(s, p) * invDet;
+    if (u < 0.0f || u > 1.0f) return false;
+    V3 q = cross(s, e1);
+    float v = dot(r.d, q) * invDet;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float tt = dot(e2, q) * invDet;
+    if (tt <= kEps || tt >= tmax) return false;
+    outT = tt;
+    outU = u;
+    outV = v;
+    return true;
+}
+
+struct Hit {
+    int tri;   // leaf-order index, -1 = miss
+    float t, u, v;
+};
+
+// Closest hit (kAny = false) or any hit within (1e-6, tmax) (kAny = true).
+template <bool kAny>
+__device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, int* stk) {
+    Hit h{-1, tmax, 0.f, 0.f};
+    int node = 0;
+    int sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            const TNode n = P.tnodes[node];
+            float ta, tb;
+            const bool ha = slab(r, n.b0xy.x, n.b0xy.y, n.b0xy.z, n.b0xy.w, n.bz.x, n.bz.y, h.t, ta);
+            const bool hb = slab(r, n.b1xy.x, n.b1xy.y, n.b1xy.z, n.b1xy.w, n.bz.z, n.bz.w, h.t, tb);
+            if (ha && hb) {
+                int nearc = n.ch.x, farc = n.ch.y;
+                if (tb < ta) { nearc = n.ch.y; farc = n.ch.x; }
+                stk[sp * kBlock] = farc;
+                ++sp;
+                node = nearc;
+                continue;
+            }
+            if (ha) { node = n.ch.x; continue; }
+            if (hb) { node = n.ch.y; continue; }
+        } else {
+            int ti = ~node;
+            for (;;) {
+                const TriV t = P.triv[ti];
+                float tt, uu, vv;
+                if (tri_hit(r, t, h.t, tt, uu, vv)) {
+                    h.tri = ti; h.t = tt; h.u = uu; h.v = vv;
+                    if (kAny) return h;
+                }
+                if (__float_as_int(t.v0.w) != 0) break;
+                ++ti;
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        node = stk[sp * kBlock];
+    }
+    return h;
+}
+
+// Shading normal at the closest hit (src/pathtracer.cu:149-155).
+__device__ __forceinline__ V3 hit_normal(const TraceParams& P, const Hit& h, int& mat) {
+    const TriS s = P.tris[h.tri];
+    float w = 1.0f - h.u - h.v;
+    mat = s.mat;
+    return normalize(v3(w * s.n[0] + h.u * s.n[3] + h.v * s.n[6], w * s.n[1] + h.u * s.n[4] + h.v * s.n[7],
+                        w * s.n[2] + h.u * s.n[5] + h.v * s.n[8]));
+}
+
+// Sun + point-light NEE (src/pathtracer.cu:230-272). A point light farther than its radius
+// contributes albedo*color*0 = +0, so its shadow ray is skipped: bit-identical result.
+__device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* stk) {
+    const float kShadowBias = 1e-3f;
+    V3 color = v3(0.f, 0.f, 0.f);
+    {
+        V3 toLight = normalize(mul(v3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]), -1.0f));
+        float ndotl = fmaxf(0.0f, dot(N, toLight));
+        if (ndotl > 0.0f) {
+            Ray shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
+            if (traverse<true>(P, shadow, 1e30f, stk).tri < 0) {
+                float I = P.sun_intensity * ndotl;
+                color = add(color, had(albedo, mul(v3(P.sun_color[0], P.sun_color[1], P.sun_color[2]), I)));
+            }
+        }
+    }
+    for (int i = 0; i < P.n_points; ++i) {
+        const lgi_point_light L = P.points[i];
+        V3 toLightVec = sub(v3(L.pos[0], L.pos[1], L.pos[2]), Pt);
+        float dist = __fsqrt_rn(dot(toLightVec, toLightVec));
+        V3 toLight = mul(toLightVec, __fdiv_rn(1.0f, dist));
+        float ndotl = fmaxf(0.0f, dot(N, toLight));
+        if (ndotl <= 0.0f || dist >= L.radius * 3.0f) continue;
+        if (dist >= L.radius) continue;  // atten == 0 exactly
+        Ray shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
+        if (traverse<true>(P, shadow, dist - kShadowBias, stk).tri >= 0) continue;
+        float r = L.radius;
+        float atten = fmaxf(0.0f, 1.0f - __fdiv_rn(dist, r));
+        atten = atten * atten;
+        float I = L.intensity * ndotl * atten;
+        color = add(color, had(albedo, mul(v3(L.color[0], L.color[1], L.color[2]), I)));
+    }
+    return color;
+}
+
+// Primary ray through (px + jx, py + jy) (src/pathtracer.cu:284-293).
+__device__ __forceinline__ Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
+    float ndcx = __fdiv_rn(2.0f * ((float)px + jx), (float)c.width) - 1.0f;
+    float ndcy = 1.0f - __fdiv_rn(2.0f * ((float)py + jy), (float)c.height);
+    float rx = ndcx * c.aspect * c.tan_half_fov;
+    float ry = ndcy * c.tan_half_fov;
+    V3 fwd = v3(c.fwd[0], c.fwd[1], c.fwd[2]);
+    V3 left = v3(c.left[0], c.left[1], c.left[2]);
+    V3 up = v3(c.up[0], c.up[1], c.up[2]);
+    V3 dw = add(add(mul(fwd, 1.0f), mul(left, -rx)), mul(up, ry));
+    return make_ray(v3(c.pos[0], c.pos[1], c.pos[2]), dw);
+}
+
+// trace_path (src/pathtracer.cu:295-331).
+__device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, int* stk) {
+    V3 L = v3(0.f, 0.f, 0.f);
+    V3 T = v3(1.f, 1.f, 1.f);
+    const bool rr = (P.flags & LGI_RENDER_NO_RR) == 0;
+    for (int bounce = 0; bounce <= P.bounces; ++bounce) {
+        Hit h = traverse<false>(P, ray, 1e30f, stk);
+        if (h.tri < 0) break;
+        int mat;
+        V3 N = hit_normal(P, h, mat);
+        if (dot(N, ray.d) > 0.0f) N = mul(N, -1.0f);
+        V3 Pt = add(ray.o, mul(ray.d, h.t));
+        V3 albedo = v3(P.mat_albedo[3 * mat], P.mat_albedo[3 * mat + 1], P.mat_albedo[3 * mat + 2]);
+        L = add(L, had(T, shade_direct(P, Pt, N, albedo, stk)));
+        if (bounce == P.bounces) break;
+        if (bounce >= 1 && rr) {
+            float q = fminf(0.95f, fmaxf(T.x, fmaxf(T.y, T.z)));
+            if (rng.next() > q) break;
+            T = mul(T, __fdiv_rn(1.0f, q));
+        }
+        V3 nd = sample_cos_hemi(N, rng);
+        ray = make_ray(add(Pt, mul(N, 1e-3f)), nd);
+        T = had(T, albedo);
+    }
+    return L;
+}
+
+// Pixel of lane `tid` in block (bx, by): 16x16 pixels per block, 8x8 per wave.
+__device__ __forceinline__ void tile_pixel(int bx, int by, int tid, int& px, int& py) {
+    const int lane = tid & 63, wave = tid >> 6;
+    px = bx * 16 + (wave & 1) * 8 + (lane & 7);
+    py = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
+// accumulate_kernel (src/pathtracer.cu:333-364): spp samples per pixel, Kahan-compensated.
+__global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
+    __shared__ int s_stack[kStack * kBlock];
+    int px, py;
+    tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
+    if (px >= P.cam.width || py >= P.cam.height) return;
+    int* stk = s_stack + threadIdx.x;
+    const uint32_t pix = (uint32_t)py * (uint32_t)P.cam.width + (uint32_t)px;
+    float s0 = P.sum[3 * pix], s1 = P.sum[3 * pix + 1], s2 = P.sum[3 * pix + 2];
+    float c0 = P.comp[3 * pix], c1 = P.comp[3 * pix + 1], c2 = P.comp[3 * pix + 2];
+    float sq = P.sumsq ? P.sumsq[pix] : 0.0f;
+    const bool xs = (P.flags & LGI_RNG_XORSHIFT_REF) != 0;
+    Rng rng;
+    rng.pcg = !xs;
+    for (int k = 0; k < P.spp; ++k) {
+        const int sample = P.sample0 + k;
+        if (xs) {
+            // src/pathtracer.cu:349 with 8-sample launch chunks (:699-702)
+            if (k == 0 || (sample & 7) == 0) {
+                const uint32_t base = (uint32_t)(sample - (sample & 7));
+                rng.xs = ((uint32_t)px * 1973u + (uint32_t)py * 9277u + base * 26699u) | 1u;
+            }
+        } else {
+            pcg_for_sample(rng.g, P.seed, pix, (uint32_t)sample);
+        }
+        float jx = rng.next();
+        float jy = rng.next();
+        Ray r = primary_ray(P.cam, px, py, jx, jy);
+        V3 L = trace_path(P, r, rng, stk);
+        kahan_add(s0, c0, L.x);
+        kahan_add(s1, c1, L.y);
+        kahan_add(s2, c2, L.z);
+        float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+        sq += lum * lum;
+    }
+    P.sum[3 * pix] = s0; P.sum[3 * pix + 1] = s1; P.sum[3 * pix + 2] = s2;
+    P.comp[3 * pix] = c0; P.comp[3 * pix + 1] = c1; P.comp[3 * pix + 2] = c2;
+    if (P.sumsq) P.sumsq[pix] = sq;
+}
+
+// Pixel-centre primary hits: original triangle index and t (src/pathtracer.cu:381-384).
+__global__ __launch_bounds__(kBlock) void k_primary(TraceParams P, int32_t* tri_out, float* t_out) {
+    __shared__ int s_stack[kStack * kBlock];
+    int px, py;
+    tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
+    if (px >= P.cam.width || py >= P.cam.height) return;
+    Ray r = primary_ray(P.cam, px, py, 0.5f, 0.5f);
+    Hit h = traverse<false>(P, r, 1e30f, s_stack + threadIdx.x);
+    const size_t i = (size_t)py * P.cam.width + px;
+    tri_out[i] = h.tri < 0 ? -1 : __float_as_int(P.triv[h.tri].e1.w);
+    t_out[i] = h.tri < 0 ? -1.0f : h.t;
+}
+
+// gbuffer_kernel (src/pathtracer.cu:366-411).
+__global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin, float3 invExt, uint8_t* alb,
+                                                    uint8_t* nrm, uint8_t* wp) {
+    __shared__ int s_stack[kStack * kBlock];
+    int px, py;
+    tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
+    if (px >= P.cam.width || py >= P.cam.height) return;
+    Ray r = primary_ray(P.cam, px, py, 0.5f, 0.5f);
+    Hit h = traverse<false>(P, r, 1e30f, s_stack + threadIdx.x);
+    const size_t i = ((size_t)py * P.cam.width + px) * 3;
+    if (h.tri < 0) {
+        alb[i] = alb[i + 1] = alb[i + 2] = 0;
+        nrm[i] = 128; nrm[i + 1] = 128; nrm[i + 2] = 255;
+        wp[i] = wp[i + 1] = wp[i + 2] = 0;
+        return;
+    }
+    int mat;
+    V3 N = hit_normal(P, h, mat);
+    if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
+    auto enc = [](float v) { return (uint8_t)(fminf(1.0f, fmaxf(0.0f, v)) * 255.0f); };
+    alb[i] = enc(P.mat_albedo[3 * mat]);
+    alb[i + 1] = enc(P.mat_albedo[3 * mat + 1]);
+    alb[i + 2] = enc(P.mat_albedo[3 * mat + 2]);
+    nrm[i] = enc(N.x * 0.5f + 0.5f);
+    nrm[i + 1] = enc(N.y * 0.5f + 0.5f);
+    nrm[i + 2] = enc(N.z * 0.5f + 0.5f);
+    V3 Pt = add(r.o, mul(r.d, h.t));
+    wp[i] = enc((Pt.x - bbMin.x) * invExt.x);
+    wp[i + 1] = enc((Pt.y - bbMin.y) * invExt.y);
+    wp[i + 2] = enc((Pt.z - bbMin.z) * invExt.z);
+}
+
+// tonemap_kernel (src/pathtracer.cu:413-429), 4 channels-of-bytes per thread.
+__global__ __launch_bounds__(kBlock) void k_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float v = __fsqrt_rn(fmaxf(0.0f, fminf(sum[i] * inv_spp, 1.0f)));
+    out[i] = (uint8_t)(v * 255.0f);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host launchers
+void lgi_launch_render(const TraceParams& P, hipStream_t st) {
+    dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);
+    hipLaunchKernelGGL(k_render, grid, dim3(kBlock), 0, st, P);
+}
+void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_t st) {
+    dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);
+    hipLaunchKernelGGL(k_primary, grid, dim3(kBlock), 0, st, P, tri, t);
+}
+void lgi_launch_gbuffer(const TraceParams& P, float3 bbMin, float3 invExt, uint8_t* alb, uint8_t* nrm, uint8_t* wp,
+                        hipStream_t st) {
+    dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);
+    hipLaunchKernelGGL(k_gbuffer, grid, dim3(kBlock), 0, st, P, bbMin, invExt, alb, nrm, wp);
+}
+void lgi_launch_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_tonemap, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, sum, n, inv_spp,
+                       out);
+}

@@ -1,0 +1,30 @@
+// trace_kernels.h — parameter block and launchers of the tracer kernels (trace.hip).
+#pragma once
+#include "lgi_internal.h"
+
+struct TraceParams {
+    const TNode* tnodes;
+    const TriV* triv;
+    const TriS* tris;
+    const float* mat_albedo;
+    const lgi_point_light* points;
+    int32_t n_points;
+    float sun_dir[3];
+    float sun_color[3];
+    float sun_intensity;
+    DevCamera cam;
+    int32_t spp;
+    int32_t bounces;
+    int32_t sample0;
+    uint32_t flags;
+    uint32_t seed;
+    float* sum;     // W*H*3
+    float* comp;    // W*H*3 Kahan compensation
+    float* sumsq;   // W*H (nullable)
+};
+
+void lgi_launch_render(const TraceParams& P, hipStream_t st);
+void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_t st);
+void lgi_launch_gbuffer(const TraceParams& P, float3 bbMin, float3 invExt, uint8_t* alb, uint8_t* nrm, uint8_t* wp,
+                        hipStream_t st);
+void lgi_launch_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out, hipStream_t st);

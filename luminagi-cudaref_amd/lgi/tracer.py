@@ -1,0 +1,183 @@
+"""Python front end of liblgi: scene I/O, GPU LBVH and the path tracer.
+
+Mirrors the reference's host interface (LoadSceneJSON, BuildBVH, RenderSceneCUDA,
+RenderSceneCUDASorted; src/main.cpp:19-27, include/scene.h:58, include/bvh.h:26) on top of the
+C ABI in include/lgi.h. Host-side calls (scene generation, JSON, batch files, PNG) work
+without a GPU; every compute call runs the HIP kernels and raises LgiError on failure.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (BatchJob, Camera, DirLight, LgiError, Material, NODE_DTYPE, POINT_DTYPE, TRI_DTYPE, lib, ptr)
+
+
+def _check(ctx, status):
+    if status != _abi.LGI_OK:
+        msg = lib().lgi_last_error(ctx).decode() if ctx else ""
+        raise LgiError(status, msg)
+
+
+def camera_look_at(cam, pos, target):
+    """overwrite_camera (src/main.cpp:29-59) on a Camera struct, in place."""
+    p = (C.c_float * 3)(*pos)
+    t = (C.c_float * 3)(*target)
+    _check(None, lib().lgi_camera_look_at(C.byref(cam), p, t))
+    return cam
+
+
+def parse_batch(path):
+    """read_batch_file (src/main.cpp:61-91): list of (out_png, spp, bounces, cam_pos, cam_tgt)."""
+    n = C.c_int32(0)
+    _check(None, lib().lgi_batch_parse(str(path).encode(), None, 0, C.byref(n)))
+    jobs = (BatchJob * max(1, n.value))()
+    _check(None, lib().lgi_batch_parse(str(path).encode(), jobs, n.value, C.byref(n)))
+    return [(j.out_png.decode(), j.spp, j.bounces, tuple(j.cam_pos), tuple(j.cam_tgt)) for j in jobs[:n.value]]
+
+
+def write_png(path, rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+    h, w = rgb.shape[:2]
+    _check(None, lib().lgi_write_png(str(path).encode(), ptr(rgb), w, h))
+
+
+class Context:
+    """One liblgi context: a scene, its device copy and BVH, and the render entry points."""
+
+    def __init__(self, device=0, stream=None):
+        h = C.c_void_p()
+        _check(None, lib().lgi_create(C.byref(h), device, stream))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().lgi_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, status):
+        _check(self._h, status)
+
+    def set_stream(self, stream):
+        self._c(lib().lgi_set_stream(self._h, stream))
+
+    # ------------------------------------------------------------------ scene
+    def load_json(self, path, verbose=False):
+        self._c(lib().lgi_scene_load_json(self._h, str(path).encode(), int(verbose)))
+
+    def save_json(self, path):
+        self._c(lib().lgi_scene_save_json(self._h, str(path).encode()))
+
+    def generate(self, kind, seed, width, height):
+        self._c(lib().lgi_scene_generate(self._h, kind, seed, width, height))
+
+    def set_scene(self, camera, sun, points, materials, tris, dirs=()):
+        pts = np.ascontiguousarray(points, dtype=POINT_DTYPE)
+        tr = np.ascontiguousarray(tris, dtype=TRI_DTYPE)
+        mats = (Material * max(1, len(materials)))()
+        for i, (alb, emi) in enumerate(materials):
+            mats[i].albedo[:] = list(alb)
+            mats[i].emissive[:] = list(emi)
+        dl = (DirLight * max(1, len(dirs)))(*dirs)
+        self._c(lib().lgi_scene_set(self._h, C.byref(camera), C.byref(sun), dl, len(dirs), ptr(pts), len(pts),
+                                    mats, len(materials), ptr(tr), len(tr)))
+
+    def counts(self):
+        nt, nm, np_, nd = C.c_int64(), C.c_int32(), C.c_int32(), C.c_int32()
+        self._c(lib().lgi_scene_counts(self._h, C.byref(nt), C.byref(nm), C.byref(np_), C.byref(nd)))
+        return nt.value, nm.value, np_.value, nd.value
+
+    def scene(self):
+        """Returns dict(camera, sun, points, materials (n,2,3), tris) — tris in BVH order after a build."""
+        nt, nm, npt, _ = self.counts()
+        cam, sun = Camera(), DirLight()
+        pts = np.zeros(npt, POINT_DTYPE)
+        mats = (Material * max(1, nm))()
+        tris = np.zeros(nt, TRI_DTYPE)
+        self._c(lib().lgi_scene_get(self._h, C.byref(cam), C.byref(sun), ptr(pts) if npt else None, mats,
+                                    ptr(tris) if nt else None))
+        m = np.array([[list(mats[i].albedo), list(mats[i].emissive)] for i in range(nm)], np.float32).reshape(nm, 2, 3)
+        return {"camera": cam, "sun": sun, "points": pts, "materials": m, "tris": tris}
+
+    def camera(self):
+        cam = Camera()
+        self._c(lib().lgi_scene_get(self._h, C.byref(cam), None, None, None, None))
+        return cam
+
+    def set_camera(self, cam):
+        self._c(lib().lgi_scene_set_camera(self._h, C.byref(cam)))
+
+    # ------------------------------------------------------------------ BVH
+    def build_bvh(self):
+        self._c(lib().lgi_bvh_build(self._h))
+
+    def bvh(self):
+        """(nodes[NODE_DTYPE], perm int32, sorted keys uint32) of the last GPU build."""
+        n = C.c_int64()
+        self._c(lib().lgi_bvh_node_count(self._h, C.byref(n)))
+        nt = self.counts()[0]
+        nodes = np.zeros(n.value, NODE_DTYPE)
+        perm = np.zeros(nt, np.int32)
+        keys = np.zeros(nt, np.uint32)
+        self._c(lib().lgi_bvh_get(self._h, ptr(nodes), ptr(perm), ptr(keys)))
+        return nodes, perm, keys
+
+    def last_build_ms(self):
+        ms = C.c_float()
+        self._c(lib().lgi_bvh_last_build_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    # ------------------------------------------------------------------ render
+    def render(self, spp, bounces, flags=0, camera=None, hdr=False, gbuffer=False):
+        """RenderSceneCUDA: returns dict(rgb u8 HxWx3, hdr f32 HxWx3?, albedo/normal/worldpos?)."""
+        cam = camera if camera is not None else self.camera()
+        W, H = cam.width, cam.height
+        rgb = np.zeros((H, W, 3), np.uint8)
+        out = {"rgb": rgb}
+        hd = np.zeros((H, W, 3), np.float32) if hdr else None
+        gb = [np.zeros((H, W, 3), np.uint8) for _ in range(3)] if gbuffer else [None, None, None]
+        self._c(lib().lgi_render(self._h, C.byref(cam), spp, bounces, flags, ptr(rgb), ptr(hd), ptr(gb[0]),
+                                 ptr(gb[1]), ptr(gb[2])))
+        if hdr:
+            out["hdr"] = hd
+        if gbuffer:
+            out["albedo"], out["normal"], out["worldpos"] = gb
+        return out
+
+    def accumulate(self, d_sum, d_comp, spp, bounces, flags=0, sample0=0, d_sumsq=None, camera=None):
+        """Add spp samples into caller-owned device buffers (torch tensors or raw pointers)."""
+        cam = camera if camera is not None else self.camera()
+        self._c(lib().lgi_render_accumulate(self._h, C.byref(cam), spp, bounces, flags, sample0, _dp(d_sum),
+                                            _dp(d_comp), _dp(d_sumsq)))
+
+    def primary_hits(self, d_tri, d_t, camera=None):
+        cam = camera if camera is not None else self.camera()
+        self._c(lib().lgi_primary_hits(self._h, C.byref(cam), _dp(d_tri), _dp(d_t)))
+
+    def gbuffer(self, d_alb, d_nrm, d_wp, camera=None):
+        cam = camera if camera is not None else self.camera()
+        self._c(lib().lgi_gbuffer(self._h, C.byref(cam), _dp(d_alb), _dp(d_nrm), _dp(d_wp)))
+
+    def tonemap(self, d_sum, n_px, inv_spp, d_rgb):
+        self._c(lib().lgi_tonemap(self._h, _dp(d_sum), n_px, inv_spp, _dp(d_rgb)))
+
+    def last_render_ms(self):
+        ms = C.c_float()
+        self._c(lib().lgi_last_render_ms(self._h, C.byref(ms)))
+        return ms.value
+
+
+def _dp(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return ptr(x)
