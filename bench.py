@@ -1,0 +1,146 @@
+"""Benchmark of the plain path tracer on one MI355X.
+
+Workload: the 62-mesh / ~1.05M-triangle procedural scene, 1728x864, 2 bounces with
+sun + 2 point-light NEE, PCG32 streams. One step = one 8-spp frame rendered into device
+accumulators (inputs resident in HBM). Prints one JSON line with Msamples/s, the
+render kernel's roofline figures and the CPU oracle timed on the host beside it.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--spp 8]
+"""
+import argparse
+import json
+import os
+import pathlib
+import platform
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "luminagi-cudaref_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "ms to 40 dB PSNR @1728x864: 8spp+UNet vs plain spp; Msamples/s, 2 bounces"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (layout-independent minimum): 24 B box + 8 B links per node visit,
+# 3 x 12 B vertices per triangle test, 40 B (normals + material) per closest hit.
+BYTES_NODE, BYTES_TRI, BYTES_HIT = 32, 36, 40
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1):
+    """Oracle render of every row_step-th row at spp samples; returns (Msamples/s, bytes/sample, counters, s)."""
+    from oracle import oracle as orc
+    osc = orc.Scene(scene["tris"], scene["materials"][:, 0, :], scene["points"], scene["sun"], cam)
+    t0 = time.perf_counter()
+    out = osc.render(spp, bounces, row_step=row_step, threads=threads)
+    dt = time.perf_counter() - t0
+    rows = (cam.height + row_step - 1) // row_step
+    samples = rows * cam.width * spp
+    c = out["counters"]
+    bytes_per_sample = (BYTES_NODE * c["node_visits"] + BYTES_TRI * c["tri_tests"]
+                        + BYTES_HIT * c["closest_hits"]) / samples
+    t0 = time.perf_counter()
+    orc.bvh_build(scene["tris"])
+    bvh_s = time.perf_counter() - t0
+    return samples / dt / 1e6, bytes_per_sample, c, dt, samples, bvh_s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--bounces", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-row-step", type=int, default=2)
+    ap.add_argument("--cpu-spp", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the tracer would shard by pose or image tile with one accumulator reduce)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import lgi
+
+    dev = torch.device("cuda:0")
+    ctx = lgi.Context(0, torch.cuda.current_stream(dev).cuda_stream)
+    ctx.generate(2, args.seed, 1728, 864)
+    scene = ctx.scene()
+    cam = scene["camera"]
+    W, H = cam.width, cam.height
+    n_px = W * H
+    ctx.build_bvh()
+    torch.cuda.synchronize()
+    build_ms = [ctx.last_build_ms()]
+    for _ in range(3):
+        ctx.build_bvh()
+        build_ms.append(ctx.last_build_ms())
+    acc = torch.zeros(n_px * 3, device=dev)
+    comp = torch.zeros(n_px * 3, device=dev)
+    sumsq = torch.zeros(n_px, device=dev)
+
+    sample0 = 0
+    for _ in range(args.warmup):
+        ctx.accumulate(acc, comp, args.spp, args.bounces, sample0=sample0, d_sumsq=sumsq)
+        sample0 += args.spp
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.accumulate(acc, comp, args.spp, args.bounces, sample0=sample0, d_sumsq=sumsq)
+        kernel_ms.append(ctx.last_render_ms())  # HIP events on the context stream, per launch
+        sample0 += args.spp
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    samples_per_step = n_px * args.spp
+    msps = samples_per_step / (wall / args.steps) / 1e6
+    avg_kernel_ms = float(np.mean(kernel_ms))
+
+    result = {
+        "metric": METRIC, "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "plain path tracer, 62-mesh procedural scene (%d triangles), %dx%d, %d spp per "
+                               "step, %d bounces, sun + 2 point lights, PCG32" % (len(scene["tris"]), W, H,
+                                                                                  args.spp, args.bounces),
+                   "triangles": int(len(scene["tris"])), "width": W, "height": H, "spp_per_step": args.spp,
+                   "bounces": args.bounces, "scene_seed": args.seed},
+        "render_kernel_ms": round(avg_kernel_ms, 3),
+        "lbvh_build_ms": round(float(np.median(build_ms[1:])), 3),
+    }
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        cpu_msps, bps, counters, cpu_s, cpu_samples, bvh_s = cpu_baseline(scene, cam, args.bounces,
+                                                                           args.cpu_row_step, threads, args.cpu_spp)
+        algo_bytes = bps * samples_per_step
+        achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                              "bytes_per_sample": round(bps, 1)}
+        result["cpu_baseline"] = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": "CPU oracle, same scene and PCG32 streams, %d spp on every %dth row "
+                                            "(%d samples, %.1f s), %s" % (args.cpu_spp, args.cpu_row_step, cpu_samples,
+                                                                          cpu_s, cpu_model()),
+                                  "bvh_build_ms_1thread": round(bvh_s * 1e3, 1)}
+        result["oracle_counters_per_sample"] = {k: round(v / cpu_samples, 2) for k, v in counters.items()}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

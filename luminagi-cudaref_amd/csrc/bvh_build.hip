@@ -229,9 +229,9 @@ __global__ __launch_bounds__(kBlock) void k_morton(const lgi_triangle* tris, int
     float cx = (t.v[0][0] + t.v[1][0] + t.v[2][0]) * third;
     float cy = (t.v[0][1] + t.v[1][1] + t.v[2][1]) * third;
     float cz = (t.v[0][2] + t.v[1][2] + t.v[2][2]) * third;
-    uint32_t x = quant10(__fdiv_rn(cx - bb[0], bb[3]));
-    uint32_t y = quant10(__fdiv_rn(cy - bb[1], bb[4]));
-    uint32_t z = quant10(__fdiv_rn(cz - bb[2], bb[5]));
+    uint32_t x = quant10(lgi_div(cx - bb[0], bb[3]));
+    uint32_t y = quant10(lgi_div(cy - bb[1], bb[4]));
+    uint32_t z = quant10(lgi_div(cz - bb[2], bb[5]));
     keys[i] = (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
     vals[i] = i;
 }

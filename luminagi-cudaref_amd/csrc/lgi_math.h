@@ -9,6 +9,12 @@
 
 #define LGI_D __device__ __forceinline__
 
+// Correctly rounded FP32 square root and division. HIP's __fsqrt_rn maps to the
+// approximate native sqrt unless OCML_BASIC_ROUNDED_OPERATIONS is defined; the builtin
+// lowers to the IEEE-exact expansion (the default -fhip-fp32-correctly-rounded-divide-sqrt).
+LGI_D float lgi_sqrt(float x) { return __builtin_sqrtf(x); }
+LGI_D float lgi_div(float a, float b) { return a / b; }
+
 namespace lgi {
 
 struct V3 { float x, y, z; };
@@ -23,15 +29,15 @@ LGI_D V3 cross(V3 a, V3 b) {
 }
 // normalize with the 1e-20 fallback to +Z (include/vecmath.h:31-35): reciprocal, then scale.
 LGI_D V3 normalize(V3 a) {
-    float l = __fsqrt_rn(dot(a, a));
-    return (l > 1e-20f) ? mul(a, __fdiv_rn(1.0f, l)) : v3(0.f, 0.f, 1.f);
+    float l = lgi_sqrt(dot(a, a));
+    return (l > 1e-20f) ? mul(a, lgi_div(1.0f, l)) : v3(0.f, 0.f, 1.f);
 }
 struct Ray { V3 o, d, inv; };
 LGI_D Ray make_ray(V3 o, V3 d) {
     Ray r;
     r.o = o;
     r.d = normalize(d);
-    r.inv = {__fdiv_rn(1.0f, r.d.x), __fdiv_rn(1.0f, r.d.y), __fdiv_rn(1.0f, r.d.z)};
+    r.inv = {lgi_div(1.0f, r.d.x), lgi_div(1.0f, r.d.y), lgi_div(1.0f, r.d.z)};
     return r;
 }
 
@@ -102,7 +108,7 @@ struct Rng {
 // Frisvad ONB (src/pathtracer.cu:206-214)
 LGI_D void onb(V3 N, V3& T, V3& B) {
     if (N.z < -0.99999f) { T = v3(0.f, -1.f, 0.f); B = v3(-1.f, 0.f, 0.f); return; }
-    float a = __fdiv_rn(1.0f, 1.0f + N.z);
+    float a = lgi_div(1.0f, 1.0f + N.z);
     float b = -N.x * N.y * a;
     T = v3(1.0f - N.x * N.x * a, b, -N.x);
     B = v3(b, 1.0f - N.y * N.y * a, -N.y);
@@ -112,13 +118,13 @@ LGI_D void onb(V3 N, V3& T, V3& B) {
 LGI_D V3 sample_cos_hemi(V3 N, Rng& rng) {
     float u1 = rng.next();
     float u2 = rng.next();
-    float r = __fsqrt_rn(u1);
+    float r = lgi_sqrt(u1);
     float phi = 2.0f * 3.14159265358979f * u2;
     float sn, cs;
     own_sincos(phi, &sn, &cs);
     float x = r * cs;
     float y = r * sn;
-    float z = __fsqrt_rn(fmaxf(0.0f, 1.0f - u1));
+    float z = lgi_sqrt(fmaxf(0.0f, 1.0f - u1));
     V3 T, B;
     onb(N, T, B);
     return normalize(add(add(mul(T, x), mul(B, y)), mul(N, z)));

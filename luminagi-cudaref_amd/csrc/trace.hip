@@ -52,7 +52,7 @@ __device__ __forceinline__ bool tri_hit(const Ray& r, const TriV& t, float tmax,
     V3 p = cross(r.d, e2);
     float det = dot(e1, p);
     if (det > -kEps && det < kEps) return false;
-    float invDet = __fdiv_rn(1.0f, det);
+    float invDet = lgi_div(1.0f, det);
     V3 s = sub(r.o, v0);
     float u = dot(s, p) * invDet;
     if (u < 0.0f || u > 1.0f) return false;
@@ -142,15 +142,15 @@ __device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* st
     for (int i = 0; i < P.n_points; ++i) {
         const lgi_point_light L = P.points[i];
         V3 toLightVec = sub(v3(L.pos[0], L.pos[1], L.pos[2]), Pt);
-        float dist = __fsqrt_rn(dot(toLightVec, toLightVec));
-        V3 toLight = mul(toLightVec, __fdiv_rn(1.0f, dist));
+        float dist = lgi_sqrt(dot(toLightVec, toLightVec));
+        V3 toLight = mul(toLightVec, lgi_div(1.0f, dist));
         float ndotl = fmaxf(0.0f, dot(N, toLight));
         if (ndotl <= 0.0f || dist >= L.radius * 3.0f) continue;
         if (dist >= L.radius) continue;  // atten == 0 exactly
         Ray shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
         if (traverse<true>(P, shadow, dist - kShadowBias, stk).tri >= 0) continue;
         float r = L.radius;
-        float atten = fmaxf(0.0f, 1.0f - __fdiv_rn(dist, r));
+        float atten = fmaxf(0.0f, 1.0f - lgi_div(dist, r));
         atten = atten * atten;
         float I = L.intensity * ndotl * atten;
         color = add(color, had(albedo, mul(v3(L.color[0], L.color[1], L.color[2]), I)));
@@ -160,8 +160,8 @@ __device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* st
 
 // Primary ray through (px + jx, py + jy) (src/pathtracer.cu:284-293).
 __device__ __forceinline__ Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
-    float ndcx = __fdiv_rn(2.0f * ((float)px + jx), (float)c.width) - 1.0f;
-    float ndcy = 1.0f - __fdiv_rn(2.0f * ((float)py + jy), (float)c.height);
+    float ndcx = lgi_div(2.0f * ((float)px + jx), (float)c.width) - 1.0f;
+    float ndcy = 1.0f - lgi_div(2.0f * ((float)py + jy), (float)c.height);
     float rx = ndcx * c.aspect * c.tan_half_fov;
     float ry = ndcy * c.tan_half_fov;
     V3 fwd = v3(c.fwd[0], c.fwd[1], c.fwd[2]);
@@ -189,7 +189,7 @@ __device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, int* stk) {
         if (bounce >= 1 && rr) {
             float q = fminf(0.95f, fmaxf(T.x, fmaxf(T.y, T.z)));
             if (rng.next() > q) break;
-            T = mul(T, __fdiv_rn(1.0f, q));
+            T = mul(T, lgi_div(1.0f, q));
         }
         V3 nd = sample_cos_hemi(N, rng);
         ray = make_ray(add(Pt, mul(N, 1e-3f)), nd);
@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin,
 __global__ __launch_bounds__(kBlock) void k_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    float v = __fsqrt_rn(fmaxf(0.0f, fminf(sum[i] * inv_spp, 1.0f)));
+    float v = lgi_sqrt(fmaxf(0.0f, fminf(sum[i] * inv_spp, 1.0f)));
     out[i] = (uint8_t)(v * 255.0f);
 }
 

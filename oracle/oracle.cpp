@@ -539,6 +539,20 @@ void orc_pcg32_stream(uint64_t initstate, uint64_t initseq, uint32_t* out, int n
     pcg_seed(g, initstate, initseq);
     for (int i = 0; i < n; ++i) out[i] = pcg_next(g);
 }
+void orc_xorshift_stream(uint32_t seed, uint32_t* out, int n) {
+    uint32_t s = seed;
+    for (int i = 0; i < n; ++i) out[i] = xorshift32(s);
+}
+// n cosine-weighted directions around N from consecutive draws of one PCG32 stream.
+void orc_sample_cos_hemi(const float N[3], uint64_t initstate, uint64_t initseq, float* dirs, int n) {
+    Rng r{};
+    r.pcg = 1;
+    pcg_seed(r.g, initstate, initseq);
+    for (int i = 0; i < n; ++i) {
+        V3 d = sample_cos_hemi(vfrom(N), r);
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+    }
+}
 // First n uniform floats of the (pixel, sample) stream used by the renderer.
 void orc_pixel_randf(uint32_t seed, uint32_t pixel, uint32_t sample, float* out, int n) {
     Rng r;

@@ -53,6 +53,8 @@ def lib():
             "orc_sincos": (None, [V, V, V, C.c_int64]),
             "orc_pcg32_stream": (None, [C.c_uint64, C.c_uint64, V, C.c_int]),
             "orc_pixel_randf": (None, [C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_int]),
+            "orc_xorshift_stream": (None, [C.c_uint32, V, C.c_int]),
+            "orc_sample_cos_hemi": (None, [V, C.c_uint64, C.c_uint64, V, C.c_int]),
             "orc_morton_codes": (None, [V, C.c_int64, V]),
             "orc_bvh_build": (C.c_int64, [V, C.c_int64, V, V, V]),
             "orc_tan_half_fov": (C.c_float, [C.c_float]),
@@ -117,6 +119,19 @@ def sincos(x):
 def pcg32(initstate, initseq, n):
     out = np.zeros(n, np.uint32)
     lib().orc_pcg32_stream(initstate, initseq, _p(out), n)
+    return out
+
+
+def xorshift(seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_xorshift_stream(seed, _p(out), n)
+    return out
+
+
+def sample_cos_hemi(normal, initstate, initseq, n):
+    N = np.asarray(normal, np.float32)
+    out = np.zeros((n, 3), np.float32)
+    lib().orc_sample_cos_hemi(_p(N), initstate, initseq, _p(out), n)
     return out
 
 

@@ -1,0 +1,147 @@
+"""Host-side checks of liblgi.so through the C ABI (no GPU needed): exports, scene.json
+loading and its error codes (src/scene_loader.cpp:26-138), JSON round trip, batch files
+(src/main.cpp:61-91), PNG output, and the procedural scene generator."""
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+import lgi
+from lgi import _abi
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def test_library_exports_every_header_symbol():
+    syms = _abi.header_symbols()
+    assert len(syms) >= 20
+    l = lgi.lib()
+    for s in syms:
+        assert hasattr(l, s), s
+        assert s in _abi.SIGNATURES, f"{s} has no ctypes signature"
+    assert lgi.lib().lgi_version().startswith(b"lgi")
+
+
+def test_struct_layouts():
+    assert C.sizeof(lgi.Camera) == 100
+    assert C.sizeof(_abi.PointLight) == 32
+    assert C.sizeof(_abi.DirLight) == 28
+    assert lgi.TRI_DTYPE.itemsize == 100 and lgi.NODE_DTYPE.itemsize == 40
+
+
+def test_mock_scene_fixture_loads_like_reference():
+    # scenes/mock_scene.json of the reference: 1920x1080, fov 60, 1 directional + 1 point
+    # light, 2 materials, 2 triangles (summary format src/scene_loader.cpp:130-136)
+    ctx = lgi.Context()
+    ctx.load_json(GOLDEN / "mock_scene.json")
+    assert ctx.counts() == (2, 2, 1, 1)
+    s = ctx.scene()
+    cam = s["camera"]
+    assert (cam.width, cam.height, cam.fov_y_deg) == (1920, 1080, 60.0)
+    assert cam.aspect == np.float32(1.777778)
+    assert list(cam.camera_to_world[12:16]) == [0.0, -5.0, 2.0, 1.0]
+    assert s["points"]["radius"][0] == 5.0 and s["points"]["intensity"][0] == np.float32(0.8)
+    assert list(s["sun"].dir) == [1.0, -1.0, -1.0]
+    t = s["tris"]
+    assert t["v"][1].tolist() == [[-2, -2, 0], [2, -2, 0], [0, 2, 5]]
+    assert t["uv"][1][2].tolist() == [0.5, 1.0] and t["material"].tolist() == [0, 1]
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return p
+
+
+def test_load_errors_return_codes(tmp_path):
+    ctx = lgi.Context()
+    good = (GOLDEN / "mock_scene.json").read_text()
+    cases = [
+        (tmp_path / "missing.json", 2),
+        (write(tmp_path, "bad.json", good[:200]), 3),
+        (write(tmp_path, "v2.json", good.replace('"version": 1', '"version": 2')), 4),
+        (write(tmp_path, "nov.json", good.replace('"version": 1,', '')), 4),
+        (write(tmp_path, "nocam.json", good.replace('"camera"', '"kamera"')), 3),
+        (write(tmp_path, "nomat.json", good.replace('"mat":1', '"matt":1')), 3),
+        (write(tmp_path, "noradius.json", good.replace('"radius": 5.0', '"r": 5.0')), 3),
+    ]
+    for path, code in cases:
+        with pytest.raises(lgi.LgiError) as e:
+            ctx.load_json(path)
+        assert e.value.status == code, path.name
+    ctx.load_json(GOLDEN / "mock_scene.json")  # context still usable
+
+
+def test_json_round_trip_bit_exact(tmp_path):
+    ctx = lgi.Context()
+    ctx.generate(1, 3, 320, 160)
+    a = ctx.scene()
+    ctx.save_json(tmp_path / "s.json")
+    ctx2 = lgi.Context()
+    ctx2.load_json(tmp_path / "s.json")
+    b = ctx2.scene()
+    assert a["tris"].tobytes() == b["tris"].tobytes()
+    assert a["points"].tobytes() == b["points"].tobytes()
+    assert np.array_equal(a["materials"], b["materials"])
+    assert bytes(a["camera"]) == bytes(b["camera"]) and bytes(a["sun"]) == bytes(b["sun"])
+
+
+def test_generator_sizes_and_determinism():
+    ctx = lgi.Context()
+    ctx.generate(1, 1, 512, 256)
+    n1, m1, p1, _ = ctx.counts()
+    assert 53000 <= n1 <= 82000 and m1 == 17 and p1 == 2
+    t1 = ctx.scene()["tris"]
+    ctx.generate(1, 1, 512, 256)
+    assert ctx.scene()["tris"].tobytes() == t1.tobytes()
+    ctx.generate(2, 1, 1728, 864)
+    n2, m2, p2, _ = ctx.counts()
+    assert abs(n2 - 1_050_000) < 0.01 * 1_050_000 and m2 == 63 and p2 == 2
+    s = ctx.scene()
+    assert np.all(s["points"]["intensity"] == 50) and s["sun"].intensity == 3.0
+    assert s["sun"].dir[2] < 0  # light travels downwards
+    alb = s["materials"][:, 0, :]
+    assert alb.min() >= 0.1 and alb.max() <= 0.9
+    with pytest.raises(lgi.LgiError):
+        ctx.generate(3, 1, 16, 16)
+
+
+def test_batch_file_parsing(tmp_path):
+    text = ("# out_png spp bounces  camx camy camz  tgtx tgty tgtz\n"
+            "out/a.png 8 2 1 2 3 4 5 6\n"
+            "out/b.png 256 2 -1.5 0 0.25 0 0 0  # trailing comment\n"
+            "broken line 1 2\n"
+            "\n"
+            "out/c.png 4 0 0 0 0 1 0 0 extra tokens ignored\n")
+    jobs = lgi.parse_batch(write(tmp_path, "batch.txt", text))
+    assert [j[0] for j in jobs] == ["out/a.png", "out/b.png", "out/c.png"]
+    assert jobs[0][1:3] == (8, 2) and jobs[1][1] == 256 and jobs[2][2] == 0
+    assert np.allclose(jobs[1][3], (-1.5, 0, 0.25))
+    with pytest.raises(lgi.LgiError):
+        lgi.parse_batch(tmp_path / "nope.txt")
+
+
+def test_reference_batch_recipe(reference_dir):
+    jobs = lgi.parse_batch(reference_dir / "ml" / "data" / "batch.txt")
+    assert len(jobs) == 100
+    assert sorted({j[1] for j in jobs}) == [8, 256]
+    assert len({j[3] for j in jobs}) == 50
+
+
+def test_png_writer_round_trip(tmp_path):
+    from PIL import Image
+    rng = np.random.default_rng(0)
+    img = rng.integers(0, 256, (37, 53, 3), dtype=np.uint8)
+    lgi.write_png(tmp_path / "x.png", img)
+    back = np.asarray(Image.open(tmp_path / "x.png").convert("RGB"))
+    assert np.array_equal(back, img)
+
+
+def test_cli_usage_and_load_failure(tmp_path):
+    cli = pathlib.Path(lgi.__file__).resolve().parent.parent / "bin" / "cuda_ref"
+    r = subprocess.run([str(cli)], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([str(cli), str(tmp_path / "missing.json")], capture_output=True, text=True)
+    assert r.returncode in (2, 4)
