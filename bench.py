@@ -65,7 +65,19 @@ def main():
     ap.add_argument("--cpu-row-step", type=int, default=2)
     ap.add_argument("--cpu-spp", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        import torch
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the tracer would shard by pose or image tile with one accumulator reduce)", file=sys.stderr)
@@ -129,9 +141,17 @@ def main():
                                                                            args.cpu_row_step, threads, args.cpu_spp)
         algo_bytes = bps * samples_per_step
         achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
+        traffic, counters_src = None, None
+        prof = ROOT / "profiles" / "k_render_config2.json"
+        if prof.exists() and args.spp == 8 and args.bounces == 2:
+            p = json.loads(prof.read_text())
+            traffic, counters_src = int(p["traffic_bytes"]), {
+                "file": "profiles/k_render_config2.json", "kernel_avg_ms": round(p["avg_ns"] / 1e6, 3),
+                "l2_hit_rate": round(p["l2_hit_rate"], 4)}
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                              "bytes_per_sample": round(bps, 1)}
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                              "algorithmic_bytes": int(algo_bytes), "bytes_per_sample": round(bps, 1),
+                              "counters": counters_src}
         result["cpu_baseline"] = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads,
                                   "kind": "port",
                                   "sample": "CPU oracle, same scene and PCG32 streams, %d spp on every %dth row "

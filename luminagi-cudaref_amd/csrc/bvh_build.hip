@@ -28,6 +28,7 @@ constexpr int kBlock = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;   // keys per radix-sort block
 constexpr int kDepthShift = 26;          // BFS key = depth << 26 | first
+constexpr int kMaxLevels = 32;           // reference depth <= 30 (one split per Morton bit) + leaf
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
@@ -191,17 +192,28 @@ __global__ __launch_bounds__(kBlock) void k_bbox_partial(const lgi_triangle* tri
 }
 
 // out = sceneMin[3], ext[3] with ext <= 0 replaced by 1 (src/bvh.cpp:175-178)
-__global__ void k_bbox_final(const float* partial, int nb, float* out) {
-    if (threadIdx.x != 0) return;
-    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nb; ++b)
+__global__ __launch_bounds__(kBlock) void k_bbox_final(const float* partial, int nb, float* out) {
+    __shared__ float s[6][kBlock];
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = threadIdx.x; b < nb; b += kBlock)
         for (int k = 0; k < 3; ++k) {
-            mn[k] = fminf(mn[k], partial[b * 6 + k]);
-            mx[k] = fmaxf(mx[k], partial[b * 6 + 3 + k]);
+            v[k] = fminf(v[k], partial[b * 6 + k]);
+            v[3 + k] = fmaxf(v[3 + k], partial[b * 6 + 3 + k]);
         }
-    for (int k = 0; k < 3; ++k) {
-        float e = mx[k] - mn[k];
-        out[k] = mn[k];
+    for (int k = 0; k < 6; ++k) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int k = 0; k < 3; ++k) {
+                s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + o]);
+                s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        const float e = s[3 + k][0] - s[k][0];
+        out[k] = s[k][0];
         out[3 + k] = (e <= 0.0f) ? 1.0f : e;
     }
 }
@@ -326,13 +338,11 @@ __device__ __forceinline__ void fit3(float* mn, float* mx, const float* p) {
     }
 }
 
-// Leaf boxes, then walk up: the second child to arrive at a parent builds its box.
-__global__ __launch_bounds__(kBlock) void k_leaves_refit(int n, int m, const int32_t* bfs_ids,
-                                                         const int32_t* new_index, const int32_t* parent,
-                                                         const int32_t* child, const int32_t* rfirst,
-                                                         const int32_t* rlast, const uint8_t* splittable,
-                                                         const float* sv, uint32_t* arrivals,
-                                                         lgi_bvh_node* nodes) {
+// Leaf boxes from the triangle vertices in leaf order (src/bvh.cpp:96-103).
+__global__ __launch_bounds__(kBlock) void k_leaf_boxes(int n, int m, const int32_t* bfs_ids,
+                                                       const int32_t* rfirst, const int32_t* rlast,
+                                                       const uint8_t* splittable, const float* sv,
+                                                       lgi_bvh_node* nodes) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
     const int x = bfs_ids[p];
@@ -347,28 +357,40 @@ __global__ __launch_bounds__(kBlock) void k_leaves_refit(int n, int m, const int
     for (int k = 0; k < 3; ++k) { nd.bb_min[k] = mn[k]; nd.bb_max[k] = mx[k]; }
     nd.left = -1; nd.right = -1; nd.first_tri = first; nd.tri_count = last - first + 1;
     nodes[p] = nd;
-    int y = x;
-    for (;;) {
-        const int py = parent[y];
-        if (py < 0) break;
-        __threadfence();
-        if (atomicAdd(&arrivals[py], 1u) == 0u) break;
-        __threadfence();
-        const int cl = child[2 * py], cr = child[2 * py + 1];
-        const int il = new_index[cl], ir = new_index[cr];
-        const lgi_bvh_node L = nodes[il];
-        const lgi_bvh_node R = nodes[ir];
-        lgi_bvh_node o;
-        for (int k = 0; k < 3; ++k) {
-            o.bb_min[k] = L.bb_min[k];
-            o.bb_max[k] = L.bb_max[k];
-            if (R.bb_min[k] < o.bb_min[k]) o.bb_min[k] = R.bb_min[k];
-            if (R.bb_max[k] > o.bb_max[k]) o.bb_max[k] = R.bb_max[k];
-        }
-        o.left = il; o.right = ir; o.first_tri = -1; o.tri_count = 0;
-        nodes[new_index[py]] = o;
-        y = py;
+}
+
+// Internal boxes of one BFS level [p0, p1): left box, then strict union with the right box
+// (src/bvh.cpp:115-117). Levels run deepest first, so both children are final. BFS order
+// stores each level contiguously, which replaces per-node arrival atomics and the
+// agent-scope release/acquire pairs they need across the eight XCD L2 caches.
+__global__ __launch_bounds__(kBlock) void k_level_boxes(int n, int p0, int p1, const int32_t* bfs_ids,
+                                                        const int32_t* new_index, const int32_t* child,
+                                                        const uint8_t* splittable, lgi_bvh_node* nodes) {
+    const int p = p0 + blockIdx.x * kBlock + threadIdx.x;
+    if (p >= p1) return;
+    const int x = bfs_ids[p];
+    if (x >= n - 1 || !splittable[x]) return;
+    const int il = new_index[child[2 * x]], ir = new_index[child[2 * x + 1]];
+    const lgi_bvh_node L = nodes[il];
+    const lgi_bvh_node R = nodes[ir];
+    lgi_bvh_node o;
+    for (int k = 0; k < 3; ++k) {
+        o.bb_min[k] = L.bb_min[k];
+        o.bb_max[k] = L.bb_max[k];
+        if (R.bb_min[k] < o.bb_min[k]) o.bb_min[k] = R.bb_min[k];
+        if (R.bb_max[k] > o.bb_max[k]) o.bb_max[k] = R.bb_max[k];
     }
+    o.left = il; o.right = ir; o.first_tri = -1; o.tri_count = 0;
+    nodes[p] = o;
+}
+
+// level_start[d] = first BFS position of depth d; level_start[max_depth + 1] = m.
+__global__ __launch_bounds__(kBlock) void k_level_starts(const uint32_t* bfs_keys, int m, int32_t* level_start) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int d = (int)(bfs_keys[p] >> kDepthShift);
+    if (p == 0 || (int)(bfs_keys[p - 1] >> kDepthShift) != d) level_start[d] = p;
+    if (p == m - 1) level_start[d + 1] = m;
 }
 
 // ------------------------------------------------------------------ traversal layout
@@ -511,18 +533,19 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
     int32_t* rlast = (int32_t*)S(13, (size_t)n * 4);
     uint8_t* splittable = (uint8_t*)S(14, (size_t)n + 64);
     int32_t* new_index = (int32_t*)S(15, (size_t)2 * n * 4);
-    uint32_t* arrivals = (uint32_t*)S(16, (size_t)n * 4);
+    int32_t* level_start = (int32_t*)S(16, (kMaxLevels + 2) * 4);
     uint32_t* tflags = (uint32_t*)S(17, m_cap * 4 + 64);
     uint8_t* leaf_end = (uint8_t*)S(18, (size_t)n + 64);
     uint32_t* tsums = (uint32_t*)S(19, (size_t)(ceil_div(m_cap, kTile) + 16) * 4);
-    for (void* p : {(void*)partial, (void*)sb.k[0], (void*)sv, (void*)child, (void*)new_index, (void*)arrivals,
+    for (void* p : {(void*)partial, (void*)sb.k[0], (void*)sv, (void*)child, (void*)new_index, (void*)level_start,
                     (void*)tflags, (void*)leaf_end, (void*)tsums})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "scratch allocation failed");
     (void)nb_sort;
 
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    LGI_HIP(ctx, hipMemsetAsync(level_start, 0xFF, (kMaxLevels + 2) * 4, st));
     hipLaunchKernelGGL(k_bbox_partial, dim3(npart), dim3(kBlock), 0, st, d.tri_in, n, partial);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, st, partial, npart, bb);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(kBlock), 0, st, partial, npart, bb);
     hipLaunchKernelGGL(k_morton, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, n, bb, sb.k[0], sb.v[0]);
     radix_sort(st, sb, n, 32);
     LGI_HIP(ctx, hipMemcpyAsync(d.keys, sb.k[0], (size_t)n * 4, hipMemcpyDeviceToDevice, st));
@@ -531,10 +554,10 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
 
     int64_t m = 1;
     if (n == 1) {
-        // single triangle: one leaf; reuse the refit kernel with a trivial tree
+        // single triangle: one leaf
         hipLaunchKernelGGL(k_init_single, dim3(1), dim3(1), 0, st, parent, sb.v[0], new_index);
-        hipLaunchKernelGGL(k_leaves_refit, dim3(1), dim3(kBlock), 0, st, n, 1, sb.v[0], new_index, parent, child,
-                           rfirst, rlast, splittable, sv, arrivals, d.nodes);
+        hipLaunchKernelGGL(k_leaf_boxes, dim3(1), dim3(kBlock), 0, st, n, 1, sb.v[0], rfirst, rlast, splittable, sv,
+                           d.nodes);
     } else {
         hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, kBlock)), dim3(kBlock), 0, st, d.keys, n, child, parent,
                            rfirst, rlast, splittable);
@@ -547,9 +570,21 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
         m = hm;
         radix_sort(st, sb, (int)m, 32);
         hipLaunchKernelGGL(k_new_index, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, sb.v[0], (int)m, new_index);
-        LGI_HIP(ctx, hipMemsetAsync(arrivals, 0, (size_t)n * 4, st));
-        hipLaunchKernelGGL(k_leaves_refit, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, n, (int)m, sb.v[0],
-                           new_index, parent, child, rfirst, rlast, splittable, sv, arrivals, d.nodes);
+        hipLaunchKernelGGL(k_level_starts, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, sb.k[0], (int)m,
+                           level_start);
+        hipLaunchKernelGGL(k_leaf_boxes, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, n, (int)m, sb.v[0], rfirst,
+                           rlast, splittable, sv, d.nodes);
+        int32_t hls[kMaxLevels + 2];
+        LGI_HIP(ctx, hipMemcpyAsync(hls, level_start, sizeof(hls), hipMemcpyDeviceToHost, st));
+        LGI_HIP(ctx, hipStreamSynchronize(st));
+        int max_depth = 0;
+        while (max_depth + 1 <= kMaxLevels && hls[max_depth + 1] != (int)m) ++max_depth;
+        if (max_depth >= kMaxLevels) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "BVH deeper than expected");
+        for (int lv = max_depth - 1; lv >= 0; --lv) {
+            const int p0 = hls[lv], p1 = hls[lv + 1];
+            hipLaunchKernelGGL(k_level_boxes, dim3(ceil_div(p1 - p0, kBlock)), dim3(kBlock), 0, st, n, p0, p1,
+                               sb.v[0], new_index, child, splittable, d.nodes);
+        }
     }
     // traversal layout
     LGI_HIP(ctx, hipMemsetAsync(leaf_end, 0, (size_t)n, st));

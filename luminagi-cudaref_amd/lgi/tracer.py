@@ -51,11 +51,16 @@ class Context:
         self._h = h
 
     def close(self):
-        if self._h:
-            lib().lgi_destroy(self._h)
-            self._h = None
+        h, self._h = getattr(self, "_h", None), None
+        l = _abi._lib
+        if h and l is not None:
+            l.lgi_destroy(h)
 
-    __del__ = close
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: modules may already be torn down
+            pass
 
     def __enter__(self):
         return self
