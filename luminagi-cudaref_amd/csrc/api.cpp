@@ -273,7 +273,10 @@ lgi_status lgi_bvh_build(lgi_ctx* ctx) {
     lgi_status st = upload_scene(ctx);
     if (st != LGI_OK) return st;
     LGI_HIP(ctx, hipSetDevice(ctx->device));
-    return lgi_gpu_build_bvh(ctx);
+    st = lgi_gpu_build_bvh(ctx);
+    if (st != LGI_OK) return st;
+    LGI_HIP(ctx, hipMemcpy(&ctx->root, ctx->dev.nodes, sizeof(lgi_bvh_node), hipMemcpyDeviceToHost));
+    return LGI_OK;
 }
 
 lgi_status lgi_bvh_node_count(const lgi_ctx* ctx, int64_t* n_nodes) {
@@ -307,12 +310,20 @@ lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t
     if (st != LGI_OK) return st;
     if (spp < 0 || bounces < 0 || bounces > 64 || sample0 < 0 || !d_sum || !d_comp)
         return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "bad render arguments");
-    if (flags & LGI_RENDER_SORTED) return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "sorted renderer not built");
+    if ((flags & LGI_RENDER_SORTED) && (flags & LGI_RNG_XORSHIFT_REF))
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED,
+                           "the sorted renderer needs per-sample RNG streams (PCG32); xorshift-ref chains "
+                           "samples within an 8-sample chunk");
     if (spp == 0) return LGI_OK;
     TraceParams P = make_params(ctx, cam, spp, bounces, flags, sample0);
     P.sum = d_sum;
     P.comp = d_comp;
     P.sumsq = d_sumsq;
+    if (flags & LGI_RENDER_SORTED) {
+        const lgi_bvh_node& r = ctx->root;
+        return lgi_render_wavefront(ctx, P, make_float3(r.bb_min[0], r.bb_min[1], r.bb_min[2]),
+                                    make_float3(r.bb_max[0], r.bb_max[1], r.bb_max[2]));
+    }
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     lgi_launch_render(P, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());

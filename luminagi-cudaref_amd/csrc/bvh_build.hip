@@ -489,6 +489,26 @@ lgi_status dev_alloc(lgi_ctx* ctx, T** p, size_t count) {
 
 }  // namespace
 
+size_t lgi_sort_scratch_bytes(int64_t n) {
+    const int nb = ceil_div(n, kTile);
+    const size_t ncount = (size_t)256 * nb;
+    return 4 * (4 * (size_t)n + ncount + ceil_div(ncount, kTile) + 64);
+}
+
+void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch) {
+    SortBufs b;
+    char* p = (char*)scratch;
+    const int nb = ceil_div(n, kTile);
+    const size_t ncount = (size_t)256 * nb;
+    b.k[0] = keys;
+    b.v[0] = vals;
+    b.k[1] = (uint32_t*)p; p += 4 * (size_t)n;
+    b.v[1] = (int32_t*)p; p += 4 * (size_t)n;
+    b.counts = (uint32_t*)p; p += 4 * ncount;
+    b.sums = (uint32_t*)p;
+    radix_sort(st, b, n, bits);
+}
+
 void lgi_free_dev_scene(DevScene& d) {
     void* ptrs[] = {d.tri_in, d.nodes, d.tnodes, d.triv, d.tris, d.mat_albedo, d.points, d.perm, d.keys};
     for (void* p : ptrs)

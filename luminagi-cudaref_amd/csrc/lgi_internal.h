@@ -80,6 +80,7 @@ struct lgi_ctx {
     bool gpu_ready = false;      // device selected, events created
     bool uploaded = false;       // dev mirrors `scene`
     DevScene dev;
+    lgi_bvh_node root{};         // host copy of the BVH root (scene box)
     bool bvh_ready = false;
     float last_build_ms = 0.f;
     float last_render_ms = 0.f;
@@ -102,6 +103,10 @@ lgi_status lgi_set_err(lgi_ctx* ctx, lgi_status st, const std::string& msg);
 
 // Kernel-side entry points implemented in the .hip files.
 lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx);
+// Stable LSD radix sort of (keys, vals) on `bits` (multiple of 16) low key bits, in place;
+// scratch must hold lgi_sort_scratch_bytes(n).
+size_t lgi_sort_scratch_bytes(int64_t n);
+void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch);
 void lgi_free_dev_scene(DevScene& d);
 DevCamera lgi_make_dev_camera(const lgi_camera& c);
 lgi_status lgi_upload_lights(lgi_ctx* ctx, DevLights* out);

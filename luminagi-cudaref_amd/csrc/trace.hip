@@ -1,4 +1,5 @@
-// trace.hip — path-tracing kernels for gfx950 (wave64).
+// trace.hip — inline ("megakernel") path tracer for gfx950 (wave64), plus the primary-hit,
+// G-buffer and tonemap kernels.
 //
 // Semantics follow the reference renderer (src/pathtracer.cu:61-429): Moller-Trumbore with
 // the (1e-6, bestT) acceptance window, slab test with the `lo > tmax` cull, sun + point-light
@@ -13,9 +14,7 @@
 // per wave, 4 waves per block. The traversal stack lives in LDS as [depth][thread], so lanes
 // at different depths still hit distinct banks; the tree depth is at most 30 internal levels
 // (one per Morton bit), so 32 entries never overflow.
-#include "lgi_internal.h"
-#include "lgi_math.h"
-#include "trace_kernels.h"
+#include "trace_device.h"
 
 using namespace lgi;
 
@@ -23,54 +22,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kStack = 32;
-
-__device__ __forceinline__ bool slab(const Ray& r, float lox, float hix, float loy, float hiy, float loz,
-                                     float hiz, float tmax, float& tenter) {
-    // src/pathtracer.cu:61-80, same expression tree
-    float t0 = (lox - r.o.x) * r.inv.x;
-    float t1 = (hix - r.o.x) * r.inv.x;
-    float lo = fminf(t0, t1), hi = fmaxf(t0, t1);
-    t0 = (loy - r.o.y) * r.inv.y;
-    t1 = (hiy - r.o.y) * r.inv.y;
-    lo = fmaxf(lo, fminf(t0, t1));
-    hi = fminf(hi, fmaxf(t0, t1));
-    t0 = (loz - r.o.z) * r.inv.z;
-    t1 = (hiz - r.o.z) * r.inv.z;
-    lo = fmaxf(lo, fminf(t0, t1));
-    hi = fminf(hi, fmaxf(t0, t1));
-    tenter = fmaxf(lo, 0.0f);
-    return !(hi < tenter || lo > tmax);
-}
-
-// Moller-Trumbore on the precomputed edges (src/pathtracer.cu:82-107).
-__device__ __forceinline__ bool tri_hit(const Ray& r, const TriV& t, float tmax, float& outT, float& outU,
-                                        float& outV) {
-    const float kEps = 1e-6f;
-    V3 v0 = v3(t.v0.x, t.v0.y, t.v0.z);
-    V3 e1 = v3(t.e1.x, t.e1.y, t.e1.z);
-    V3 e2 = v3(t.e2.x, t.e2.y, t.e2.z);
-    V3 p = cross(r.d, e2);
-    float det = dot(e1, p);
-    if (det > -kEps && det < kEps) return false;
-    float invDet = lgi_div(1.0f, det);
-    V3 s = sub(r.o, v0);
-    float u = dot(s, p) * invDet;
-    if (u < 0.0f || u > 1.0f) return false;
-    V3 q = cross(s, e1);
-    float v = dot(r.d, q) * invDet;
-    if (v < 0.0f || u + v > 1.0f) return false;
-    float tt = dot(e2, q) * invDet;
-    if (tt <= kEps || tt >= tmax) return false;
-    outT = tt;
-    outU = u;
-    outV = v;
-    return true;
-}
-
-struct Hit {
-    int tri;   // leaf-order index, -1 = miss
-    float t, u, v;
-};
 
 // Closest hit (kAny = false) or any hit within (1e-6, tmax) (kAny = true).
 template <bool kAny>
@@ -114,61 +65,19 @@ __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, int* stk
     return h;
 }
 
-// Shading normal at the closest hit (src/pathtracer.cu:149-155).
-__device__ __forceinline__ V3 hit_normal(const TraceParams& P, const Hit& h, int& mat) {
-    const TriS s = P.tris[h.tri];
-    float w = 1.0f - h.u - h.v;
-    mat = s.mat;
-    return normalize(v3(w * s.n[0] + h.u * s.n[3] + h.v * s.n[6], w * s.n[1] + h.u * s.n[4] + h.v * s.n[7],
-                        w * s.n[2] + h.u * s.n[5] + h.v * s.n[8]));
-}
-
-// Sun + point-light NEE (src/pathtracer.cu:230-272). A point light farther than its radius
-// contributes albedo*color*0 = +0, so its shadow ray is skipped: bit-identical result.
+// shade_direct (src/pathtracer.cu:230-272): lights in order sun, points; the colour is
+// accumulated only for unoccluded lights.
 __device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* stk) {
-    const float kShadowBias = 1e-3f;
     V3 color = v3(0.f, 0.f, 0.f);
-    {
-        V3 toLight = normalize(mul(v3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]), -1.0f));
-        float ndotl = fmaxf(0.0f, dot(N, toLight));
-        if (ndotl > 0.0f) {
-            Ray shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
-            if (traverse<true>(P, shadow, 1e30f, stk).tri < 0) {
-                float I = P.sun_intensity * ndotl;
-                color = add(color, had(albedo, mul(v3(P.sun_color[0], P.sun_color[1], P.sun_color[2]), I)));
-            }
-        }
-    }
-    for (int i = 0; i < P.n_points; ++i) {
-        const lgi_point_light L = P.points[i];
-        V3 toLightVec = sub(v3(L.pos[0], L.pos[1], L.pos[2]), Pt);
-        float dist = lgi_sqrt(dot(toLightVec, toLightVec));
-        V3 toLight = mul(toLightVec, lgi_div(1.0f, dist));
-        float ndotl = fmaxf(0.0f, dot(N, toLight));
-        if (ndotl <= 0.0f || dist >= L.radius * 3.0f) continue;
-        if (dist >= L.radius) continue;  // atten == 0 exactly
-        Ray shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
-        if (traverse<true>(P, shadow, dist - kShadowBias, stk).tri >= 0) continue;
-        float r = L.radius;
-        float atten = fmaxf(0.0f, 1.0f - lgi_div(dist, r));
-        atten = atten * atten;
-        float I = L.intensity * ndotl * atten;
-        color = add(color, had(albedo, mul(v3(L.color[0], L.color[1], L.color[2]), I)));
+    for (int li = 0; li <= P.n_points; ++li) {
+        Ray shadow;
+        float tmax;
+        V3 contrib;
+        if (!nee_candidate(P, li, Pt, N, albedo, shadow, tmax, contrib)) continue;
+        if (traverse<true>(P, shadow, tmax, stk).tri >= 0) continue;
+        color = add(color, contrib);
     }
     return color;
-}
-
-// Primary ray through (px + jx, py + jy) (src/pathtracer.cu:284-293).
-__device__ __forceinline__ Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
-    float ndcx = lgi_div(2.0f * ((float)px + jx), (float)c.width) - 1.0f;
-    float ndcy = 1.0f - lgi_div(2.0f * ((float)py + jy), (float)c.height);
-    float rx = ndcx * c.aspect * c.tan_half_fov;
-    float ry = ndcy * c.tan_half_fov;
-    V3 fwd = v3(c.fwd[0], c.fwd[1], c.fwd[2]);
-    V3 left = v3(c.left[0], c.left[1], c.left[2]);
-    V3 up = v3(c.up[0], c.up[1], c.up[2]);
-    V3 dw = add(add(mul(fwd, 1.0f), mul(left, -rx)), mul(up, ry));
-    return make_ray(v3(c.pos[0], c.pos[1], c.pos[2]), dw);
 }
 
 // trace_path (src/pathtracer.cu:295-331).
@@ -180,7 +89,7 @@ __device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, int* stk) {
         Hit h = traverse<false>(P, ray, 1e30f, stk);
         if (h.tri < 0) break;
         int mat;
-        V3 N = hit_normal(P, h, mat);
+        V3 N = hit_normal(P.tris, h, mat);
         if (dot(N, ray.d) > 0.0f) N = mul(N, -1.0f);
         V3 Pt = add(ray.o, mul(ray.d, h.t));
         V3 albedo = v3(P.mat_albedo[3 * mat], P.mat_albedo[3 * mat + 1], P.mat_albedo[3 * mat + 2]);
@@ -275,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin,
         return;
     }
     int mat;
-    V3 N = hit_normal(P, h, mat);
+    V3 N = hit_normal(P.tris, h, mat);
     if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
     auto enc = [](float v) { return (uint8_t)(fminf(1.0f, fmaxf(0.0f, v)) * 255.0f); };
     alb[i] = enc(P.mat_albedo[3 * mat]);
@@ -290,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin,
     wp[i + 2] = enc((Pt.z - bbMin.z) * invExt.z);
 }
 
-// tonemap_kernel (src/pathtracer.cu:413-429), 4 channels-of-bytes per thread.
+// tonemap_kernel (src/pathtracer.cu:413-429), one byte per thread.
 __global__ __launch_bounds__(kBlock) void k_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;

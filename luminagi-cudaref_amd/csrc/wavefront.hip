@@ -1,0 +1,454 @@
+// wavefront.hip — sorted ("wavefront") path tracer for gfx950.
+//
+// The reference's --sort path (src/pathtracer.cu:431-547, 761-909) splits a sample into a
+// primary-ray kernel, a trace-and-bounce kernel per bounce and a Thrust sort of the
+// surviving rays by a direction key. This renderer keeps that structure and its purpose
+// (coherent secondary rays) but is organised for MI355X:
+//   * all spp samples of a frame are in flight at once (one path per (pixel, sample): up to
+//     tens of millions of paths, a few GB of queues in HBM);
+//   * closest-hit and shadow rays are traced by one persistent kernel that pulls 64-ray
+//     chunks from a queue with one atomic per chunk and refills each lane the moment its ray
+//     finishes, so no lane idles while its wave still has work;
+//   * secondary rays are sorted between bounces by a 30-bit key (18-bit origin Morton code in
+//     the scene box, 12-bit direction Morton code) with the project's LSD radix sort;
+//   * results are written per path, never with atomics: each path keeps its own radiance, and
+//     samples are added to the pixel in sample order with Kahan compensation at the end.
+// Unlike the reference sorted path, Russian roulette and the per-path RNG stream are kept,
+// so the image is bit-identical to the inline megakernel (same PCG32 stream per
+// (pixel, sample), same expression trees, same light order).
+#include "trace_device.h"
+
+using namespace lgi;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kStack = 32;
+
+struct alignas(16) QRay {
+    float4 o;   // origin, w = path id (int bits)
+    float4 d;   // direction, w = tmax (shadow rays) / unused
+};
+
+struct WfParams {
+    TraceParams P;
+    int32_t npx;          // W*H
+    int32_t nl;           // lights: sun + points
+    float3 bb_min;        // scene box for the sort key
+    float3 bb_inv;        // 1 / extent
+    uint64_t* rng;        // PCG32 state per path
+    float* T;             // throughput per path (3)
+    float* Tb;            // throughput before the bounce update (3)
+    float* L;             // radiance per path (3)
+    float* contrib;       // per (path, light): contribution if unoccluded (3)
+    uint8_t* nee;         // per (path, light): 0 none, 1 candidate, 2 occluded
+};
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t q01(float f, float scale, uint32_t maxv) {
+    f = f * scale;
+    f = fminf(fmaxf(f, 0.0f), (float)maxv);
+    return (uint32_t)f;
+}
+// 30-bit sort key: origin Morton code (6 bits per axis) above direction Morton code (4 bits).
+__device__ __forceinline__ uint32_t ray_key(const WfParams& W, V3 o, V3 d) {
+    uint32_t ox = q01((o.x - W.bb_min.x) * W.bb_inv.x, 64.0f, 63), oy = q01((o.y - W.bb_min.y) * W.bb_inv.y, 64.0f, 63),
+             oz = q01((o.z - W.bb_min.z) * W.bb_inv.z, 64.0f, 63);
+    uint32_t dx = q01(d.x * 0.5f + 0.5f, 16.0f, 15), dy = q01(d.y * 0.5f + 0.5f, 16.0f, 15),
+             dz = q01(d.z * 0.5f + 0.5f, 16.0f, 15);
+    return (((spread3(ox) << 2) | (spread3(oy) << 1) | spread3(oz)) << 12) |
+           ((spread3(dx) << 2) | (spread3(dy) << 1) | spread3(dz));
+}
+
+// Wave-aggregated append: one atomic per wave; returns this lane's slot (or -1).
+__device__ __forceinline__ int wave_append(bool want, uint32_t* counter) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!want) return -1;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return (int)(base + (uint32_t)__popcll(m & lt));
+}
+
+// ------------------------------------------------------------------ primary rays
+// Queue index i -> (sample s, 8x8 pixel tile, lane) so each wave starts on one tile.
+__global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int tiles_x, int padded, QRay* q) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)S * padded) return;
+    const int s = (int)(i / padded), j = (int)(i % padded);
+    const int tile = j >> 6, lane = j & 63;
+    const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
+    const DevCamera& c = W.P.cam;
+    QRay r;
+    if (px >= c.width || py >= c.height) {
+        r.o = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        r.d = make_float4(0.f, 0.f, 1.f, 0.f);
+        q[i] = r;
+        return;
+    }
+    const uint32_t pix = (uint32_t)py * c.width + px;
+    const int64_t p = (int64_t)s * W.npx + pix;
+    Rng rng;
+    rng.pcg = true;
+    pcg_for_sample(rng.g, W.P.seed, pix, (uint32_t)(W.P.sample0 + s));
+    const float jx = rng.next();
+    const float jy = rng.next();
+    Ray ray = primary_ray(c, px, py, jx, jy);
+    W.rng[p] = rng.g.state;
+    W.T[3 * p] = 1.f; W.T[3 * p + 1] = 1.f; W.T[3 * p + 2] = 1.f;
+    W.L[3 * p] = 0.f; W.L[3 * p + 1] = 0.f; W.L[3 * p + 2] = 0.f;
+    r.o = make_float4(ray.o.x, ray.o.y, ray.o.z, __int_as_float((int)p));
+    r.d = make_float4(ray.d.x, ray.d.y, ray.d.z, 1e30f);
+    q[i] = r;
+}
+
+// ------------------------------------------------------------------ persistent traversal
+// Closest hit (hits[i] = t, u, v, tri) or any hit (nee[slot] = 2 when occluded) for queue
+// entries [0, n). Lanes whose ray finishes take the next ray from the wave's 64-entry chunk;
+// a wave grabs a new chunk with one atomicAdd when the current one runs out.
+template <bool kAny>
+__global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, const TriV* __restrict__ triv,
+                                                     const QRay* __restrict__ q, int n, float4* hits, uint8_t* nee,
+                                                     uint32_t* counter) {
+    __shared__ int s_stack[kStack * kBlock];
+    int* stk = s_stack + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // wave-uniform chunk cursor
+    uint32_t cbase = 0, cend = 0;
+    bool exhausted = false;
+    bool active = false;
+    int qi = 0, node = 0, sp = 0, best_tri = -1, slot = 0;
+    float best_t = 0.f, best_u = 0.f, best_v = 0.f;
+    Ray r;
+    for (;;) {
+        const uint64_t need = __ballot(!active);
+        if (need && !exhausted) {
+            const uint32_t cnt = (uint32_t)__popcll(need);
+            const uint32_t rank = (uint32_t)__popcll(need & lt);
+            uint32_t avail = cend - cbase;
+            uint32_t nb = 0xFFFFFFFFu, nend = 0;
+            if (avail < cnt) {
+                if (lane == 0) nb = atomicAdd(counter, 64u);
+                nb = __shfl(nb, 0, 64);
+                if (nb >= (uint32_t)n) {
+                    exhausted = true;
+                    nb = 0xFFFFFFFFu;
+                } else {
+                    nend = min(nb + 64u, (uint32_t)n);
+                }
+            }
+            if (!active) {
+                uint32_t idx = 0xFFFFFFFFu;
+                if (rank < avail) idx = cbase + rank;
+                else if (nb != 0xFFFFFFFFu && rank - avail < nend - nb) idx = nb + (rank - avail);
+                if (idx != 0xFFFFFFFFu) {
+                    const QRay e = q[idx];
+                    qi = (int)idx;
+                    slot = __float_as_int(e.o.w);
+                    const float tmax = e.d.w;
+                    if (slot >= 0) {
+                        r.o = v3(e.o.x, e.o.y, e.o.z);
+                        r.d = v3(e.d.x, e.d.y, e.d.z);
+                        r.inv = v3(lgi_div(1.0f, r.d.x), lgi_div(1.0f, r.d.y), lgi_div(1.0f, r.d.z));
+                        best_t = tmax;
+                        best_tri = -1;
+                        node = 0;
+                        sp = 0;
+                        active = true;
+                    } else if (!kAny) {
+                        hits[idx] = make_float4(-1.f, 0.f, 0.f, __int_as_float(-1));
+                    }
+                }
+            }
+            // advance the cursor past what was consumed
+            const uint32_t from_old = min(cnt, avail);
+            if (nb != 0xFFFFFFFFu) {
+                const uint32_t from_new = min(cnt - from_old, nend - nb);
+                cbase = nb + from_new;
+                cend = nend;
+            } else {
+                cbase += from_old;
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (!active) continue;
+        // one traversal step
+        bool done = false;
+        if (node >= 0) {
+            const TNode nd = tnodes[node];
+            float ta, tb;
+            const bool ha = slab(r, nd.b0xy.x, nd.b0xy.y, nd.b0xy.z, nd.b0xy.w, nd.bz.x, nd.bz.y, best_t, ta);
+            const bool hb = slab(r, nd.b1xy.x, nd.b1xy.y, nd.b1xy.z, nd.b1xy.w, nd.bz.z, nd.bz.w, best_t, tb);
+            if (ha && hb) {
+                int nearc = nd.ch.x, farc = nd.ch.y;
+                if (tb < ta) { nearc = nd.ch.y; farc = nd.ch.x; }
+                stk[sp * kBlock] = farc;
+                ++sp;
+                node = nearc;
+            } else if (ha) {
+                node = nd.ch.x;
+            } else if (hb) {
+                node = nd.ch.y;
+            } else if (sp == 0) {
+                done = true;
+            } else {
+                --sp;
+                node = stk[sp * kBlock];
+            }
+        } else {
+            int ti = ~node;
+            for (;;) {
+                const TriV t = triv[ti];
+                float tt, uu, vv;
+                if (tri_hit(r, t, best_t, tt, uu, vv)) {
+                    best_tri = ti; best_t = tt; best_u = uu; best_v = vv;
+                    if (kAny) break;
+                }
+                if (__float_as_int(t.v0.w) != 0) break;
+                ++ti;
+            }
+            if ((kAny && best_tri >= 0) || sp == 0) {
+                done = true;
+            } else {
+                --sp;
+                node = stk[sp * kBlock];
+            }
+        }
+        if (done) {
+            if (kAny) {
+                if (best_tri >= 0) nee[slot] = 2;
+            } else {
+                hits[qi] = make_float4(best_t, best_u, best_v, __int_as_float(best_tri));
+            }
+            active = false;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ shading
+// For each traced path: NEE candidates -> shadow queue, Russian roulette + cosine bounce ->
+// next queue with its sort key (src/pathtracer.cu:295-331, same order of RNG draws).
+__global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n, int bounce,
+                                                     QRay* sq, uint32_t* sq_count, QRay* nq, uint32_t* nkeys,
+                                                     int32_t* nidx, uint32_t* nq_count) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    int p = -1;
+    Hit h{-1, 0.f, 0.f, 0.f};
+    QRay e;
+    if (i < n) {
+        e = q[i];
+        p = __float_as_int(e.o.w);
+        if (p >= 0) {
+            const float4 hv = hits[i];
+            h.tri = __float_as_int(hv.w);
+            h.t = hv.x; h.u = hv.y; h.v = hv.z;
+        }
+    }
+    const bool live = p >= 0 && h.tri >= 0;
+    V3 N = v3(0.f, 0.f, 1.f), Pt = v3(0.f, 0.f, 0.f), albedo = v3(0.f, 0.f, 0.f);
+    if (live) {
+        const V3 d = v3(e.d.x, e.d.y, e.d.z);
+        int mat;
+        N = hit_normal(W.P.tris, h, mat);
+        if (dot(N, d) > 0.0f) N = mul(N, -1.0f);
+        Pt = add(v3(e.o.x, e.o.y, e.o.z), mul(d, h.t));
+        albedo = v3(W.P.mat_albedo[3 * mat], W.P.mat_albedo[3 * mat + 1], W.P.mat_albedo[3 * mat + 2]);
+        const float* T = W.T + 3 * (int64_t)p;
+        float* Tb = W.Tb + 3 * (int64_t)p;
+        Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
+    }
+    // NEE candidates, one wave-aggregated append per light
+    for (int li = 0; li < W.nl; ++li) {
+        Ray shadow;
+        float tmax = 0.f;
+        V3 c = v3(0.f, 0.f, 0.f);
+        const bool cand = live && nee_candidate(W.P, li, Pt, N, albedo, shadow, tmax, c);
+        if (live) {
+            const int64_t k = (int64_t)p * W.nl + li;
+            W.nee[k] = cand ? 1 : 0;
+            if (cand) { W.contrib[3 * k] = c.x; W.contrib[3 * k + 1] = c.y; W.contrib[3 * k + 2] = c.z; }
+        }
+        const int s = wave_append(cand, sq_count);
+        if (s >= 0) {
+            QRay sr;
+            sr.o = make_float4(shadow.o.x, shadow.o.y, shadow.o.z, __int_as_float((int)((int64_t)p * W.nl + li)));
+            sr.d = make_float4(shadow.d.x, shadow.d.y, shadow.d.z, tmax);
+            sq[s] = sr;
+        }
+    }
+    // continuation
+    bool cont = live && bounce < W.P.bounces;
+    Ray next;
+    if (cont) {
+        float* T = W.T + 3 * (int64_t)p;
+        V3 Tv = v3(T[0], T[1], T[2]);
+        Rng rng;
+        rng.pcg = true;
+        const uint32_t pix = (uint32_t)(p % W.npx);
+        const uint64_t stream = ((uint64_t)W.P.seed << 32) | pix;
+        rng.g.inc = (stream << 1u) | 1u;
+        rng.g.state = W.rng[p];
+        if (bounce >= 1 && (W.P.flags & LGI_RENDER_NO_RR) == 0) {
+            float qv = fminf(0.95f, fmaxf(Tv.x, fmaxf(Tv.y, Tv.z)));
+            if (rng.next() > qv) cont = false;
+            else Tv = mul(Tv, lgi_div(1.0f, qv));
+        }
+        if (cont) {
+            V3 nd = sample_cos_hemi(N, rng);
+            next = make_ray(add(Pt, mul(N, 1e-3f)), nd);
+            Tv = had(Tv, albedo);
+            T[0] = Tv.x; T[1] = Tv.y; T[2] = Tv.z;
+        }
+        W.rng[p] = rng.g.state;
+    }
+    const int s = wave_append(cont, nq_count);
+    if (s >= 0) {
+        QRay nr;
+        nr.o = make_float4(next.o.x, next.o.y, next.o.z, __int_as_float(p));
+        nr.d = make_float4(next.d.x, next.d.y, next.d.z, 1e30f);
+        nq[s] = nr;
+        nkeys[s] = ray_key(W, next.o, next.d);
+        nidx[s] = s;
+    }
+}
+
+// L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313).
+__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q, const float4* hits, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int p = __float_as_int(q[i].o.w);
+    if (p < 0 || __float_as_int(hits[i].w) < 0) return;
+    V3 color = v3(0.f, 0.f, 0.f);
+    for (int li = 0; li < W.nl; ++li) {
+        const int64_t k = (int64_t)p * W.nl + li;
+        if (W.nee[k] == 1) color = add(color, v3(W.contrib[3 * k], W.contrib[3 * k + 1], W.contrib[3 * k + 2]));
+    }
+    float* L = W.L + 3 * (int64_t)p;
+    const float* Tb = W.Tb + 3 * (int64_t)p;
+    V3 Lv = add(v3(L[0], L[1], L[2]), had(v3(Tb[0], Tb[1], Tb[2]), color));
+    L[0] = Lv.x; L[1] = Lv.y; L[2] = Lv.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_wf_gather(const QRay* src, const int32_t* idx, QRay* dst, int n) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// Kahan-add the S samples of every pixel in sample order (same as the inline kernel).
+__global__ __launch_bounds__(kBlock) void k_wf_accum(WfParams W, int S) {
+    const int pix = blockIdx.x * kBlock + threadIdx.x;
+    if (pix >= W.npx) return;
+    float* sum = W.P.sum + 3 * (int64_t)pix;
+    float* comp = W.P.comp + 3 * (int64_t)pix;
+    float s0 = sum[0], s1 = sum[1], s2 = sum[2], c0 = comp[0], c1 = comp[1], c2 = comp[2];
+    float sq = W.P.sumsq ? W.P.sumsq[pix] : 0.f;
+    for (int s = 0; s < S; ++s) {
+        const float* L = W.L + 3 * ((int64_t)s * W.npx + pix);
+        kahan_add(s0, c0, L[0]);
+        kahan_add(s1, c1, L[1]);
+        kahan_add(s2, c2, L[2]);
+        float lum = 0.2126f * L[0] + 0.7152f * L[1] + 0.0722f * L[2];
+        sq += lum * lum;
+    }
+    sum[0] = s0; sum[1] = s1; sum[2] = s2;
+    comp[0] = c0; comp[1] = c1; comp[2] = c2;
+    if (W.P.sumsq) W.P.sumsq[pix] = sq;
+}
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+int trace_grid(int n) {
+    // persistent: one resident round (32 KB of LDS stack per block -> 5 blocks per CU on
+    // 256 CUs), never more blocks than the work needs
+    return std::max(1, std::min(cdiv(n, kBlock), 256 * 5));
+}
+
+}  // namespace
+
+// Host driver of the wavefront renderer. Adds spp samples into P.sum/P.comp/P.sumsq.
+lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max) {
+    hipStream_t st = ctx->stream;
+    const int Wd = P.cam.width, Hd = P.cam.height;
+    const int npx = Wd * Hd;
+    const int tiles_x = (Wd + 7) / 8, tiles_y = (Hd + 7) / 8;
+    const int padded = tiles_x * tiles_y * 64;
+    const int nl = 1 + P.n_points;
+    // samples per pass: keep the path count under ~24M (a few GB of queues)
+    const int max_paths = 24 << 20;
+    const int S_max = std::max(1, std::min(P.spp, max_paths / std::max(1, padded)));
+    const int64_t cap = (int64_t)S_max * padded;
+    auto S_ = [&](int slot, size_t bytes) { return ctx->scratch_get(slot, bytes); };
+    QRay* qa = (QRay*)S_(50, cap * sizeof(QRay));
+    QRay* qb = (QRay*)S_(51, cap * sizeof(QRay));
+    float4* hits = (float4*)S_(52, cap * sizeof(float4));
+    QRay* sq = (QRay*)S_(53, cap * nl * sizeof(QRay));
+    uint32_t* keys = (uint32_t*)S_(54, cap * 4);
+    int32_t* idx = (int32_t*)S_(55, cap * 4);
+    void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
+    uint32_t* counters = (uint32_t*)S_(57, 64);
+    WfParams W;
+    W.P = P;
+    W.npx = npx;
+    W.nl = nl;
+    W.bb_min = bb_min;
+    W.bb_inv = make_float3(1.0f / std::max(bb_max.x - bb_min.x, 1e-6f), 1.0f / std::max(bb_max.y - bb_min.y, 1e-6f),
+                           1.0f / std::max(bb_max.z - bb_min.z, 1e-6f));
+    const int64_t paths = (int64_t)S_max * npx;
+    W.rng = (uint64_t*)S_(58, paths * 8);
+    W.T = (float*)S_(59, paths * 12);
+    W.Tb = (float*)S_(60, paths * 12);
+    W.L = (float*)S_(61, paths * 12);
+    W.contrib = (float*)S_(62, paths * nl * 12);
+    W.nee = (uint8_t*)S_(63, paths * nl);
+    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sq, (void*)keys, (void*)idx, sort_scratch,
+                    (void*)counters, (void*)W.rng, (void*)W.T, (void*)W.Tb, (void*)W.L, (void*)W.contrib,
+                    (void*)W.nee})
+        if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
+    LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    for (int s0 = 0; s0 < P.spp; s0 += S_max) {
+        const int S = std::min(S_max, P.spp - s0);
+        W.P.sample0 = P.sample0 + s0;
+        int n = S * padded;
+        hipLaunchKernelGGL(k_wf_primary, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, S, tiles_x, padded, qa);
+        QRay* cur = qa;
+        QRay* other = qb;
+        for (int b = 0; b <= P.bounces && n > 0; ++b) {
+            LGI_HIP(ctx, hipMemsetAsync(counters, 0, 64, st));
+            hipLaunchKernelGGL(k_wf_trace<false>, dim3(trace_grid(n)), dim3(kBlock), 0, st, ctx->dev.tnodes,
+                               ctx->dev.triv, cur, n, hits, (uint8_t*)nullptr, counters + 0);
+            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sq,
+                               counters + 1, other, keys, idx, counters + 2);
+            uint32_t hc[4];
+            LGI_HIP(ctx, hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
+            LGI_HIP(ctx, hipStreamSynchronize(st));
+            const int ns = (int)hc[1], nn = (int)hc[2];
+            if (ns > 0)
+                hipLaunchKernelGGL(k_wf_trace<true>, dim3(trace_grid(ns)), dim3(kBlock), 0, st, ctx->dev.tnodes,
+                                   ctx->dev.triv, sq, ns, (float4*)nullptr, W.nee, counters + 3);
+            hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
+            if (b == P.bounces || nn == 0) break;
+            // sort the next bounce's rays by origin/direction key, then gather them in order
+            lgi_sort_pairs(st, keys, idx, nn, 32, sort_scratch);
+            hipLaunchKernelGGL(k_wf_gather, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, other, idx, cur, nn);
+            n = nn;
+        }
+        hipLaunchKernelGGL(k_wf_accum, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, st, W, S);
+    }
+    LGI_HIP(ctx, hipGetLastError());
+    LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));
+    return LGI_OK;
+}

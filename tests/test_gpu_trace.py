@@ -103,6 +103,29 @@ def test_two_bounce_64spp_small(gpu, flags):
     assert np.all(np.abs(g - o["sum64"]) <= 1e-6 * np.abs(o["sum64"]) + 1e-6)
 
 
+@pytest.mark.parametrize("flags", [2, 2 | 4], ids=["sorted", "sorted_no_rr"])
+def test_sorted_wavefront_matches_inline_and_oracle(gpu, flags):
+    """The sorted wavefront renderer produces the inline kernel's sums bit for bit."""
+    ctx, osc, s = make_pair(seed=6)
+    cam = small_camera(s["camera"], 200, 100)
+    inline = gpu_accumulate(ctx, cam, 16, 2, flags=flags & 4)
+    wave = gpu_accumulate(ctx, cam, 16, 2, flags=flags)
+    for a, b in zip(inline, wave):
+        a, b = a.cpu().numpy(), b.cpu().numpy()
+        print(f"sorted vs inline: {np.mean(a == b):.6f} equal", flush=True)
+        assert np.array_equal(a, b)
+    o = osc.render(16, 2, flags=flags & 4, camera=cam)
+    assert_close_images(wave[0].cpu().numpy(), o["sum"], f"sorted flags={flags}")
+
+
+def test_sorted_rejects_xorshift(gpu):
+    import lgi
+    ctx, _, s = make_pair(seed=6, w=32, h=16)
+    with pytest.raises(lgi.LgiError) as e:
+        gpu_accumulate(ctx, s["camera"], 1, 0, flags=lgi.RENDER_SORTED | lgi.RNG_XORSHIFT_REF)
+    assert e.value.status == 8
+
+
 def test_chunked_accumulation_equals_single_call(gpu):
     ctx, osc, s = make_pair(seed=2)
     cam = small_camera(s["camera"], 96, 48)
