@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-row-step", type=int, default=2)
     ap.add_argument("--cpu-spp", type=int, default=8)
+    ap.add_argument("--mode", choices=["sorted", "inline"], default="sorted",
+                    help="sorted wavefront renderer (default) or the inline megakernel; same image bits")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
@@ -105,15 +107,16 @@ def main():
     comp = torch.zeros(n_px * 3, device=dev)
     sumsq = torch.zeros(n_px, device=dev)
 
+    flags = lgi.RENDER_SORTED if args.mode == "sorted" else 0
     sample0 = 0
     for _ in range(args.warmup):
-        ctx.accumulate(acc, comp, args.spp, args.bounces, sample0=sample0, d_sumsq=sumsq)
+        ctx.accumulate(acc, comp, args.spp, args.bounces, flags=flags, sample0=sample0, d_sumsq=sumsq)
         sample0 += args.spp
     torch.cuda.synchronize()
     kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.accumulate(acc, comp, args.spp, args.bounces, sample0=sample0, d_sumsq=sumsq)
+        ctx.accumulate(acc, comp, args.spp, args.bounces, flags=flags, sample0=sample0, d_sumsq=sumsq)
         kernel_ms.append(ctx.last_render_ms())  # HIP events on the context stream, per launch
         sample0 += args.spp
     torch.cuda.synchronize()
@@ -127,11 +130,11 @@ def main():
         "metric": METRIC, "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "plain path tracer, 62-mesh procedural scene (%d triangles), %dx%d, %d spp per "
-                               "step, %d bounces, sun + 2 point lights, PCG32" % (len(scene["tris"]), W, H,
-                                                                                  args.spp, args.bounces),
+        "config": {"workload": "plain path tracer (%s), 62-mesh procedural scene (%d triangles), %dx%d, %d spp "
+                               "per step, %d bounces, sun + 2 point lights, PCG32" % (args.mode, len(scene["tris"]),
+                                                                                      W, H, args.spp, args.bounces),
                    "triangles": int(len(scene["tris"])), "width": W, "height": H, "spp_per_step": args.spp,
-                   "bounces": args.bounces, "scene_seed": args.seed},
+                   "bounces": args.bounces, "scene_seed": args.seed, "renderer": args.mode},
         "render_kernel_ms": round(avg_kernel_ms, 3),
         "lbvh_build_ms": round(float(np.median(build_ms[1:])), 3),
     }
@@ -141,13 +144,16 @@ def main():
                                                                            args.cpu_row_step, threads, args.cpu_spp)
         algo_bytes = bps * samples_per_step
         achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
+        # HBM traffic per frame from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+        # this workload (calibrated against a plain copy); the timing here is live.
         traffic, counters_src = None, None
-        prof = ROOT / "profiles" / "k_render_config2.json"
-        if prof.exists() and args.spp == 8 and args.bounces == 2:
+        rel = "profiles/render_%s_config2.json" % args.mode
+        prof = ROOT / rel
+        if prof.exists() and args.spp == 8 and args.bounces == 2 and args.seed == 1:
             p = json.loads(prof.read_text())
             traffic, counters_src = int(p["traffic_bytes"]), {
-                "file": "profiles/k_render_config2.json", "kernel_avg_ms": round(p["avg_ns"] / 1e6, 3),
-                "l2_hit_rate": round(p["l2_hit_rate"], 4)}
+                "file": rel, "profiled_frame_ms": round(p["avg_ns"] / 1e6, 3),
+                "l2_hit_rate": round(p["l2_hit_rate"], 4), "kernels": p.get("kernels")}
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                               "algorithmic_bytes": int(algo_bytes), "bytes_per_sample": round(bps, 1),

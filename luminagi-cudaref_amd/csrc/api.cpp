@@ -122,6 +122,14 @@ TraceParams make_params(lgi_ctx* ctx, const lgi_camera& cam, int spp, int bounce
     return P;
 }
 
+// Global slots for traversal-stack entries beyond the LDS part, one column per thread.
+lgi_status attach_spill(lgi_ctx* ctx, TraceParams& P, int64_t threads) {
+    P.spill = (int*)ctx->scratch_get(44, (size_t)kStackSpill * threads * 4);
+    P.spill_stride = (int32_t)threads;
+    if (!P.spill) return lgi_set_err(ctx, LGI_ERR_HIP, "stack spill allocation failed");
+    return LGI_OK;
+}
+
 lgi_status check_render_ready(lgi_ctx* ctx, const lgi_camera& cam) {
     if (!ctx) return LGI_ERR_INVALID_ARG;
     if (ctx->scene.tris.empty()) return lgi_set_err(ctx, LGI_ERR_NO_SCENE, "no scene");
@@ -324,6 +332,7 @@ lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t
         return lgi_render_wavefront(ctx, P, make_float3(r.bb_min[0], r.bb_min[1], r.bb_min[2]),
                                     make_float3(r.bb_max[0], r.bb_max[1], r.bb_max[2]));
     }
+    if ((st = attach_spill(ctx, P, lgi_grid_threads(cam.width, cam.height))) != LGI_OK) return st;
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     lgi_launch_render(P, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());
@@ -357,6 +366,7 @@ lgi_status lgi_primary_hits(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t* d_t
     lgi_status st = check_render_ready(ctx, cam);
     if (st != LGI_OK) return st;
     TraceParams P = make_params(ctx, cam, 1, 0, 0, 0);
+    if ((st = attach_spill(ctx, P, lgi_grid_threads(cam.width, cam.height))) != LGI_OK) return st;
     lgi_launch_primary(P, d_tri, d_t, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());
     return LGI_OK;
@@ -376,6 +386,7 @@ lgi_status lgi_gbuffer(lgi_ctx* ctx, const lgi_camera* cam_in, uint8_t* d_alb, u
           ez = root.bb_max[2] - root.bb_min[2];
     float3 inv = make_float3(1.0f / std::fmax(ex, 1e-6f), 1.0f / std::fmax(ey, 1e-6f), 1.0f / std::fmax(ez, 1e-6f));
     TraceParams P = make_params(ctx, cam, 1, 0, 0, 0);
+    if ((st = attach_spill(ctx, P, lgi_grid_threads(cam.width, cam.height))) != LGI_OK) return st;
     lgi_launch_gbuffer(P, mn, inv, d_alb, d_nrm, d_wp, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());
     return LGI_OK;

@@ -509,6 +509,16 @@ void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bi
     radix_sort(st, b, n, bits);
 }
 
+size_t lgi_scan_scratch_bytes(int64_t n) { return 4 * ((size_t)ceil_div(n, kTile) + 64); }
+
+void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, void* scratch) {
+    const int nb = ceil_div(n, kTile);
+    uint32_t* sums = (uint32_t*)scratch;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(std::max(1, nb)), dim3(kBlock), 0, st, data, n, sums);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, sums, nb, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(std::max(1, nb)), dim3(kBlock), 0, st, data, n, sums);
+}
+
 void lgi_free_dev_scene(DevScene& d) {
     void* ptrs[] = {d.tri_in, d.nodes, d.tnodes, d.triv, d.tris, d.mat_albedo, d.points, d.perm, d.keys};
     for (void* p : ptrs)

@@ -107,6 +107,9 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx);
 // scratch must hold lgi_sort_scratch_bytes(n).
 size_t lgi_sort_scratch_bytes(int64_t n);
 void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch);
+// In-place exclusive scan of n u32 values; *total (device) receives the sum.
+size_t lgi_scan_scratch_bytes(int64_t n);
+void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, void* scratch);
 void lgi_free_dev_scene(DevScene& d);
 DevCamera lgi_make_dev_camera(const lgi_camera& c);
 lgi_status lgi_upload_lights(lgi_ctx* ctx, DevLights* out);

@@ -21,11 +21,16 @@ using namespace lgi;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStack = 32;
+static_assert(TravStack::kBlockThreads == kBlock, "stack stride");
+
+__device__ __forceinline__ TravStack make_stack(const TraceParams& P, int* s_stack) {
+    const int gtid = (blockIdx.y * gridDim.x + blockIdx.x) * kBlock + threadIdx.x;
+    return TravStack{s_stack + threadIdx.x, P.spill + gtid, P.spill_stride};
+}
 
 // Closest hit (kAny = false) or any hit within (1e-6, tmax) (kAny = true).
 template <bool kAny>
-__device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, int* stk) {
+__device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, const TravStack& stk) {
     Hit h{-1, tmax, 0.f, 0.f};
     int node = 0;
     int sp = 0;
@@ -38,7 +43,7 @@ __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, int* stk
             if (ha && hb) {
                 int nearc = n.ch.x, farc = n.ch.y;
                 if (tb < ta) { nearc = n.ch.y; farc = n.ch.x; }
-                stk[sp * kBlock] = farc;
+                stk.push(sp, farc);
                 ++sp;
                 node = nearc;
                 continue;
@@ -60,14 +65,14 @@ __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, int* stk
         }
         if (sp == 0) break;
         --sp;
-        node = stk[sp * kBlock];
+        node = stk.pop(sp);
     }
     return h;
 }
 
 // shade_direct (src/pathtracer.cu:230-272): lights in order sun, points; the colour is
 // accumulated only for unoccluded lights.
-__device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* stk) {
+__device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, const TravStack& stk) {
     V3 color = v3(0.f, 0.f, 0.f);
     for (int li = 0; li <= P.n_points; ++li) {
         Ray shadow;
@@ -81,7 +86,7 @@ __device__ V3 shade_direct(const TraceParams& P, V3 Pt, V3 N, V3 albedo, int* st
 }
 
 // trace_path (src/pathtracer.cu:295-331).
-__device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, int* stk) {
+__device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, const TravStack& stk) {
     V3 L = v3(0.f, 0.f, 0.f);
     V3 T = v3(1.f, 1.f, 1.f);
     const bool rr = (P.flags & LGI_RENDER_NO_RR) == 0;
@@ -116,11 +121,11 @@ __device__ __forceinline__ void tile_pixel(int bx, int by, int tid, int& px, int
 
 // accumulate_kernel (src/pathtracer.cu:333-364): spp samples per pixel, Kahan-compensated.
 __global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
-    __shared__ int s_stack[kStack * kBlock];
+    __shared__ int s_stack[kStackLds * kBlock];
     int px, py;
     tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
     if (px >= P.cam.width || py >= P.cam.height) return;
-    int* stk = s_stack + threadIdx.x;
+    const TravStack stk = make_stack(P, s_stack);
     const uint32_t pix = (uint32_t)py * (uint32_t)P.cam.width + (uint32_t)px;
     float s0 = P.sum[3 * pix], s1 = P.sum[3 * pix + 1], s2 = P.sum[3 * pix + 2];
     float c0 = P.comp[3 * pix], c1 = P.comp[3 * pix + 1], c2 = P.comp[3 * pix + 2];
@@ -156,12 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
 
 // Pixel-centre primary hits: original triangle index and t (src/pathtracer.cu:381-384).
 __global__ __launch_bounds__(kBlock) void k_primary(TraceParams P, int32_t* tri_out, float* t_out) {
-    __shared__ int s_stack[kStack * kBlock];
+    __shared__ int s_stack[kStackLds * kBlock];
     int px, py;
     tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
     if (px >= P.cam.width || py >= P.cam.height) return;
     Ray r = primary_ray(P.cam, px, py, 0.5f, 0.5f);
-    Hit h = traverse<false>(P, r, 1e30f, s_stack + threadIdx.x);
+    Hit h = traverse<false>(P, r, 1e30f, make_stack(P, s_stack));
     const size_t i = (size_t)py * P.cam.width + px;
     tri_out[i] = h.tri < 0 ? -1 : __float_as_int(P.triv[h.tri].e1.w);
     t_out[i] = h.tri < 0 ? -1.0f : h.t;
@@ -170,12 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_primary(TraceParams P, int32_t* tri_
 // gbuffer_kernel (src/pathtracer.cu:366-411).
 __global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin, float3 invExt, uint8_t* alb,
                                                     uint8_t* nrm, uint8_t* wp) {
-    __shared__ int s_stack[kStack * kBlock];
+    __shared__ int s_stack[kStackLds * kBlock];
     int px, py;
     tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
     if (px >= P.cam.width || py >= P.cam.height) return;
     Ray r = primary_ray(P.cam, px, py, 0.5f, 0.5f);
-    Hit h = traverse<false>(P, r, 1e30f, s_stack + threadIdx.x);
+    Hit h = traverse<false>(P, r, 1e30f, make_stack(P, s_stack));
     const size_t i = ((size_t)py * P.cam.width + px) * 3;
     if (h.tri < 0) {
         alb[i] = alb[i + 1] = alb[i + 2] = 0;

@@ -21,7 +21,16 @@ struct TraceParams {
     float* sum;     // W*H*3
     float* comp;    // W*H*3 Kahan compensation
     float* sumsq;   // W*H (nullable)
+    int* spill;     // traversal-stack spill slots, [depth][thread]
+    int32_t spill_stride;
 };
+
+// Traversal stack split: entries [0, kStackLds) in LDS, [kStackLds, 32) in global spill slots.
+constexpr int kStackLds = 16;
+constexpr int kStackSpill = 16;
+
+// Threads of the pixel-grid kernels (16x16-pixel blocks) for a W x H image.
+inline int64_t lgi_grid_threads(int w, int h) { return (int64_t)((w + 15) / 16) * ((h + 15) / 16) * 256; }
 
 void lgi_launch_render(const TraceParams& P, hipStream_t st);
 void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_t st);

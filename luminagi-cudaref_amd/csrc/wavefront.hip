@@ -23,7 +23,8 @@ using namespace lgi;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kStack = 32;
+constexpr int kTraceGridMax = 256 * 8;
+static_assert(TravStack::kBlockThreads == 256, "stack stride");
 
 struct alignas(16) QRay {
     float4 o;   // origin, w = path id (int bits)
@@ -66,20 +67,6 @@ __device__ __forceinline__ uint32_t ray_key(const WfParams& W, V3 o, V3 d) {
            ((spread3(dx) << 2) | (spread3(dy) << 1) | spread3(dz));
 }
 
-// Wave-aggregated append: one atomic per wave; returns this lane's slot (or -1).
-__device__ __forceinline__ int wave_append(bool want, uint32_t* counter) {
-    const uint64_t m = __ballot(want);
-    if (m == 0) return -1;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (!want) return -1;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    return (int)(base + (uint32_t)__popcll(m & lt));
-}
-
 // ------------------------------------------------------------------ primary rays
 // Queue index i -> (sample s, 8x8 pixel tile, lane) so each wave starts on one tile.
 __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int tiles_x, int padded, QRay* q) {
@@ -119,9 +106,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
 template <bool kAny>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, const TriV* __restrict__ triv,
                                                      const QRay* __restrict__ q, int n, float4* hits, uint8_t* nee,
-                                                     uint32_t* counter) {
-    __shared__ int s_stack[kStack * kBlock];
-    int* stk = s_stack + threadIdx.x;
+                                                     uint32_t* counter, int* spill) {
+    __shared__ int s_stack[kStackLds * kBlock];
+    const TravStack stk{s_stack + threadIdx.x, spill + blockIdx.x * kBlock + threadIdx.x, (int)gridDim.x * kBlock};
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     // wave-uniform chunk cursor
@@ -196,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
             if (ha && hb) {
                 int nearc = nd.ch.x, farc = nd.ch.y;
                 if (tb < ta) { nearc = nd.ch.y; farc = nd.ch.x; }
-                stk[sp * kBlock] = farc;
+                stk.push(sp, farc);
                 ++sp;
                 node = nearc;
             } else if (ha) {
@@ -207,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 done = true;
             } else {
                 --sp;
-                node = stk[sp * kBlock];
+                node = stk.pop(sp);
             }
         } else {
             int ti = ~node;
@@ -225,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 done = true;
             } else {
                 --sp;
-                node = stk[sp * kBlock];
+                node = stk.pop(sp);
             }
         }
         if (done) {
@@ -243,8 +230,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
 // For each traced path: NEE candidates -> shadow queue, Russian roulette + cosine bounce ->
 // next queue with its sort key (src/pathtracer.cu:295-331, same order of RNG draws).
 __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n, int bounce,
-                                                     QRay* sq, uint32_t* sq_count, QRay* nq, uint32_t* nkeys,
-                                                     int32_t* nidx, uint32_t* nq_count) {
+                                                     QRay* sq, uint32_t* sflag, QRay* nq, uint32_t* nkeys,
+                                                     uint32_t* nflag) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
@@ -271,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         float* Tb = W.Tb + 3 * (int64_t)p;
         Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
     }
-    // NEE candidates, one wave-aggregated append per light
+    if (i >= n) return;
+    // NEE candidates into fixed slots i*nl + li (compacted afterwards by a scan)
     for (int li = 0; li < W.nl; ++li) {
         Ray shadow;
         float tmax = 0.f;
@@ -282,13 +270,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             W.nee[k] = cand ? 1 : 0;
             if (cand) { W.contrib[3 * k] = c.x; W.contrib[3 * k + 1] = c.y; W.contrib[3 * k + 2] = c.z; }
         }
-        const int s = wave_append(cand, sq_count);
-        if (s >= 0) {
-            QRay sr;
-            sr.o = make_float4(shadow.o.x, shadow.o.y, shadow.o.z, __int_as_float((int)((int64_t)p * W.nl + li)));
-            sr.d = make_float4(shadow.d.x, shadow.d.y, shadow.d.z, tmax);
-            sq[s] = sr;
-        }
+        const int64_t j = (int64_t)i * W.nl + li;
+        QRay sr;
+        sr.o = make_float4(shadow.o.x, shadow.o.y, shadow.o.z,
+                           __int_as_float(cand ? (int)((int64_t)p * W.nl + li) : -1));
+        sr.d = make_float4(shadow.d.x, shadow.d.y, shadow.d.z, tmax);
+        if (cand) sq[j] = sr;
+        else sq[j].o.w = __int_as_float(-1);
+        sflag[j] = cand ? 1u : 0u;
     }
     // continuation
     bool cont = live && bounce < W.P.bounces;
@@ -315,14 +304,30 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         }
         W.rng[p] = rng.g.state;
     }
-    const int s = wave_append(cont, nq_count);
-    if (s >= 0) {
+    if (cont) {
         QRay nr;
         nr.o = make_float4(next.o.x, next.o.y, next.o.z, __int_as_float(p));
         nr.d = make_float4(next.d.x, next.d.y, next.d.z, 1e30f);
-        nq[s] = nr;
-        nkeys[s] = ray_key(W, next.o, next.d);
-        nidx[s] = s;
+        nq[i] = nr;
+        nkeys[i] = ray_key(W, next.o, next.d);
+    } else {
+        nq[i].o.w = __int_as_float(-1);
+    }
+    nflag[i] = cont ? 1u : 0u;
+}
+
+// Scatter flagged entries to their scanned positions (order-preserving compaction).
+__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint32_t* src_key, const uint32_t* off,
+                                                       int64_t n, QRay* dst, uint32_t* dst_key, int32_t* dst_idx) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const QRay e = src[i];
+    if (__float_as_int(e.o.w) < 0) return;
+    const uint32_t o = off[i];
+    dst[o] = e;
+    if (dst_key) {
+        dst_key[o] = src_key[i];
+        dst_idx[o] = (int32_t)o;
     }
 }
 
@@ -372,9 +377,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_accum(WfParams W, int S) {
 inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 int trace_grid(int n) {
-    // persistent: one resident round (32 KB of LDS stack per block -> 5 blocks per CU on
-    // 256 CUs), never more blocks than the work needs
-    return std::max(1, std::min(cdiv(n, kBlock), 256 * 5));
+    // persistent: one resident round (16 KB of LDS stack per block -> 8 blocks = 32 waves per
+    // CU on 256 CUs), never more blocks than the work needs
+    return std::max(1, std::min(cdiv(n, kBlock), kTraceGridMax));
 }
 
 }  // namespace
@@ -395,11 +400,18 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     QRay* qa = (QRay*)S_(50, cap * sizeof(QRay));
     QRay* qb = (QRay*)S_(51, cap * sizeof(QRay));
     float4* hits = (float4*)S_(52, cap * sizeof(float4));
-    QRay* sq = (QRay*)S_(53, cap * nl * sizeof(QRay));
+    QRay* sq_tmp = (QRay*)S_(53, cap * nl * sizeof(QRay));
+    QRay* sq = (QRay*)S_(64, cap * nl * sizeof(QRay));
+    uint32_t* sflag = (uint32_t*)S_(65, cap * nl * 4);
+    uint32_t* nflag = (uint32_t*)S_(66, cap * 4);
+    uint32_t* keys_tmp = (uint32_t*)S_(67, cap * 4);
     uint32_t* keys = (uint32_t*)S_(54, cap * 4);
     int32_t* idx = (int32_t*)S_(55, cap * 4);
     void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
+    void* scan_scratch = S_(68, lgi_scan_scratch_bytes(cap * nl));
     uint32_t* counters = (uint32_t*)S_(57, 64);
+    int* spill = (int*)S_(69, (size_t)kStackSpill * kTraceGridMax * kBlock * 4);
+    if (!spill) return lgi_set_err(ctx, LGI_ERR_HIP, "stack spill allocation failed");
     WfParams W;
     W.P = P;
     W.npx = npx;
@@ -414,9 +426,9 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     W.L = (float*)S_(61, paths * 12);
     W.contrib = (float*)S_(62, paths * nl * 12);
     W.nee = (uint8_t*)S_(63, paths * nl);
-    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sq, (void*)keys, (void*)idx, sort_scratch,
-                    (void*)counters, (void*)W.rng, (void*)W.T, (void*)W.Tb, (void*)W.L, (void*)W.contrib,
-                    (void*)W.nee})
+    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sq_tmp, (void*)sq, (void*)sflag, (void*)nflag,
+                    (void*)keys_tmp, (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters,
+                    (void*)W.rng, (void*)W.T, (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
     for (int s0 = 0; s0 < P.spp; s0 += S_max) {
@@ -429,21 +441,31 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         for (int b = 0; b <= P.bounces && n > 0; ++b) {
             LGI_HIP(ctx, hipMemsetAsync(counters, 0, 64, st));
             hipLaunchKernelGGL(k_wf_trace<false>, dim3(trace_grid(n)), dim3(kBlock), 0, st, ctx->dev.tnodes,
-                               ctx->dev.triv, cur, n, hits, (uint8_t*)nullptr, counters + 0);
-            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sq,
-                               counters + 1, other, keys, idx, counters + 2);
+                               ctx->dev.triv, cur, n, hits, (uint8_t*)nullptr, counters + 0, spill);
+            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sq_tmp,
+                               sflag, other, keys_tmp, nflag);
+            // order-preserving compaction of shadow rays and continuations (no global atomics)
+            const int64_t nslots = (int64_t)n * nl;
+            lgi_exclusive_scan(st, sflag, (int)nslots, counters + 1, scan_scratch);
+            lgi_exclusive_scan(st, nflag, n, counters + 2, scan_scratch);
             uint32_t hc[4];
             LGI_HIP(ctx, hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
             LGI_HIP(ctx, hipStreamSynchronize(st));
             const int ns = (int)hc[1], nn = (int)hc[2];
-            if (ns > 0)
+            if (ns > 0) {
+                hipLaunchKernelGGL(k_wf_compact, dim3(cdiv(nslots, kBlock)), dim3(kBlock), 0, st, sq_tmp,
+                                   (const uint32_t*)nullptr, sflag, nslots, sq, (uint32_t*)nullptr, (int32_t*)nullptr);
                 hipLaunchKernelGGL(k_wf_trace<true>, dim3(trace_grid(ns)), dim3(kBlock), 0, st, ctx->dev.tnodes,
-                                   ctx->dev.triv, sq, ns, (float4*)nullptr, W.nee, counters + 3);
+                                   ctx->dev.triv, sq, ns, (float4*)nullptr, W.nee, counters + 3, spill);
+            }
             hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
             if (b == P.bounces || nn == 0) break;
-            // sort the next bounce's rays by origin/direction key, then gather them in order
+            // compact the continuations, sort them by origin/direction key, gather in key order
+            hipLaunchKernelGGL(k_wf_compact, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, other, keys_tmp, nflag,
+                               (int64_t)n, cur, keys, idx);
             lgi_sort_pairs(st, keys, idx, nn, 32, sort_scratch);
-            hipLaunchKernelGGL(k_wf_gather, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, other, idx, cur, nn);
+            hipLaunchKernelGGL(k_wf_gather, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, cur, idx, other, nn);
+            std::swap(cur, other);
             n = nn;
         }
         hipLaunchKernelGGL(k_wf_accum, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, st, W, S);
