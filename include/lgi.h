@@ -171,6 +171,43 @@ LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_
 /* ---- image output (stbi_write_png call sites, src/main.cpp:176,214) ---- */
 LGI_API lgi_status lgi_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h);
 
+/* ---- denoiser layers (UNet of ml/train.py:37-75; NHWC BF16 activations on device) ----
+ * All pointers are device pointers; `stream` is a hipStream_t (null = default stream).
+ * Channel counts of BF16 tensors must be multiples of 32. Weights: [cout][9][cin] BF16 for
+ * the kernels, FP32 gradients in the same layout. Replaces the cuDNN/PyTorch calls of
+ * ml/train.py:41-44 (Conv2d + activation), :62-64 (MaxPool2d, Upsample), :71-75 (cat, tail). */
+/* y = relu?(conv3x3(cat(src0, src1)) + bias); src1 may be null (c1 ignored). */
+LGI_API lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1,
+                                        const void* w, const float* bias, int32_t cout, int32_t n,
+                                        int32_t h, int32_t w_, int32_t relu, void* out, void* stream);
+/* dx = conv3x3^T(dy * (ymask > 0)); channels [0, cin_split) -> dx0, the rest -> dx1.
+ * ymask may be null (no ReLU); wt_scratch holds cout*9*cin BF16. */
+LGI_API lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout,
+                                             int32_t cin, int32_t cin_split, int32_t n, int32_t h, int32_t w_,
+                                             void* dx0, void* dx1, void* wt_scratch, void* stream);
+/* dw = sum_pixels (dy * (ymask > 0)) (x) patches(cat(src0, src1)) in FP32; db likewise (nullable).
+ * scratch: FP32 workspace of scratch_floats (>= cout*9*cin). Deterministic (fixed-order sums). */
+LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_t cout, const void* src0,
+                                               int32_t c0, const void* src1, int32_t c1, int32_t n, int32_t h,
+                                               int32_t w_, float* dw, float* db, float* scratch,
+                                               int64_t scratch_floats, void* stream);
+LGI_API lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w_, int32_t c, void* out,
+                                        void* stream);
+LGI_API lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int32_t h, int32_t w_,
+                                        int32_t c, void* din, void* stream);
+/* bilinear 2x, align_corners = False: (n, h, w, c) -> (n, 2h, 2w, c) */
+LGI_API lgi_status lgi_unet_upsample_fwd(const void* in, int32_t n, int32_t h, int32_t w_, int32_t c, void* out,
+                                         void* stream);
+LGI_API lgi_status lgi_unet_upsample_bwd(const void* dout, int32_t n, int32_t h, int32_t w_, int32_t c, void* din,
+                                         void* stream);
+/* FP32 3-channel NHWC image -> BF16 NHWC zero-padded to 32 channels */
+LGI_API lgi_status lgi_unet_pack_input(const float* x, int64_t npx, void* out, void* stream);
+/* out = x + W_t feat + b_t (1x1, 32 -> 3, FP32 out); backward gives dfeat, dW_t, db_t */
+LGI_API lgi_status lgi_unet_tail_fwd(const void* feat, const float* x, const float* w, const float* b, int64_t npx,
+                                     float* out, void* stream);
+LGI_API lgi_status lgi_unet_tail_bwd(const void* feat, const float* dout, const float* w, int64_t npx, void* dfeat,
+                                     float* dw, float* db, float* scratch, int64_t scratch_floats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

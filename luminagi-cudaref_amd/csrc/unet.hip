@@ -1,0 +1,587 @@
+// unet.hip — denoiser layers for gfx950: 3x3 convolution as implicit GEMM on MFMA
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), max-pool 2x2, bilinear 2x upsample, the
+// 1x1 residual tail, and their backward passes.
+//
+// Layer semantics follow the reference UNet (ml/train.py:37-75: Conv2d 3x3 pad 1 + bias +
+// activation, MaxPool2d(2), 2x upsample, channel concat, 1x1 tail, residual), with the
+// denoiser configuration used here: ReLU, bilinear (align_corners=False) upsampling.
+//
+// Layout: NHWC, BF16 activations, weights [Cout][tap][Cin] (K = tap*Cin + ci contiguous per
+// output channel), FP32 bias / master weights / weight gradients.
+//
+// Convolution tiling: one 256-thread block computes an 8x16-pixel x TN-channel output tile.
+// Per 32-channel input chunk it stages the 10x18-pixel input halo and the TN x 9 x 32 weight
+// slice in LDS, then each wave (two output rows) issues 9 taps x 2 x (TN/16) MFMAs. The
+// channel concat of the decoder is fused into the halo gather (two source tensors), and the
+// ReLU backward mask is fused into the gather of the data-gradient pass.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lgi.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr int kTH = 8, kTW = 16;               // output tile (rows x cols)
+constexpr int kHH = kTH + 2, kHW = kTW + 2;    // input halo tile
+constexpr int kCK = 32;                        // input channels per K chunk
+constexpr int kWRow = 9 * kCK + 8;             // LDS weight row (bf16), padded against bank conflicts
+
+inline int cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+struct ConvArgs {
+    const bf16* src0;      // [N,H,W,C0]
+    const bf16* src1;      // [N,H,W,C1] or null (channel concat after src0)
+    const bf16* mask0;     // optional ReLU-backward masks, same shapes as src0/src1
+    const bf16* mask1;
+    const bf16* wk;        // [Cout][9][Cin] (or [Cout][1][Cin] for 1x1)
+    const float* bias;     // [Cout] or null
+    bf16* out0;            // channels [0, Cs)
+    bf16* out1;            // channels [Cs, Cout) or null
+    int C0, C1, Cout, Cs;
+    int N, H, W;
+    int relu;
+};
+
+__device__ __forceinline__ uint4 zero16() { return make_uint4(0u, 0u, 0u, 0u); }
+
+// Multiply 8 bf16 values by (mask > 0).
+__device__ __forceinline__ uint4 relu_mask(uint4 v, uint4 m) {
+    const bf16x8 mv = *reinterpret_cast<const bf16x8*>(&m);
+    bf16x8 x = *reinterpret_cast<const bf16x8*>(&v);
+    for (int i = 0; i < 8; ++i)
+        if (!((float)mv[i] > 0.0f)) x[i] = (bf16)0.0f;
+    return *reinterpret_cast<const uint4*>(&x);
+}
+
+template <int TN, int KS>
+__global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
+    constexpr int kTaps = KS * KS;
+    constexpr int kPad = KS / 2;
+    constexpr int kHalH = kTH + KS - 1, kHalW = kTW + KS - 1;
+    __shared__ __attribute__((aligned(16))) bf16 s_in[kHalH * kHalW * kCK];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[TN * kWRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * kTH, x0 = tx * kTW;
+    const int co0 = blockIdx.y * TN;
+    const int Cin = a.C0 + a.C1;
+    f32x4 acc[2][TN / 16];
+    for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < TN / 16; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int g = lane >> 4, r16 = lane & 15;
+    for (int ci0 = 0; ci0 < Cin; ci0 += kCK) {
+        // source tensor of this chunk (chunks never straddle the concat boundary)
+        const bool first = ci0 < a.C0;
+        const bf16* src = first ? a.src0 : a.src1;
+        const bf16* msk = first ? a.mask0 : a.mask1;
+        const int Cs = first ? a.C0 : a.C1;
+        const int cc0 = first ? ci0 : ci0 - a.C0;
+        // stage the input halo: kHalH x kHalW pixels x 32 channels, 16 B per thread-load
+        for (int t = tid; t < kHalH * kHalW * 4; t += kBlock) {
+            const int px = t >> 2, part = t & 3;
+            const int hy = px / kHalW, hx = px % kHalW;
+            const int gy = y0 + hy - kPad, gx = x0 + hx - kPad;
+            uint4 v = zero16();
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                const int64_t off = (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8;
+                v = *reinterpret_cast<const uint4*>(src + off);
+                if (msk) v = relu_mask(v, *reinterpret_cast<const uint4*>(msk + off));
+            }
+            *reinterpret_cast<uint4*>(&s_in[px * kCK + part * 8]) = v;
+        }
+        // stage weights: TN output channels x taps x 32 input channels
+        for (int t = tid; t < TN * kTaps * 4; t += kBlock) {
+            const int co = t / (kTaps * 4), r = t % (kTaps * 4), tap = r >> 2, part = r & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * kTaps + tap) * Cin + ci0 +
+                                                              part * 8);
+            *reinterpret_cast<uint4*>(&s_w[co * kWRow + tap * kCK + part * 8]) = v;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < kTaps; ++tap) {
+            const int ky = tap / KS, kx = tap % KS;
+            bf16x8 A[2], B[TN / 16];
+            for (int mi = 0; mi < 2; ++mi) {
+                const int row = wave * 2 + mi + ky;
+                A[mi] = *reinterpret_cast<const bf16x8*>(&s_in[(row * kHalW + r16 + kx) * kCK + g * 8]);
+            }
+            for (int ni = 0; ni < TN / 16; ++ni)
+                B[ni] = *reinterpret_cast<const bf16x8*>(&s_w[(ni * 16 + r16) * kWRow + tap * kCK + g * 8]);
+            for (int mi = 0; mi < 2; ++mi)
+                for (int ni = 0; ni < TN / 16; ++ni)
+                    acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mi], B[ni], acc[mi][ni], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue: D[m = 4g + j][n = r16] -> pixel column 4g + j, channel ni*16 + r16
+    for (int mi = 0; mi < 2; ++mi) {
+        const int y = y0 + wave * 2 + mi;
+        if (y >= a.H) continue;
+        for (int ni = 0; ni < TN / 16; ++ni) {
+            const int co = co0 + ni * 16 + r16;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+            bf16* out = co < a.Cs ? a.out0 : a.out1;
+            const int oc = co < a.Cs ? co : co - a.Cs;
+            const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + 4 * g + j;
+                if (x >= a.W) continue;
+                float v = acc[mi][ni][j] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
+            }
+        }
+    }
+}
+
+// Wt[ci][tap][co] = W[co][taps-1-tap][ci]: weights of the data-gradient convolution.
+__global__ __launch_bounds__(kBlock) void k_flip_weights(const bf16* w, int Cout, int Cin, int taps, bf16* wt) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t total = (int64_t)Cout * taps * Cin;
+    if (i >= total) return;
+    const int ci = (int)(i % Cin);
+    const int tap = (int)((i / Cin) % taps);
+    const int co = (int)(i / ((int64_t)Cin * taps));
+    wt[((int64_t)ci * taps + (taps - 1 - tap)) * Cout + co] = w[i];
+}
+
+// ------------------------------------------------------------------ weight gradient
+// dW[co][tap][ci] = sum over pixels of dYm[p][co] * X[p + tap offset][ci]. One block: 32
+// output channels x 32 input channels x all 9 taps (three waves, one kernel row each), over a
+// contiguous range of 8x16 pixel tiles; writes its FP32 partial sum (reduced afterwards).
+constexpr int kWgBlock = 192;
+__global__ __launch_bounds__(kWgBlock) void k_conv_wgrad(const bf16* dy, const bf16* ymask, int Cout,
+                                                         const bf16* src0, int C0, const bf16* src1, int C1, int N,
+                                                         int H, int W, int tiles_per_split, float* partial) {
+    // LDS: dY tile transposed [32 co][128 px]; X halo transposed and shifted by kx:
+    // [3 kx][32 ci][10 rows][16 cols] so that every MFMA operand read is 16-byte aligned.
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[32 * 128];
+    __shared__ __attribute__((aligned(16))) bf16 s_x[3 * 32 * kHH * kTW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;  // wave = ky
+    const int g = lane >> 4, r16 = lane & 15;
+    const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32, split = blockIdx.z;
+    const int Cin = C0 + C1;
+    const bool first = ci0 < C0;
+    const bf16* src = first ? src0 : src1;
+    const int Cs = first ? C0 : C1;
+    const int cc0 = first ? ci0 : ci0 - C0;
+    const int tiles_x = (W + kTW - 1) / kTW, tiles_y = (H + kTH - 1) / kTH;
+    const int ntiles = tiles_x * tiles_y * N;
+    f32x4 acc[3][2][2];  // [kx][co subtile][ci subtile]
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 2; ++c) acc[a][b][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int t_begin = split * tiles_per_split;
+    const int t_end = min(ntiles, t_begin + tiles_per_split);
+    for (int tile = t_begin; tile < t_end; ++tile) {
+        const int n = tile / (tiles_x * tiles_y);
+        const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+        const int y0 = ty * kTH, x0 = tx * kTW;
+        // dY (masked by ReLU) -> s_dy[co][px]
+        for (int t = tid; t < 128 * 4; t += kWgBlock) {
+            const int px = t >> 2, part = t & 3;
+            const int y = y0 + (px >> 4), x = x0 + (px & 15);
+            uint4 v = zero16();
+            if (y < H && x < W) {
+                const int64_t off = (((int64_t)n * H + y) * W + x) * Cout + co0 + part * 8;
+                v = *reinterpret_cast<const uint4*>(dy + off);
+                if (ymask) v = relu_mask(v, *reinterpret_cast<const uint4*>(ymask + off));
+            }
+            const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+            for (int i = 0; i < 8; ++i) s_dy[(part * 8 + i) * 128 + px] = e[i];
+        }
+        // X halo -> s_x[kx][ci][row][col] = X[y0 + row - 1][x0 + col + kx - 1][ci]
+        for (int t = tid; t < kHH * kHW * 4; t += kWgBlock) {
+            const int px = t >> 2, part = t & 3;
+            const int hy = px / kHW, hx = px % kHW;
+            const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+            uint4 v = zero16();
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *reinterpret_cast<const uint4*>(src + (((int64_t)n * H + gy) * W + gx) * Cs + cc0 + part * 8);
+            const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+            for (int kx = 0; kx < 3; ++kx) {
+                const int col = hx - kx;
+                if (col < 0 || col >= kTW) continue;
+                for (int i = 0; i < 8; ++i) s_x[((kx * 32 + part * 8 + i) * kHH + hy) * kTW + col] = e[i];
+            }
+        }
+        __syncthreads();
+        // K = 128 pixels in 4 steps of 32 (two 16-pixel rows per step)
+        for (int ks = 0; ks < 4; ++ks) {
+            const int prow = ks * 2 + (g >> 1), pcol = (g & 1) * 8;
+            bf16x8 A[2];
+            for (int mi = 0; mi < 2; ++mi)
+                A[mi] = *reinterpret_cast<const bf16x8*>(&s_dy[(mi * 16 + r16) * 128 + prow * 16 + pcol]);
+            for (int kx = 0; kx < 3; ++kx)
+                for (int ni = 0; ni < 2; ++ni) {
+                    const bf16x8 B = *reinterpret_cast<const bf16x8*>(
+                        &s_x[((kx * 32 + ni * 16 + r16) * kHH + prow + wave) * kTW + pcol]);
+                    for (int mi = 0; mi < 2; ++mi)
+                        acc[kx][mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mi], B, acc[kx][mi][ni], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    // partial[split][co][tap][ci]; D[m = 4g + j][n = r16]: co = m, ci = n
+    for (int kx = 0; kx < 3; ++kx) {
+        const int tap = wave * 3 + kx;
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni)
+                for (int j = 0; j < 4; ++j) {
+                    const int co = co0 + mi * 16 + 4 * g + j, ci = ci0 + ni * 16 + r16;
+                    partial[(((int64_t)split * Cout + co) * 9 + tap) * Cin + ci] = acc[kx][mi][ni][j];
+                }
+    }
+}
+
+// out[i] = sum over s of partial[s][i] (fixed order: deterministic).
+__global__ __launch_bounds__(kBlock) void k_reduce_splits(const float* partial, int64_t size, int splits, float* out,
+                                                          int accumulate) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= size) return;
+    float s = 0.f;
+    for (int k = 0; k < splits; ++k) s += partial[(int64_t)k * size + i];
+    out[i] = accumulate ? out[i] + s : s;
+}
+
+// db[co] = sum over pixels of dYm[p][co]; one block per 64 channels x pixel split.
+__global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* dy, const bf16* ymask, int64_t npx, int C,
+                                                      int px_per_split, float* partial) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int split = blockIdx.y;
+    __shared__ float s[kBlock];
+    float acc = 0.f;
+    if (c < C) {
+        const int64_t p0 = (int64_t)split * px_per_split;
+        const int64_t p1 = min(npx, p0 + px_per_split);
+        for (int64_t p = p0 + (threadIdx.x >> 6); p < p1; p += kBlock / 64) {
+            float v = (float)dy[p * C + c];
+            if (ymask && !((float)ymask[p * C + c] > 0.0f)) v = 0.f;
+            acc += v;
+        }
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < C) {
+        float t = s[threadIdx.x] + s[threadIdx.x + 64] + s[threadIdx.x + 128] + s[threadIdx.x + 192];
+        partial[(int64_t)split * C + c] = t;
+    }
+}
+
+// ------------------------------------------------------------------ pool / upsample
+// MaxPool2d(2): out[n,y,x,c] = max over the 2x2 window (first maximum in row-major order).
+__global__ __launch_bounds__(kBlock) void k_pool_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    if (i >= (int64_t)N * Ho * Wo * C) return;
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
+    float m = (float)in[base];
+    const float v1 = (float)in[base + C], v2 = (float)in[base + (int64_t)W * C], v3 = (float)in[base + (int64_t)W * C + C];
+    if (v1 > m || v1 != v1) m = v1;
+    if (v2 > m || v2 != v2) m = v2;
+    if (v3 > m || v3 != v3) m = v3;
+    out[i] = (bf16)m;
+}
+
+// Gradient to the first maximum of each window (the pooled input is re-read, not stored).
+__global__ __launch_bounds__(kBlock) void k_pool_bwd(const bf16* in, const bf16* dout, int N, int H, int W, int C,
+                                                     bf16* din) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    if (i >= (int64_t)N * Ho * Wo * C) return;
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
+    const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
+    float m = (float)in[base];
+    int arg = 0;
+    for (int k = 1; k < 4; ++k) {
+        const float v = (float)in[base + o[k]];
+        if (v > m || v != v) { m = v; arg = k; }
+    }
+    const bf16 g = dout[i];
+    for (int k = 0; k < 4; ++k) din[base + o[k]] = k == arg ? g : (bf16)0.0f;
+}
+
+// Bilinear 2x (align_corners = False): source coordinate (d + 0.5) / 2 - 0.5 clamped at 0.
+__device__ __forceinline__ void lin_src(int d, int in, int& i0, int& i1, float& l1) {
+    float s = (d + 0.5f) * 0.5f - 0.5f;
+    if (s < 0.f) s = 0.f;
+    i0 = (int)s;
+    i1 = min(i0 + 1, in - 1);
+    l1 = s - (float)i0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int Ho = 2 * H, Wo = 2 * W;
+    if (i >= (int64_t)N * Ho * Wo * C) return;
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    int y0, y1, x0, x1;
+    float ly, lx;
+    lin_src(y, H, y0, y1, ly);
+    lin_src(x, W, x0, x1, lx);
+    auto at = [&](int yy, int xx) { return (float)in[(((int64_t)n * H + yy) * W + xx) * C + c]; };
+    const float v = (1.f - ly) * ((1.f - lx) * at(y0, x0) + lx * at(y0, x1)) + ly * ((1.f - lx) * at(y1, x0) + lx * at(y1, x1));
+    out[i] = (bf16)v;
+}
+
+// Adjoint of k_up_fwd as a gather: each input pixel sums the weighted gradients of the
+// (at most 4 x 4) output pixels that read it.
+__global__ __launch_bounds__(kBlock) void k_up_bwd(const bf16* dout, int N, int H, int W, int C, bf16* din) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= (int64_t)N * H * W * C) return;
+    const int c = (int)(i % C);
+    const int64_t p = i / C;
+    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
+    const int Ho = 2 * H, Wo = 2 * W;
+    float acc = 0.f;
+    for (int oy = max(0, 2 * y - 2); oy <= min(Ho - 1, 2 * y + 2); ++oy) {
+        int y0, y1;
+        float ly;
+        lin_src(oy, H, y0, y1, ly);
+        float wy = (y0 == y ? 1.f - ly : 0.f) + (y1 == y ? ly : 0.f);
+        if (wy == 0.f) continue;
+        for (int ox = max(0, 2 * x - 2); ox <= min(Wo - 1, 2 * x + 2); ++ox) {
+            int x0, x1;
+            float lx;
+            lin_src(ox, W, x0, x1, lx);
+            float wx = (x0 == x ? 1.f - lx : 0.f) + (x1 == x ? lx : 0.f);
+            if (wx == 0.f) continue;
+            acc += wy * wx * (float)dout[(((int64_t)n * Ho + oy) * Wo + ox) * C + c];
+        }
+    }
+    din[i] = (bf16)acc;
+}
+
+// ------------------------------------------------------------------ input / tail
+// log1p radiance (FP32, NHWC with 3 channels) -> BF16 NHWC padded to 32 channels.
+__global__ __launch_bounds__(kBlock) void k_pack_input(const float* x, int64_t npx, bf16* out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= npx * 32) return;
+    const int c = (int)(i % 32);
+    out[i] = c < 3 ? (bf16)x[(i / 32) * 3 + c] : (bf16)0.0f;
+}
+
+// out = x + W_t * feat + b_t (1x1 conv, 32 -> 3, FP32 math).
+__global__ __launch_bounds__(kBlock) void k_tail_fwd(const bf16* feat, const float* x, const float* w, const float* b,
+                                                     int64_t npx, float* out) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npx) return;
+    float f[32];
+    const uint4* fp = reinterpret_cast<const uint4*>(feat + p * 32);
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = fp[q];
+        const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+        for (int i = 0; i < 8; ++i) f[q * 8 + i] = (float)e[i];
+    }
+    for (int o = 0; o < 3; ++o) {
+        float s = b[o];
+        for (int c = 0; c < 32; ++c) s += w[o * 32 + c] * f[c];
+        out[p * 3 + o] = x[p * 3 + o] + s;
+    }
+}
+
+// dfeat = W_t^T dout (BF16), per-block partial dW_t (3 x 32) and db_t (3).
+__global__ __launch_bounds__(kBlock) void k_tail_bwd(const bf16* feat, const float* dout, const float* w, int64_t npx,
+                                                     bf16* dfeat, float* partial /* [blocks][99] */) {
+    __shared__ float s[kBlock];
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    float f[32], d[3] = {0.f, 0.f, 0.f};
+    const bool ok = p < npx;
+    if (ok) {
+        const uint4* fp = reinterpret_cast<const uint4*>(feat + p * 32);
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = fp[q];
+            const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+            for (int i = 0; i < 8; ++i) f[q * 8 + i] = (float)e[i];
+        }
+        for (int o = 0; o < 3; ++o) d[o] = dout[p * 3 + o];
+        for (int c = 0; c < 32; ++c) {
+            const float g = w[c] * d[0] + w[32 + c] * d[1] + w[64 + c] * d[2];
+            dfeat[p * 32 + c] = (bf16)g;
+        }
+    } else {
+        for (int c = 0; c < 32; ++c) f[c] = 0.f;
+    }
+    // block reductions of d[o] * f[c] and d[o]
+    for (int k = 0; k < 99; ++k) {
+        const int o = k / 33, c = k % 33;
+        s[threadIdx.x] = c < 32 ? d[o] * f[c] : d[o];
+        __syncthreads();
+        for (int st = kBlock / 2; st > 0; st >>= 1) {
+            if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 99 + k] = s[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_tail_reduce(const float* partial, int blocks, float* dw, float* db) {
+    const int k = threadIdx.x;
+    if (k >= 99) return;
+    float s = 0.f;
+    for (int b = 0; b < blocks; ++b) s += partial[(int64_t)b * 99 + k];
+    const int o = k / 33, c = k % 33;
+    if (c < 32) dw[o * 32 + c] = s;
+    else db[o] = s;
+}
+
+lgi_status conv_launch(const ConvArgs& a, int taps, hipStream_t st) {
+    const int Cin = a.C0 + a.C1;
+    if (a.N <= 0 || a.H <= 0 || a.W <= 0 || Cin <= 0 || a.Cout <= 0) return LGI_ERR_INVALID_ARG;
+    if (a.C0 % kCK || a.C1 % kCK || a.Cout % 32 || a.Cs % 32 || a.Cs > a.Cout || (a.C1 && !a.src1))
+        return LGI_ERR_INVALID_ARG;
+    if (a.Cs < a.Cout && !a.out1) return LGI_ERR_INVALID_ARG;
+    const int tiles = cdiv(a.W, kTW) * cdiv(a.H, kTH) * a.N;
+    const bool tn64 = (a.Cout % 64 == 0) && (a.Cs % 64 == 0);
+    if (taps == 9) {
+        if (tn64) hipLaunchKernelGGL((k_conv<64, 3>), dim3(tiles, a.Cout / 64), dim3(kBlock), 0, st, a);
+        else hipLaunchKernelGGL((k_conv<32, 3>), dim3(tiles, a.Cout / 32), dim3(kBlock), 0, st, a);
+    } else if (taps == 1) {
+        if (tn64) hipLaunchKernelGGL((k_conv<64, 1>), dim3(tiles, a.Cout / 64), dim3(kBlock), 0, st, a);
+        else hipLaunchKernelGGL((k_conv<32, 1>), dim3(tiles, a.Cout / 32), dim3(kBlock), 0, st, a);
+    } else {
+        return LGI_ERR_INVALID_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1, const void* w,
+                                const float* bias, int32_t cout, int32_t n, int32_t h, int32_t wd, int32_t relu,
+                                void* out, void* stream) {
+    ConvArgs a{};
+    a.src0 = (const bf16*)src0; a.src1 = (const bf16*)src1; a.C0 = c0; a.C1 = src1 ? c1 : 0;
+    a.wk = (const bf16*)w; a.bias = bias; a.out0 = (bf16*)out; a.out1 = nullptr;
+    a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu;
+    if (!src0 || !w || !out) return LGI_ERR_INVALID_ARG;
+    return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout, int32_t cin,
+                                     int32_t cin_split, int32_t n, int32_t h, int32_t wd, void* dx0, void* dx1,
+                                     void* wt_scratch, void* stream) {
+    if (!dy || !w || !dx0 || !wt_scratch || cin_split <= 0 || cin_split > cin) return LGI_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t total = (int64_t)cout * 9 * cin;
+    hipLaunchKernelGGL(k_flip_weights, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, st, (const bf16*)w, cout, cin, 9,
+                       (bf16*)wt_scratch);
+    ConvArgs a{};
+    a.src0 = (const bf16*)dy; a.C0 = cout; a.src1 = nullptr; a.C1 = 0;
+    a.mask0 = (const bf16*)ymask; a.mask1 = nullptr;
+    a.wk = (const bf16*)wt_scratch; a.bias = nullptr;
+    a.out0 = (bf16*)dx0; a.out1 = (bf16*)dx1; a.Cout = cin; a.Cs = cin_split;
+    a.N = n; a.H = h; a.W = wd; a.relu = 0;
+    return conv_launch(a, 9, st);
+}
+
+lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_t cout, const void* src0, int32_t c0,
+                                       const void* src1, int32_t c1, int32_t n, int32_t h, int32_t wd, float* dw,
+                                       float* db, float* scratch, int64_t scratch_floats, void* stream) {
+    const int cin = c0 + (src1 ? c1 : 0);
+    if (!dy || !src0 || !dw || !scratch || cout % 32 || c0 % 32 || (src1 && c1 % 32)) return LGI_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const int tiles = cdiv(wd, kTW) * cdiv(h, kTH) * n;
+    const int64_t wsize = (int64_t)cout * 9 * cin;
+    // as many pixel splits as the scratch holds, at most one per tile, at most 64
+    int splits = (int)std::min<int64_t>(std::min<int64_t>(64, tiles), scratch_floats / wsize);
+    if (splits < 1) return LGI_ERR_INVALID_ARG;
+    const int per = cdiv(tiles, splits);
+    splits = cdiv(tiles, per);
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(cout / 32, cin / 32, splits), dim3(kWgBlock), 0, st, (const bf16*)dy,
+                       (const bf16*)ymask, cout, (const bf16*)src0, c0, (const bf16*)src1, src1 ? c1 : 0, n, h, wd, per,
+                       scratch);
+    hipLaunchKernelGGL(k_reduce_splits, dim3(cdiv(wsize, kBlock)), dim3(kBlock), 0, st, scratch, wsize, splits, dw, 0);
+    if (db) {
+        const int64_t npx = (int64_t)n * h * wd;
+        const int bsplits = (int)std::min<int64_t>(std::min<int64_t>(256, npx), scratch_floats / cout);
+        const int bper = cdiv(npx, bsplits);
+        const int bs = cdiv(npx, bper);
+        hipLaunchKernelGGL(k_bias_grad, dim3(cdiv(cout, 64), bs), dim3(kBlock), 0, st, (const bf16*)dy,
+                           (const bf16*)ymask, npx, cout, bper, scratch);
+        hipLaunchKernelGGL(k_reduce_splits, dim3(cdiv(cout, kBlock)), dim3(kBlock), 0, st, scratch, (int64_t)cout, bs, db,
+                           0);
+    }
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
+    if (!in || !out || h % 2 || w % 2) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(k_pool_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n,
+                       h, w, c, (bf16*)out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int32_t h, int32_t w, int32_t c,
+                                void* din, void* stream) {
+    if (!in || !dout || !din || h % 2 || w % 2) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(k_pool_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in,
+                       (const bf16*)dout, n, h, w, c, (bf16*)din);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_upsample_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
+    if (!in || !out) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * 2 * h * 2 * w * c;
+    hipLaunchKernelGGL(k_up_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n, h,
+                       w, c, (bf16*)out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_upsample_bwd(const void* dout, int32_t n, int32_t h, int32_t w, int32_t c, void* din, void* stream) {
+    if (!dout || !din) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * h * w * c;
+    hipLaunchKernelGGL(k_up_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)dout, n,
+                       h, w, c, (bf16*)din);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_pack_input(const float* x, int64_t npx, void* out, void* stream) {
+    if (!x || !out) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_pack_input, dim3(cdiv(npx * 32, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, x, npx,
+                       (bf16*)out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_tail_fwd(const void* feat, const float* x, const float* w, const float* b, int64_t npx, float* out,
+                             void* stream) {
+    if (!feat || !x || !w || !b || !out) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_tail_fwd, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)feat, x, w,
+                       b, npx, out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_tail_bwd(const void* feat, const float* dout, const float* w, int64_t npx, void* dfeat, float* dw,
+                             float* db, float* scratch, int64_t scratch_floats, void* stream) {
+    if (!feat || !dout || !w || !dfeat || !dw || !db || !scratch) return LGI_ERR_INVALID_ARG;
+    const int blocks = cdiv(npx, kBlock);
+    if ((int64_t)blocks * 99 > scratch_floats) return LGI_ERR_INVALID_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tail_bwd, dim3(blocks), dim3(kBlock), 0, st, (const bf16*)feat, dout, w, npx, (bf16*)dfeat,
+                       scratch);
+    hipLaunchKernelGGL(k_tail_reduce, dim3(1), dim3(128), 0, st, scratch, blocks, dw, db);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+}  // extern "C"

@@ -88,6 +88,20 @@ SIGNATURES = {
     "lgi_last_render_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "lgi_batch_parse": (C.c_int, [C.c_char_p, C.POINTER(BatchJob), C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_write_png": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
+    # denoiser layers (device pointers + hipStream_t)
+    "lgi_unet_conv3x3_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32,
+                                       C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_conv3x3_bwd_data": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_int32, _P, _P, _P, _P]),
+    "lgi_unet_conv3x3_bwd_weight": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_int32, _P, _P, _P, C.c_int64, _P]),
+    "lgi_unet_maxpool_fwd": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_maxpool_bwd": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_upsample_fwd": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_upsample_bwd": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_pack_input": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "lgi_unet_tail_fwd": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P]),
+    "lgi_unet_tail_bwd": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P, _P, C.c_int64, _P]),
 }
 
 _lib = None

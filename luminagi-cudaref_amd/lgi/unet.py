@@ -1,0 +1,252 @@
+"""Denoiser UNet on the project's HIP kernels (BF16 MFMA convolutions, FP32 accumulation).
+
+Architecture (reference ml/train.py:37-75, in the configuration of this project): encoder
+widths 32/64/128/256 with two 3x3 conv + ReLU per level and 2x2 max-pool between levels, a
+512-wide bottleneck, decoder levels of bilinear 2x upsample + skip concat + two 3x3 conv +
+ReLU, and a 1x1 conv to 3 channels added to the input (residual). Inputs and outputs are
+log1p radiance, NHWC FP32 [N, H, W, 3]; H and W must be multiples of 16.
+
+Every layer is a torch.autograd.Function whose forward and backward call liblgi through its C
+ABI on the current PyTorch stream; activations are BF16 NHWC, master weights FP32.
+"""
+import ctypes as C
+
+import torch
+
+from . import _abi
+from ._abi import LgiError
+
+ENC = (32, 64, 128, 256)
+BOTTLENECK = 512
+
+WGRAD_SCRATCH_MAX = 64 << 20  # floats (256 MB) for weight-gradient partial sums
+
+
+def _call(name, *args):
+    st = getattr(_abi.lib(), name)(*args)
+    if st != 0:
+        raise LgiError(st, name)
+
+
+def _p(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check(t, dtype, channels=None):
+    assert t.is_cuda and t.dtype == dtype and t.is_contiguous(), (t.device, t.dtype, t.is_contiguous())
+    if channels is not None:
+        assert t.shape[-1] == channels, (t.shape, channels)
+
+
+# ------------------------------------------------------------------ ops
+def conv3x3_fwd(src0, src1, w_bf16, bias, relu):
+    n, h, w, c0 = src0.shape
+    c1 = src1.shape[-1] if src1 is not None else 0
+    cout = w_bf16.shape[0]
+    _check(src0, torch.bfloat16)
+    if src1 is not None:
+        _check(src1, torch.bfloat16)
+    out = torch.empty((n, h, w, cout), dtype=torch.bfloat16, device=src0.device)
+    _call("lgi_unet_conv3x3_fwd", _p(src0), c0, _p(src1), c1, _p(w_bf16), _p(bias), cout, n, h, w, int(relu),
+          _p(out), _stream())
+    return out
+
+
+def conv3x3_bwd(dy, y_relu, w_bf16, src0, src1, need_dx=True):
+    """Returns (dx0, dx1, dw fp32 [cout,9,cin], db fp32)."""
+    n, h, w, cout = dy.shape
+    c0 = src0.shape[-1]
+    c1 = src1.shape[-1] if src1 is not None else 0
+    cin = c0 + c1
+    dy = dy.contiguous()
+    dx0 = dx1 = None
+    if need_dx:
+        dx0 = torch.empty((n, h, w, c0), dtype=torch.bfloat16, device=dy.device)
+        dx1 = torch.empty((n, h, w, c1), dtype=torch.bfloat16, device=dy.device) if c1 else None
+        wt = torch.empty(cin * 9 * cout, dtype=torch.bfloat16, device=dy.device)
+        _call("lgi_unet_conv3x3_bwd_data", _p(dy), _p(y_relu), _p(w_bf16), cout, cin, c0, n, h, w, _p(dx0), _p(dx1),
+              _p(wt), _stream())
+    dw = torch.empty((cout, 9, cin), dtype=torch.float32, device=dy.device)
+    db = torch.empty(cout, dtype=torch.float32, device=dy.device)
+    wsize = cout * 9 * cin
+    tiles = ((w + 15) // 16) * ((h + 7) // 8) * n
+    splits = max(1, min(64, tiles, WGRAD_SCRATCH_MAX // wsize))
+    scratch = torch.empty(max(splits * wsize, 256 * cout), dtype=torch.float32, device=dy.device)
+    _call("lgi_unet_conv3x3_bwd_weight", _p(dy), _p(y_relu), cout, _p(src0), c0, _p(src1), c1, n, h, w, _p(dw),
+          _p(db), _p(scratch), scratch.numel(), _stream())
+    return dx0, dx1, dw, db
+
+
+class Conv3x3Fn(torch.autograd.Function):
+    """relu?(conv3x3(cat(src0, src1)) + b); weights FP32 [cout, 9, cin] cast to BF16 per call."""
+
+    @staticmethod
+    def forward(ctx, src0, src1, weight, bias, relu):
+        wb = weight.to(torch.bfloat16).contiguous()
+        y = conv3x3_fwd(src0, src1, wb, bias, relu)
+        ctx.save_for_backward(src0, src1, wb, y)
+        ctx.relu = relu
+        ctx.has_src1 = src1 is not None
+        return y
+
+    @staticmethod
+    def backward(ctx, dy):
+        src0, src1, wb, y = ctx.saved_tensors
+        need_dx = ctx.needs_input_grad[0] or ctx.needs_input_grad[1]
+        dx0, dx1, dw, db = conv3x3_bwd(dy.to(torch.bfloat16), y if ctx.relu else None, wb, src0,
+                                       src1 if ctx.has_src1 else None, need_dx)
+        return dx0, dx1 if ctx.has_src1 else None, dw, db, None
+
+
+class MaxPoolFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        n, h, w, c = x.shape
+        _check(x, torch.bfloat16)
+        out = torch.empty((n, h // 2, w // 2, c), dtype=torch.bfloat16, device=x.device)
+        _call("lgi_unet_maxpool_fwd", _p(x), n, h, w, c, _p(out), _stream())
+        ctx.save_for_backward(x)
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        (x,) = ctx.saved_tensors
+        n, h, w, c = x.shape
+        dout = dout.to(torch.bfloat16).contiguous()
+        din = torch.empty_like(x)
+        _call("lgi_unet_maxpool_bwd", _p(x), _p(dout), n, h, w, c, _p(din), _stream())
+        return din
+
+
+class UpsampleFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        n, h, w, c = x.shape
+        _check(x, torch.bfloat16)
+        out = torch.empty((n, 2 * h, 2 * w, c), dtype=torch.bfloat16, device=x.device)
+        _call("lgi_unet_upsample_fwd", _p(x), n, h, w, c, _p(out), _stream())
+        ctx.shape = (n, h, w, c)
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        n, h, w, c = ctx.shape
+        dout = dout.to(torch.bfloat16).contiguous()
+        din = torch.empty((n, h, w, c), dtype=torch.bfloat16, device=dout.device)
+        _call("lgi_unet_upsample_bwd", _p(dout), n, h, w, c, _p(din), _stream())
+        return din
+
+
+class TailFn(torch.autograd.Function):
+    """out = x + conv1x1(feat) (32 -> 3), FP32 output; x is the (non-differentiated) input."""
+
+    @staticmethod
+    def forward(ctx, feat, x, w, b):
+        n, h, wd, c = feat.shape
+        _check(feat, torch.bfloat16, 32)
+        _check(x, torch.float32, 3)
+        out = torch.empty((n, h, wd, 3), dtype=torch.float32, device=feat.device)
+        _call("lgi_unet_tail_fwd", _p(feat), _p(x), _p(w.contiguous()), _p(b.contiguous()), n * h * wd, _p(out),
+              _stream())
+        ctx.save_for_backward(feat, w)
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        feat, w = ctx.saved_tensors
+        n, h, wd, c = feat.shape
+        npx = n * h * wd
+        dout = dout.float().contiguous()
+        dfeat = torch.empty_like(feat)
+        dw = torch.empty((3, 32), dtype=torch.float32, device=feat.device)
+        db = torch.empty(3, dtype=torch.float32, device=feat.device)
+        blocks = (npx + 255) // 256
+        scratch = torch.empty(blocks * 99, dtype=torch.float32, device=feat.device)
+        _call("lgi_unet_tail_bwd", _p(feat), _p(dout), _p(w.contiguous()), npx, _p(dfeat), _p(dw), _p(db), _p(scratch),
+              scratch.numel(), _stream())
+        return dfeat, dout, dw, db
+
+
+def pack_input(x):
+    n, h, w, _ = x.shape
+    _check(x, torch.float32, 3)
+    out = torch.empty((n, h, w, 32), dtype=torch.bfloat16, device=x.device)
+    _call("lgi_unet_pack_input", _p(x), n * h * w, _p(out), _stream())
+    return out
+
+
+# ------------------------------------------------------------------ model
+def conv_shapes():
+    """(name, cin, cout) of every 3x3 convolution in forward order (cin of the first = 32 padded)."""
+    shapes = []
+    cin = 32
+    for i, c in enumerate(ENC):
+        shapes += [(f"enc{i}a", cin, c), (f"enc{i}b", c, c)]
+        cin = c
+    shapes += [("mid_a", ENC[-1], BOTTLENECK), ("mid_b", BOTTLENECK, BOTTLENECK)]
+    up = BOTTLENECK
+    for i in reversed(range(len(ENC))):
+        c = ENC[i]
+        shapes += [(f"dec{i}a", up + c, c), (f"dec{i}b", c, c)]
+        up = c
+    return shapes
+
+
+class UNet(torch.nn.Module):
+    """Denoiser on log1p radiance: forward(x [N,H,W,3] fp32) -> [N,H,W,3] fp32."""
+
+    def __init__(self, seed=0, recompute=False):
+        super().__init__()
+        g = torch.Generator().manual_seed(seed)
+        self.shapes = conv_shapes()
+        self.weights = torch.nn.ParameterDict()
+        self.biases = torch.nn.ParameterDict()
+        for name, cin, cout in self.shapes:
+            fan_in = 9 * (3 if name == "enc0a" else cin)
+            w = torch.randn(cout, 9, cin, generator=g) * (2.0 / fan_in) ** 0.5  # He init
+            if name == "enc0a":
+                w[:, :, 3:] = 0.0  # padded input channels
+            self.weights[name] = torch.nn.Parameter(w)
+            self.biases[name] = torch.nn.Parameter(torch.zeros(cout))
+        self.tail_w = torch.nn.Parameter(torch.randn(3, 32, generator=g) * (1.0 / 32) ** 0.5 * 0.1)
+        self.tail_b = torch.nn.Parameter(torch.zeros(3))
+        self.recompute = recompute
+
+    def conv(self, name, a, b=None, relu=True):
+        return Conv3x3Fn.apply(a, b, self.weights[name], self.biases[name], relu)
+
+    def block(self, prefix, a, b=None):
+        y = self.conv(prefix + "a", a, b)
+        return self.conv(prefix + "b", y)
+
+    def _block(self, prefix, a, b=None):
+        if self.recompute and torch.is_grad_enabled():
+            from torch.utils.checkpoint import checkpoint
+            return checkpoint(self.block, prefix, a, b, use_reentrant=False)
+        return self.block(prefix, a, b)
+
+    def forward(self, x, taps=None):
+        """taps (optional dict) receives every level's activation for per-layer error reports."""
+        h = pack_input(x.contiguous())
+        skips = []
+        for i in range(len(ENC)):
+            h = self._block(f"enc{i}", h)
+            skips.append(h)
+            if taps is not None:
+                taps[f"enc{i}"] = h
+            h = MaxPoolFn.apply(h)
+        h = self._block("mid_", h)
+        if taps is not None:
+            taps["mid"] = h
+        for i in reversed(range(len(ENC))):
+            h = self._block(f"dec{i}", UpsampleFn.apply(h), skips[i])
+            if taps is not None:
+                taps[f"dec{i}"] = h
+        return TailFn.apply(h, x.contiguous(), self.tail_w, self.tail_b)
+
+    def state_for_checkpoint(self):
+        return {k: v.detach().cpu() for k, v in self.state_dict().items()}

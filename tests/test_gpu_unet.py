@@ -1,0 +1,251 @@
+"""Denoiser kernels against float64 CPU references (torch.nn.functional) and finite
+differences. BF16 operands, FP32 accumulation: forward and data-gradient outputs are BF16
+(relative rounding 2^-9), so the bar is stated as a relative error on the tensor's scale."""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import unet_ref
+
+pytestmark = pytest.mark.gpu
+
+BF16_TOL = 2e-2   # max |err| / max |ref| for BF16 outputs of FP32-accumulated sums
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
+
+
+def bf16_rand(*shape, scale=1.0, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(*shape, generator=g) * scale).to(torch.bfloat16)
+
+
+def ref_conv_nhwc(x0, x1, w, b, relu):
+    x = x0.double() if x1 is None else torch.cat([x0.double(), x1.double()], -1)
+    y = F.conv2d(x.permute(0, 3, 1, 2), unet_ref.torch_weight(w.double()), None if b is None else b.double(), padding=1)
+    if relu:
+        y = F.relu(y)
+    return y.permute(0, 2, 3, 1)
+
+
+CONV_CASES = [  # n, h, w, c0, c1, cout, relu
+    (2, 16, 32, 64, 0, 64, True),
+    (1, 12, 20, 32, 32, 32, True),      # ragged tiles + fused concat
+    (1, 8, 16, 128, 64, 64, False),
+    (2, 4, 6, 256, 0, 96, True),        # image smaller than one tile, Cout not a multiple of 64
+]
+
+
+@pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_forward(gpu, case):
+    from lgi import unet
+    n, h, w, c0, c1, cout, relu = case
+    x0 = bf16_rand(n, h, w, c0, seed=1)
+    x1 = bf16_rand(n, h, w, c1, seed=2) if c1 else None
+    wt = bf16_rand(cout, 9, c0 + c1, scale=(2.0 / (9 * (c0 + c1))) ** 0.5, seed=3)
+    b = torch.randn(cout, generator=torch.Generator().manual_seed(4)) * 0.1
+    y = unet.conv3x3_fwd(x0.cuda(), None if x1 is None else x1.cuda(), wt.cuda(), b.cuda(), relu)
+    ref = ref_conv_nhwc(x0, x1, wt, b, relu)
+    e = rel(y, ref)
+    print(f"conv fwd {case}: rel err {e:.2e}", flush=True)
+    assert y.shape == ref.shape and e < BF16_TOL
+
+
+@pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_backward(gpu, case):
+    from lgi import unet
+    n, h, w, c0, c1, cout, relu = case
+    x0 = bf16_rand(n, h, w, c0, seed=5)
+    x1 = bf16_rand(n, h, w, c1, seed=6) if c1 else None
+    wt = bf16_rand(cout, 9, c0 + c1, scale=(2.0 / (9 * (c0 + c1))) ** 0.5, seed=7)
+    b = torch.randn(cout, generator=torch.Generator().manual_seed(8)) * 0.1
+    dy = bf16_rand(n, h, w, cout, seed=9)
+    y = unet.conv3x3_fwd(x0.cuda(), None if x1 is None else x1.cuda(), wt.cuda(), b.cuda(), relu)
+    dx0, dx1, dw, db = unet.conv3x3_bwd(dy.cuda(), y if relu else None, wt.cuda(), x0.cuda(),
+                                        None if x1 is None else x1.cuda())
+    # float64 reference; the ReLU mask comes from the kernel's own BF16 output so that ties at
+    # exactly zero are decided the same way
+    xs = [x0.double().requires_grad_()] + ([x1.double().requires_grad_()] if c1 else [])
+    wd = wt.double().requires_grad_()
+    bd = b.double().requires_grad_()
+    z = ref_conv_nhwc(xs[0], xs[1] if c1 else None, wd, bd, False)
+    mask = (y.cpu().double() > 0).double() if relu else torch.ones_like(z)
+    (z * mask * dy.double()).sum().backward()
+    errs = {"dx0": rel(dx0, xs[0].grad), "dw": rel(dw, wd.grad), "db": rel(db, bd.grad)}
+    if c1:
+        errs["dx1"] = rel(dx1, xs[1].grad)
+    print(f"conv bwd {case}: {errs}", flush=True)
+    assert errs["dx0"] < BF16_TOL and errs.get("dx1", 0) < BF16_TOL
+    assert errs["dw"] < 1e-2 and errs["db"] < 1e-2   # FP32 outputs of BF16 products
+
+
+def test_maxpool_forward_backward(gpu):
+    from lgi import unet
+    x = bf16_rand(2, 8, 12, 64, seed=10)
+    x[0, 0, 0, :] = x[0, 0, 1, :]  # exact ties: gradient goes to the first maximum
+    y = unet.MaxPoolFn.apply(x.cuda())
+    ref = F.max_pool2d(x.double().permute(0, 3, 1, 2), 2).permute(0, 2, 3, 1)
+    assert torch.equal(y.cpu().double(), ref)
+    dy = bf16_rand(2, 4, 6, 64, seed=11)
+    xd = x.double().requires_grad_()
+    (F.max_pool2d(xd.permute(0, 3, 1, 2), 2).permute(0, 2, 3, 1) * dy.double()).sum().backward()
+    xc = x.cuda().requires_grad_()
+    unet.MaxPoolFn.apply(xc).backward(dy.cuda())
+    assert torch.equal(xc.grad.cpu().double(), xd.grad)
+
+
+def test_upsample_forward_backward(gpu):
+    from lgi import unet
+    x = bf16_rand(2, 5, 7, 32, seed=12)
+    y = unet.UpsampleFn.apply(x.cuda())
+    ref = unet_ref.up(x.double().permute(0, 3, 1, 2)).permute(0, 2, 3, 1)
+    assert rel(y, ref) < 1e-2
+    dy = bf16_rand(2, 10, 14, 32, seed=13)
+    xd = x.double().requires_grad_()
+    (unet_ref.up(xd.permute(0, 3, 1, 2)).permute(0, 2, 3, 1) * dy.double()).sum().backward()
+    xc = x.cuda().requires_grad_()
+    unet.UpsampleFn.apply(xc).backward(dy.cuda())
+    assert rel(xc.grad, xd.grad) < 1e-2
+
+
+def test_tail_forward_backward(gpu):
+    from lgi import unet
+    feat = bf16_rand(1, 8, 16, 32, seed=14)
+    x = torch.rand(1, 8, 16, 3)
+    w = torch.randn(3, 32) * 0.2
+    b = torch.randn(3) * 0.1
+    fc, wc, bc = feat.cuda().requires_grad_(), w.cuda().requires_grad_(), b.cuda().requires_grad_()
+    out = unet.TailFn.apply(fc, x.cuda(), wc, bc)
+    fd, wd, bd = feat.double().requires_grad_(), w.double().requires_grad_(), b.double().requires_grad_()
+    ref = x.double() + torch.einsum("nhwc,oc->nhwo", fd, wd) + bd
+    assert rel(out, ref) < 1e-5
+    g = torch.randn(1, 8, 16, 3)
+    out.backward(g.cuda())
+    (ref * g.double()).sum().backward()
+    assert rel(fc.grad, fd.grad) < BF16_TOL and rel(wc.grad, wd.grad) < 1e-5 and rel(bc.grad, bd.grad) < 1e-5
+
+
+def make_model_and_input(seed=0, n=1, h=32, w=32):
+    from lgi import unet
+    model = unet.UNet(seed=seed).cuda()
+    g = torch.Generator().manual_seed(seed + 100)
+    x = torch.log1p(torch.rand(n, h, w, 3, generator=g) * 2.0)
+    return model, x
+
+
+def test_unet_forward_per_layer_error_vs_fp64(gpu):
+    """Per-level error of the BF16 MFMA forward against the FP64 CPU forward of the same
+    (BF16-representable) weights."""
+    model, x = make_model_and_input(seed=1, n=2, h=32, w=64)
+    with torch.no_grad():
+        for p in model.parameters():
+            p.copy_(p.to(torch.bfloat16).float())
+    taps, rtaps = {}, {}
+    with torch.no_grad():
+        y = model(x.cuda(), taps=taps)
+    ref = unet_ref.forward(model.state_dict(), x, taps=rtaps)
+    report = {k: rel(taps[k], rtaps[k]) for k in rtaps}
+    report["output"] = rel(y, ref)
+    report["output_minus_input"] = rel(y - x.cuda(), ref - x.double())
+    print("per-layer max-rel error vs fp64:", {k: f"{v:.2e}" for k, v in report.items()}, flush=True)
+    assert all(v < 5e-2 for k, v in report.items() if k != "output_minus_input")
+    assert report["output_minus_input"] < 5e-2
+
+
+def test_unet_backward_vs_fp64_and_finite_differences(gpu):
+    from lgi import unet
+    model, x = make_model_and_input(seed=2, n=1, h=32, w=32)
+    with torch.no_grad():
+        for p in model.parameters():
+            p.copy_(p.to(torch.bfloat16).float())
+    g = torch.Generator().manual_seed(7)
+    target = torch.randn(1, 32, 32, 3, generator=g)
+    xc, tc = x.cuda(), target.cuda()
+    out = model(xc)
+    loss = (out * tc).sum()
+    loss.backward()
+    # Float64 autograd of the reference network, twice: (a) plain FP64 activations, (b) FP64
+    # math with every stored activation rounded to BF16 (straight-through), as the GPU stores
+    # them. Against (a) the BF16 network differs mostly where a BF16 activation lands on the
+    # other side of a ReLU (few pixels at the 2x2 / 4x4 levels of a 32x32 crop), so (a) gets a
+    # direction bar; (b) follows the same masks and is held to an L2 bar of a few BF16 ulps per
+    # layer (the two computations round independently, so their activations drift apart by
+    # about one BF16 ulp per layer over the 18 convolutions).
+    named = dict(model.named_parameters())
+    for bf16_acts, max_l2, min_cos in ((False, 0.25, 0.97), (True, 0.06, 0.998)):
+        params = {k: v.detach().cpu().double().requires_grad_() for k, v in model.state_dict().items()}
+        rloss = (unet_ref.forward(params, x, bf16_activations=bf16_acts) * target.double()).sum()
+        rloss.backward()
+        worst, l2 = {}, {}
+        for k, p in params.items():
+            gd = named[k].grad.detach().cpu().double()
+            if k == "weights.enc0a":
+                gd, pg = gd[:, :, :3], p.grad[:, :, :3]
+            else:
+                pg = p.grad
+            worst[k] = rel(gd, pg)
+            l2[k] = float((gd - pg).norm() / pg.norm().clamp_min(1e-30))
+            cos = float((gd * pg).sum() / (gd.norm() * pg.norm()).clamp_min(1e-30))
+            assert cos > min_cos, (bf16_acts, k, cos)
+        tag = "bf16-activation fp64" if bf16_acts else "fp64"
+        print(f"grad max-rel error vs {tag}:", {k: f"{v:.1e}" for k, v in sorted(worst.items())}, flush=True)
+        print(f"grad rel-L2 error vs {tag}:", {k: f"{v:.1e}" for k, v in sorted(l2.items())}, flush=True)
+        assert max(l2.values()) < max_l2, tag
+
+
+def test_conv_gradients_by_finite_differences(gpu):
+    """Central differences of the GPU forward kernel against the GPU gradient kernels, on a
+    32x32 crop. Without ReLU, L(W) = sum(conv(X, W) * G) is linear in W and in X, so the
+    central difference along a BF16 direction V with a power-of-two step is exact up to the
+    BF16 rounding of the outputs; steps are taken from W = 0 (resp. X = 0) so every perturbed
+    operand is exactly representable."""
+    from lgi import unet
+    n, h, w, c0, c1, cout = 1, 32, 32, 64, 32, 64
+    X0, X1 = bf16_rand(n, h, w, c0, seed=20).cuda(), bf16_rand(n, h, w, c1, seed=21).cuda()
+    Wt = bf16_rand(cout, 9, c0 + c1, scale=0.05, seed=22).cuda()
+    G = bf16_rand(n, h, w, cout, seed=23).cuda()
+    V = bf16_rand(cout, 9, c0 + c1, scale=0.05, seed=24).cuda()
+    VX0, VX1 = bf16_rand(n, h, w, c0, seed=25).cuda(), bf16_rand(n, h, w, c1, seed=26).cuda()
+
+    def L(x0, x1, wt):
+        return float((unet.conv3x3_fwd(x0, x1, wt, None, False).double() * G.double()).sum())
+
+    dx0, dx1, dw, _ = unet.conv3x3_bwd(G, None, Wt, X0, X1)
+    eps = 2.0 ** -3
+    zero_w = torch.zeros_like(Wt)
+    fd_w = (L(X0, X1, (zero_w + eps * V.float()).to(torch.bfloat16)) -
+            L(X0, X1, (zero_w - eps * V.float()).to(torch.bfloat16))) / (2 * eps)
+    an_w = float((dw.double() * V.double()).sum())
+    zx0, zx1 = torch.zeros_like(X0), torch.zeros_like(X1)
+    fd_x = (L((eps * VX0.float()).to(torch.bfloat16), (eps * VX1.float()).to(torch.bfloat16), Wt) -
+            L((-eps * VX0.float()).to(torch.bfloat16), (-eps * VX1.float()).to(torch.bfloat16), Wt)) / (2 * eps)
+    an_x = float((dx0.double() * VX0.double()).sum() + (dx1.double() * VX1.double()).sum())
+    print(f"finite differences: dW {fd_w:.6g} vs {an_w:.6g}; dX {fd_x:.6g} vs {an_x:.6g}", flush=True)
+    # the forward's outputs are rounded to BF16 (2^-9 relative); summed against random G
+    # that leaves ~0.2-0.4% on the difference quotient
+    assert abs(fd_w - an_w) <= 1e-2 * abs(an_w)
+    assert abs(fd_x - an_x) <= 1e-2 * abs(an_x)
+    # pooling is piecewise linear: the four values of every window are distinct on a 2/16 grid,
+    # so steps of 1/64 cannot change which element is the maximum
+    g = torch.Generator().manual_seed(27)
+    base = torch.randint(-20, 21, (1, 16, 16, 32, 1), generator=g).float()
+    offs = torch.argsort(torch.rand(1, 16, 16, 32, 4, generator=g), dim=-1).float() * 2
+    win = (base + offs) / 16.0                                  # [1, 16, 16, 32, 4] (dy, dx) row-major
+    Xp = win.reshape(1, 16, 16, 32, 2, 2).permute(0, 1, 4, 2, 5, 3).reshape(1, 32, 32, 32)
+    Xp = Xp.to(torch.bfloat16).cuda()
+    Vp = torch.randint(-1, 2, (1, 32, 32, 32), generator=g).to(torch.bfloat16).cuda()
+    Gp = bf16_rand(1, 16, 16, 32, seed=28).cuda()
+    Xr = Xp.clone().requires_grad_()
+    unet.MaxPoolFn.apply(Xr).backward(Gp)
+
+    def Lp(x):
+        return float((unet.MaxPoolFn.apply(x).double() * Gp.double()).sum())
+    fd_p = (Lp((Xp.float() + Vp.float() / 64).to(torch.bfloat16)) -
+            Lp((Xp.float() - Vp.float() / 64).to(torch.bfloat16))) * 32
+    an_p = float((Xr.grad.double() * Vp.double()).sum())
+    print(f"finite differences: maxpool {fd_p:.6g} vs {an_p:.6g}", flush=True)
+    assert abs(fd_p - an_p) <= 1e-2 * max(1.0, abs(an_p))
