@@ -165,6 +165,31 @@ LGI_API lgi_status lgi_gbuffer(lgi_ctx* ctx, const lgi_camera* cam, uint8_t* d_a
 /* Kernel time of the last render call (ms, HIP events). */
 LGI_API lgi_status lgi_last_render_ms(const lgi_ctx* ctx, float* ms);
 
+/* ---- many poses / selected tiles in one persistent launch ----
+ * No reference counterpart beyond the --batch loop (src/main.cpp:158-189), which re-uploads the
+ * scene and launches once per job. Work is split into (pose, 8x8-pixel tile, spp, first
+ * sample) items on a device queue; one persistent kernel drains it. */
+typedef struct lgi_work_item {
+    int32_t pose;     /* index into the camera list */
+    int32_t tile;     /* 8x8 tile, row-major over ceil(W/8) x ceil(H/8) */
+    int32_t spp;
+    int32_t sample0;  /* first sample index (PCG32 stream per (pixel, sample)) */
+} lgi_work_item;
+typedef struct lgi_pose_job {
+    lgi_camera cam;   /* all poses share the first pose's width and height */
+    int32_t spp;
+    int32_t sample0;
+} lgi_pose_job;
+/* Renders every pose into d_sum/d_comp ([n][W*H*3] f32) and d_sumsq ([n][W*H], nullable).
+ * use_graph: capture the launch in a hipGraph on first use and replay it for repeated calls
+ * with the same jobs. */
+LGI_API lgi_status lgi_render_batch(lgi_ctx* ctx, const lgi_pose_job* jobs, int32_t n, int32_t bounces,
+                                    uint32_t flags, float* d_sum, float* d_comp, float* d_sumsq, int32_t use_graph);
+/* Adds spp samples (from sample0) to the listed 8x8 tiles of one image (host tile list). */
+LGI_API lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam, const int32_t* tiles, int32_t n_tiles,
+                                    int32_t spp, int32_t sample0, int32_t bounces, uint32_t flags, float* d_sum,
+                                    float* d_comp, float* d_sumsq);
+
 /* ---- batch (read_batch_file, src/main.cpp:61-91) ---- */
 LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs);
 

@@ -130,6 +130,64 @@ lgi_status attach_spill(lgi_ctx* ctx, TraceParams& P, int64_t threads) {
     return LGI_OK;
 }
 
+constexpr int kQueueGrid = 256 * 8;  // one resident round of 256-thread blocks on 256 CUs
+
+// Uploads the work items and cameras and drains them with the persistent queue kernel,
+// directly or through a cached hipGraph (captured on first use of identical inputs).
+lgi_status run_queue(lgi_ctx* ctx, const std::vector<lgi_work_item>& items, const std::vector<DevCamera>& cams,
+                     const lgi_camera& cam0, int bounces, uint32_t flags, float* sum, float* comp, float* sumsq,
+                     bool use_graph) {
+    lgi_status st;
+    TraceParams P = make_params(ctx, cam0, 0, bounces, flags, 0);
+    P.sum = sum;
+    P.comp = comp;
+    P.sumsq = sumsq;
+    if ((st = attach_spill(ctx, P, (int64_t)kQueueGrid * 256)) != LGI_OK) return st;
+    auto* d_items = (lgi_work_item*)ctx->scratch_get(45, std::max<size_t>(1, items.size()) * sizeof(lgi_work_item));
+    auto* d_cams = (DevCamera*)ctx->scratch_get(46, std::max<size_t>(1, cams.size()) * sizeof(DevCamera));
+    auto* counter = (uint32_t*)ctx->scratch_get(47, 64);
+    if (!d_items || !d_cams || !counter) return lgi_set_err(ctx, LGI_ERR_HIP, "queue allocation failed");
+    // key of the captured graph: everything the kernels read by value or address
+    std::string key((const char*)items.data(), items.size() * sizeof(lgi_work_item));
+    key.append((const char*)cams.data(), cams.size() * sizeof(DevCamera));
+    key.append((const char*)&P, sizeof(P));
+    for (const void* p : {(const void*)d_items, (const void*)d_cams, (const void*)counter})
+        key.append((const char*)&p, sizeof(p));
+    const int grid = std::max(1, std::min<int>(kQueueGrid, (int)items.size()));
+    LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (use_graph && ctx->graph_exec && ctx->graph_key == key) {
+        LGI_HIP(ctx, hipGraphLaunch((hipGraphExec_t)ctx->graph_exec, ctx->stream));
+    } else {
+        LGI_HIP(ctx, hipMemcpy(d_items, items.data(), items.size() * sizeof(lgi_work_item), hipMemcpyHostToDevice));
+        LGI_HIP(ctx, hipMemcpy(d_cams, cams.data(), cams.size() * sizeof(DevCamera), hipMemcpyHostToDevice));
+        if (!use_graph) {
+            lgi_launch_render_queue(P, d_cams, d_items, (int)items.size(), counter, grid, ctx->stream);
+        } else {
+            if (!ctx->cap_stream) LGI_HIP(ctx, hipStreamCreateWithFlags((hipStream_t*)&ctx->cap_stream,
+                                                                        hipStreamNonBlocking));
+            hipStream_t cs = (hipStream_t)ctx->cap_stream;
+            if (ctx->graph_exec) {
+                (void)hipGraphExecDestroy((hipGraphExec_t)ctx->graph_exec);
+                ctx->graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            LGI_HIP(ctx, hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+            lgi_launch_render_queue(P, d_cams, d_items, (int)items.size(), counter, grid, cs);
+            LGI_HIP(ctx, hipStreamEndCapture(cs, &g));
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return lgi_set_err(ctx, LGI_ERR_HIP, "hipGraphInstantiate failed");
+            ctx->graph_exec = exec;
+            ctx->graph_key = key;
+            LGI_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+        }
+    }
+    LGI_HIP(ctx, hipGetLastError());
+    LGI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return LGI_OK;
+}
+
 lgi_status check_render_ready(lgi_ctx* ctx, const lgi_camera& cam) {
     if (!ctx) return LGI_ERR_INVALID_ARG;
     if (ctx->scene.tris.empty()) return lgi_set_err(ctx, LGI_ERR_NO_SCENE, "no scene");
@@ -167,6 +225,8 @@ void lgi_destroy(lgi_ctx* ctx) {
             if (e.first) (void)hipFree(e.first);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        if (ctx->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->graph_exec);
+        if (ctx->cap_stream) (void)hipStreamDestroy((hipStream_t)ctx->cap_stream);
     }
     delete ctx;
 }
@@ -440,6 +500,53 @@ lgi_status lgi_render(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t spp, int32
         LGI_HIP(ctx, hipMemcpy(wp, g + npx * 6, npx * 3, hipMemcpyDeviceToHost));
     }
     return LGI_OK;
+}
+
+lgi_status lgi_render_batch(lgi_ctx* ctx, const lgi_pose_job* jobs, int32_t n, int32_t bounces, uint32_t flags,
+                            float* d_sum, float* d_comp, float* d_sumsq, int32_t use_graph) {
+    if (!ctx || !jobs || n <= 0 || !d_sum || !d_comp || bounces < 0 || bounces > 64) return LGI_ERR_INVALID_ARG;
+    const lgi_camera& cam0 = jobs[0].cam;
+    lgi_status st = check_render_ready(ctx, cam0);
+    if (st != LGI_OK) return st;
+    if (flags & (LGI_RENDER_SORTED | LGI_RNG_XORSHIFT_REF))
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "the batch queue renders the PCG32 inline path");
+    const int tiles = ((cam0.width + 7) / 8) * ((cam0.height + 7) / 8);
+    std::vector<DevCamera> cams(n);
+    std::vector<lgi_work_item> items;
+    items.reserve((size_t)tiles * n);
+    for (int p = 0; p < n; ++p) {
+        if (jobs[p].cam.width != cam0.width || jobs[p].cam.height != cam0.height || jobs[p].spp < 0 ||
+            jobs[p].sample0 < 0)
+            return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "batch poses must share the image size");
+        cams[p] = lgi_make_dev_camera(jobs[p].cam);
+        for (int t = 0; t < tiles && jobs[p].spp > 0; ++t) items.push_back({p, t, jobs[p].spp, jobs[p].sample0});
+    }
+    // longest items first: the queue then ends on short items (less idle tail)
+    std::stable_sort(items.begin(), items.end(),
+                     [](const lgi_work_item& a, const lgi_work_item& b) { return a.spp > b.spp; });
+    if (items.empty()) return LGI_OK;
+    return run_queue(ctx, items, cams, cam0, bounces, flags, d_sum, d_comp, d_sumsq, use_graph != 0);
+}
+
+lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam_in, const int32_t* tiles, int32_t n_tiles, int32_t spp,
+                            int32_t sample0, int32_t bounces, uint32_t flags, float* d_sum, float* d_comp,
+                            float* d_sumsq) {
+    if (!ctx || (n_tiles > 0 && !tiles) || n_tiles < 0 || spp < 0 || sample0 < 0 || !d_sum || !d_comp)
+        return LGI_ERR_INVALID_ARG;
+    const lgi_camera& cam = cam_in ? *cam_in : ctx->scene.camera;
+    lgi_status st = check_render_ready(ctx, cam);
+    if (st != LGI_OK) return st;
+    if (flags & (LGI_RENDER_SORTED | LGI_RNG_XORSHIFT_REF))
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "tile renders use the PCG32 inline path");
+    const int ntiles = ((cam.width + 7) / 8) * ((cam.height + 7) / 8);
+    std::vector<lgi_work_item> items;
+    for (int i = 0; i < n_tiles; ++i) {
+        if (tiles[i] < 0 || tiles[i] >= ntiles) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "tile index out of range");
+        if (spp > 0) items.push_back({0, tiles[i], spp, sample0});
+    }
+    if (items.empty()) return LGI_OK;
+    std::vector<DevCamera> cams{lgi_make_dev_camera(cam)};
+    return run_queue(ctx, items, cams, cam, bounces, flags, d_sum, d_comp, d_sumsq, false);
 }
 
 lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs) {

@@ -85,6 +85,9 @@ struct lgi_ctx {
     float last_build_ms = 0.f;
     float last_render_ms = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* cap_stream = nullptr;  // hipStream_t used for graph capture
+    void* graph_exec = nullptr;  // hipGraphExec_t of the last captured batch launch
+    std::string graph_key;       // inputs the captured graph was built for
     // scratch buffers (grow-only)
     std::vector<std::pair<void*, size_t>> scratch;
     void* scratch_get(int slot, size_t bytes);

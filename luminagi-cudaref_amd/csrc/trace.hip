@@ -120,21 +120,19 @@ __device__ __forceinline__ void tile_pixel(int bx, int by, int tid, int& px, int
 }
 
 // accumulate_kernel (src/pathtracer.cu:333-364): spp samples per pixel, Kahan-compensated.
-__global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
-    __shared__ int s_stack[kStackLds * kBlock];
-    int px, py;
-    tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
-    if (px >= P.cam.width || py >= P.cam.height) return;
-    const TravStack stk = make_stack(P, s_stack);
-    const uint32_t pix = (uint32_t)py * (uint32_t)P.cam.width + (uint32_t)px;
-    float s0 = P.sum[3 * pix], s1 = P.sum[3 * pix + 1], s2 = P.sum[3 * pix + 2];
-    float c0 = P.comp[3 * pix], c1 = P.comp[3 * pix + 1], c2 = P.comp[3 * pix + 2];
-    float sq = P.sumsq ? P.sumsq[pix] : 0.0f;
+// Samples [sample0, sample0 + spp) of pixel (px, py) through `cam`, Kahan-added to the
+// pixel's sum/comp (and squared luminance to sumsq) in sample order.
+__device__ void render_pixel(const TraceParams& P, const DevCamera& cam, int px, int py, int spp, int sample0,
+                             float* sum, float* comp, float* sumsq, const TravStack& stk) {
+    const uint32_t pix = (uint32_t)py * (uint32_t)cam.width + (uint32_t)px;
+    float s0 = sum[3 * pix], s1 = sum[3 * pix + 1], s2 = sum[3 * pix + 2];
+    float c0 = comp[3 * pix], c1 = comp[3 * pix + 1], c2 = comp[3 * pix + 2];
+    float sq = sumsq ? sumsq[pix] : 0.0f;
     const bool xs = (P.flags & LGI_RNG_XORSHIFT_REF) != 0;
     Rng rng;
     rng.pcg = !xs;
-    for (int k = 0; k < P.spp; ++k) {
-        const int sample = P.sample0 + k;
+    for (int k = 0; k < spp; ++k) {
+        const int sample = sample0 + k;
         if (xs) {
             // src/pathtracer.cu:349 with 8-sample launch chunks (:699-702)
             if (k == 0 || (sample & 7) == 0) {
@@ -146,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
         }
         float jx = rng.next();
         float jy = rng.next();
-        Ray r = primary_ray(P.cam, px, py, jx, jy);
+        Ray r = primary_ray(cam, px, py, jx, jy);
         V3 L = trace_path(P, r, rng, stk);
         kahan_add(s0, c0, L.x);
         kahan_add(s1, c1, L.y);
@@ -154,10 +152,46 @@ __global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
         float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
         sq += lum * lum;
     }
-    P.sum[3 * pix] = s0; P.sum[3 * pix + 1] = s1; P.sum[3 * pix + 2] = s2;
-    P.comp[3 * pix] = c0; P.comp[3 * pix + 1] = c1; P.comp[3 * pix + 2] = c2;
-    if (P.sumsq) P.sumsq[pix] = sq;
+    sum[3 * pix] = s0; sum[3 * pix + 1] = s1; sum[3 * pix + 2] = s2;
+    comp[3 * pix] = c0; comp[3 * pix + 1] = c1; comp[3 * pix + 2] = c2;
+    if (sumsq) sumsq[pix] = sq;
 }
+
+__global__ __launch_bounds__(kBlock) void k_render(TraceParams P) {
+    __shared__ int s_stack[kStackLds * kBlock];
+    int px, py;
+    tile_pixel(blockIdx.x, blockIdx.y, threadIdx.x, px, py);
+    if (px >= P.cam.width || py >= P.cam.height) return;
+    render_pixel(P, P.cam, px, py, P.spp, P.sample0, P.sum, P.comp, P.sumsq, make_stack(P, s_stack));
+}
+
+// Persistent multi-pose renderer: every wave takes (pose, 8x8 tile, spp, first sample) work
+// items from a device queue (one atomic per item) until the queue is empty, so poses with
+// uneven spp budgets share one launch and no wave idles while items remain. Per-pose
+// accumulators are [pose][W*H*3] (sum, comp) and [pose][W*H] (sumsq).
+__global__ __launch_bounds__(kBlock) void k_render_queue(TraceParams P, const DevCamera* cams,
+                                                         const lgi_work_item* items, int n_items, uint32_t* counter) {
+    __shared__ int s_stack[kStackLds * kBlock];
+    const TravStack stk{s_stack + threadIdx.x, P.spill + blockIdx.x * kBlock + threadIdx.x, P.spill_stride};
+    const int lane = threadIdx.x & 63;
+    const int W = P.cam.width, H = P.cam.height;
+    const int tiles_x = (W + 7) / 8;
+    const int64_t npx = (int64_t)W * H;
+    for (;;) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(counter, 1u);
+        idx = __shfl(idx, 0, 64);
+        if (idx >= (uint32_t)n_items) break;
+        const lgi_work_item it = items[idx];
+        const int px = (it.tile % tiles_x) * 8 + (lane & 7), py = (it.tile / tiles_x) * 8 + (lane >> 3);
+        if (px < W && py < H)
+            render_pixel(P, cams[it.pose], px, py, it.spp, it.sample0, P.sum + it.pose * npx * 3,
+                         P.comp + it.pose * npx * 3, P.sumsq ? P.sumsq + it.pose * npx : nullptr, stk);
+    }
+}
+
+// Zeroes the queue counter inside a captured graph (no memset nodes in replayed graphs).
+__global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
 // Pixel-centre primary hits: original triangle index and t (src/pathtracer.cu:381-384).
 __global__ __launch_bounds__(kBlock) void k_primary(TraceParams P, int32_t* tri_out, float* t_out) {
@@ -218,6 +252,11 @@ __global__ __launch_bounds__(kBlock) void k_tonemap(const float* sum, int64_t n,
 void lgi_launch_render(const TraceParams& P, hipStream_t st) {
     dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);
     hipLaunchKernelGGL(k_render, grid, dim3(kBlock), 0, st, P);
+}
+void lgi_launch_render_queue(const TraceParams& P, const DevCamera* cams, const lgi_work_item* items, int n_items,
+                             uint32_t* counter, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(k_render_queue, dim3(grid), dim3(kBlock), 0, st, P, cams, items, n_items, counter);
 }
 void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_t st) {
     dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);

@@ -37,5 +37,8 @@ void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_
 void lgi_launch_gbuffer(const TraceParams& P, float3 bbMin, float3 invExt, uint8_t* alb, uint8_t* nrm, uint8_t* wp,
                         hipStream_t st);
 void lgi_launch_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out, hipStream_t st);
+// Persistent queue renderer over (pose, 8x8 tile, spp, sample0) items; resets the counter first.
+void lgi_launch_render_queue(const TraceParams& P, const DevCamera* cams, const lgi_work_item* items, int n_items,
+                             uint32_t* counter, int grid, hipStream_t st);
 // Sorted wavefront renderer (wavefront.hip); bb = scene box used by the ray sort key.
 lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max);

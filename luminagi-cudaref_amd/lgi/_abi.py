@@ -43,6 +43,10 @@ class Material(C.Structure):
     _fields_ = [("albedo", C.c_float * 3), ("emissive", C.c_float * 3)]
 
 
+class PoseJob(C.Structure):
+    _fields_ = [("cam", Camera), ("spp", C.c_int32), ("sample0", C.c_int32)]
+
+
 class BatchJob(C.Structure):
     _fields_ = [("out_png", C.c_char * 512), ("spp", C.c_int32), ("bounces", C.c_int32),
                 ("cam_pos", C.c_float * 3), ("cam_tgt", C.c_float * 3)]
@@ -86,6 +90,10 @@ SIGNATURES = {
     "lgi_tonemap": (C.c_int, [_P, _P, C.c_int32, C.c_float, _P]),
     "lgi_gbuffer": (C.c_int, [_P, C.POINTER(Camera), _P, _P, _P]),
     "lgi_last_render_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "lgi_render_batch": (C.c_int, [_P, C.POINTER(PoseJob), C.c_int32, C.c_int32, C.c_uint32, _P, _P, _P,
+                                    C.c_int32]),
+    "lgi_render_tiles": (C.c_int, [_P, C.POINTER(Camera), _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
+                                    _P, _P, _P]),
     "lgi_batch_parse": (C.c_int, [C.c_char_p, C.POINTER(BatchJob), C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_write_png": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
     # denoiser layers (device pointers + hipStream_t)

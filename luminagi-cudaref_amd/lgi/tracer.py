@@ -163,6 +163,23 @@ class Context:
         self._c(lib().lgi_render_accumulate(self._h, C.byref(cam), spp, bounces, flags, sample0, _dp(d_sum),
                                             _dp(d_comp), _dp(d_sumsq)))
 
+    def render_batch(self, jobs, d_sum, d_comp, bounces, d_sumsq=None, flags=0, use_graph=False):
+        """jobs: list of (camera, spp, sample0); buffers [n][W*H*3] (sum, comp), [n][W*H] (sumsq)."""
+        arr = (_abi.PoseJob * len(jobs))()
+        for i, (cam, spp, s0) in enumerate(jobs):
+            arr[i].cam = cam
+            arr[i].spp = spp
+            arr[i].sample0 = s0
+        self._c(lib().lgi_render_batch(self._h, arr, len(jobs), bounces, flags, _dp(d_sum), _dp(d_comp),
+                                       _dp(d_sumsq), int(use_graph)))
+
+    def render_tiles(self, tiles, spp, d_sum, d_comp, bounces, sample0=0, d_sumsq=None, flags=0, camera=None):
+        """Adds spp samples to the listed 8x8 tiles (row-major tile indices)."""
+        cam = camera if camera is not None else self.camera()
+        t = np.ascontiguousarray(tiles, np.int32)
+        self._c(lib().lgi_render_tiles(self._h, C.byref(cam), ptr(t) if len(t) else None, len(t), spp, sample0,
+                                       bounces, flags, _dp(d_sum), _dp(d_comp), _dp(d_sumsq)))
+
     def primary_hits(self, d_tri, d_t, camera=None):
         cam = camera if camera is not None else self.camera()
         self._c(lib().lgi_primary_hits(self._h, C.byref(cam), _dp(d_tri), _dp(d_t)))
