@@ -190,6 +190,12 @@ LGI_API lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam, const i
                                     int32_t spp, int32_t sample0, int32_t bounces, uint32_t flags, float* d_sum,
                                     float* d_comp, float* d_sumsq);
 
+/* ---- image metrics (tools/diff.py:25-29, 100-101), FP64 reductions on the GPU ----
+ * d_a, d_b: h*w*3 u8 RGB; d_scratch: >= 3*1024 doubles; d_out (device, 3 doubles) receives
+ * {MSE, SSIM (7x7 uniform window, sample covariance, K1 .01, K2 .03, range 255), mean |a-b|}. */
+LGI_API lgi_status lgi_image_metrics(const uint8_t* d_a, const uint8_t* d_b, int32_t h, int32_t w,
+                                     double* d_scratch, double* d_out, void* stream);
+
 /* ---- batch (read_batch_file, src/main.cpp:61-91) ---- */
 LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs);
 

@@ -1,0 +1,129 @@
+"""Denoising pipeline on the GPU: render with per-pixel variance, UNet on log1p radiance,
+variance-gated tile fallback to more samples, FP64 image metrics.
+
+The plain tracer is both the fallback and the judge: an 8x8 tile whose denoised mean luminance
+leaves the confidence band of the noisy tile mean (k standard errors, from the per-pixel
+sample variance of the luminance samples) is rendered again with more samples, and its plain
+estimate replaces the denoised one.
+"""
+import ctypes as C
+
+import torch
+
+from . import _abi
+from ._abi import LgiError
+
+LUM = (0.2126, 0.7152, 0.0722)
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def image_metrics(a_u8, b_u8):
+    """(MSE, SSIM, MAD) of two HxWx3 uint8 CUDA tensors, reduced in FP64 on the GPU."""
+    assert a_u8.dtype == torch.uint8 and a_u8.shape == b_u8.shape and a_u8.is_cuda
+    h, w, _ = a_u8.shape
+    scratch = torch.empty(3 * 1024 + 8, dtype=torch.float64, device=a_u8.device)
+    out = torch.empty(3, dtype=torch.float64, device=a_u8.device)
+    st = _abi.lib().lgi_image_metrics(C.c_void_p(a_u8.contiguous().data_ptr()), C.c_void_p(b_u8.contiguous().data_ptr()),
+                                      h, w, C.c_void_p(scratch.data_ptr()), C.c_void_p(out.data_ptr()), _stream())
+    if st != 0:
+        raise LgiError(st, "lgi_image_metrics")
+    mse, ssim, mad = out.cpu().tolist()
+    return mse, ssim, mad
+
+
+def psnr_from_mse(mse):
+    import math
+    return float("inf") if mse == 0 else 20.0 * math.log10(255.0 / math.sqrt(mse))
+
+
+def tonemap(ctx, hdr):
+    """Reference sqrt tonemap (src/pathtracer.cu:413-429) of mean radiance HxWx3 -> uint8."""
+    h, w, _ = hdr.shape
+    out = torch.empty((h, w, 3), dtype=torch.uint8, device=hdr.device)
+    ctx.tonemap(hdr.contiguous(), h * w, 1.0, out)
+    return out
+
+
+class Frame:
+    """Device accumulators of one image: sums, Kahan terms, squared luminance."""
+
+    def __init__(self, cam, device="cuda"):
+        self.cam = cam
+        n = cam.width * cam.height
+        self.sum = torch.zeros(n * 3, device=device)
+        self.comp = torch.zeros(n * 3, device=device)
+        self.sumsq = torch.zeros(n, device=device)
+        self.spp = torch.zeros((cam.height, cam.width), device=device)  # samples per pixel
+
+    def render(self, ctx, spp, bounces=2, sample0=0, flags=0):
+        ctx.accumulate(self.sum, self.comp, spp, bounces, flags=flags, sample0=sample0, d_sumsq=self.sumsq,
+                       camera=self.cam)
+        self.spp += spp
+
+    def mean(self):
+        h, w = self.cam.height, self.cam.width
+        return self.sum.view(h, w, 3) / self.spp.unsqueeze(-1)
+
+    def lum_se(self):
+        """Standard error of the mean luminance per pixel (unbiased sample variance)."""
+        h, w = self.cam.height, self.cam.width
+        n = self.spp
+        m = (self.mean() * torch.tensor(LUM, device=n.device)).sum(-1)
+        var = (self.sumsq.view(h, w) / n - m * m) * n / (n - 1).clamp_min(1)
+        return torch.sqrt(var.clamp_min(0) / n)
+
+
+def denoise(model, hdr):
+    """UNet on log1p radiance; hdr HxWx3 (H, W multiples of 16) -> denoised radiance."""
+    with torch.no_grad():
+        y = model(torch.log1p(hdr.clamp_min(0)).unsqueeze(0).contiguous())
+    return torch.expm1(y[0]).clamp_min(0)
+
+
+def _tile_sum(x, th, tw):
+    h, w = x.shape
+    pad = torch.zeros((th * 8, tw * 8), dtype=x.dtype, device=x.device)
+    pad[:h, :w] = x
+    return pad.view(th, 8, tw, 8).sum(dim=(1, 3))
+
+
+def gate_tiles(noisy_mean, se, denoised, k=4.0, rel_floor=0.02, abs_floor=1e-3):
+    """Row-major indices of the 8x8 tiles whose denoised mean luminance leaves the confidence
+    band of the noisy tile mean: |mean(den) - mean(noisy)| > k * SE_tile + floor, where SE_tile =
+    sqrt(sum of per-pixel SE^2) / pixels and floor = rel_floor * mean(noisy) + abs_floor (the
+    per-pixel variance from a few samples is itself noisy; the tile mean is not)."""
+    lw = torch.tensor(LUM, device=noisy_mean.device)
+    ln = (noisy_mean * lw).sum(-1)
+    ld = (denoised * lw).sum(-1)
+    h, w = ln.shape
+    th, tw = (h + 7) // 8, (w + 7) // 8
+    cnt = _tile_sum(torch.ones_like(ln), th, tw)
+    mn = _tile_sum(ln, th, tw) / cnt
+    md = _tile_sum(ld, th, tw) / cnt
+    se_t = torch.sqrt(_tile_sum(se * se, th, tw)) / cnt
+    bad = (md - mn).abs() > k * se_t + rel_floor * mn.abs() + abs_floor
+    return torch.nonzero(bad.reshape(-1)).flatten().to(torch.int32)
+
+
+def denoise_with_fallback(ctx, model, frame, extra_spp=248, bounces=2, sample0=1 << 22, k=4.0, rel_floor=0.02):
+    """Denoise frame; re-render gated tiles with extra_spp more samples and keep their plain mean.
+    Returns (radiance HxWx3, number of fallback tiles)."""
+    noisy = frame.mean()
+    den = denoise(model, noisy)
+    tiles = gate_tiles(noisy, frame.lum_se(), den, k=k, rel_floor=rel_floor)
+    if tiles.numel() == 0:
+        return den, 0
+    cam = frame.cam
+    ctx.render_tiles(tiles.cpu().numpy(), extra_spp, frame.sum, frame.comp, bounces, sample0=sample0,
+                     d_sumsq=frame.sumsq, camera=cam)
+    h, w = cam.height, cam.width
+    th, tw = (h + 7) // 8, (w + 7) // 8
+    flag = torch.zeros(th * tw, dtype=torch.bool, device=noisy.device)
+    flag[tiles.long()] = True
+    mask = flag.view(th, 1, tw, 1).expand(th, 8, tw, 8).reshape(th * 8, tw * 8)[:h, :w]
+    frame.spp[mask] += extra_spp
+    out = torch.where(mask.unsqueeze(-1), frame.mean(), den)
+    return out, int(tiles.numel())

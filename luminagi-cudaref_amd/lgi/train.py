@@ -1,0 +1,247 @@
+"""Denoiser training on pairs rendered by the project's own tracer, and the time-to-quality
+evaluation of 8 spp + UNet against the plain tracer.
+
+Training recipe (reference ml/train.py:128-225, in this project's configuration): random
+256x256 crops of (8-spp noisy, 256-spp clean) log1p radiance pairs, batch 16, L1 loss, Adam with
+FP32 master weights, BF16 activations and gradients inside the kernels, dynamic loss scaling,
+activations recomputed per UNet level; the best validation checkpoint is kept. Checkpoints
+keep the reference keys {"model", "base", "in_ch"} and add {"arch", "optimizer", "loss_scale"}.
+"""
+import argparse
+import json
+import math
+import pathlib
+import time
+
+import numpy as np
+import torch
+
+from . import denoise as D
+from .poses import camera_for, orbit_poses
+from .tracer import Context
+from .unet import UNet
+
+ARCH = "unet-relu-bilinear-32-64-128-256-512-log1p"
+
+
+class LossScaler:
+    """Dynamic loss scaling: halve on inf/nan gradients (step skipped), double every `interval`
+    clean steps."""
+
+    def __init__(self, scale=2.0 ** 12, interval=200):
+        self.scale, self.interval, self.clean = scale, interval, 0
+
+    def step(self, params, opt):
+        grads = [p.grad for p in params if p.grad is not None]
+        finite = all(bool(torch.isfinite(g).all()) for g in grads)
+        if not finite:
+            self.scale /= 2.0
+            self.clean = 0
+            opt.zero_grad(set_to_none=True)
+            return False
+        for g in grads:
+            g.div_(self.scale)
+        opt.step()
+        opt.zero_grad(set_to_none=True)
+        self.clean += 1
+        if self.clean % self.interval == 0:
+            self.scale *= 2.0
+        return True
+
+
+def make_scene(kind=1, seed=1, width=1728, height=864):
+    ctx = Context()
+    ctx.generate(kind, seed, width, height)
+    ctx.build_bvh()
+    return ctx
+
+
+def render_mean(ctx, cam, spp, sample0, bounces=2, batch=None):
+    """Mean radiance HxWx3 (CUDA) of spp samples starting at sample0."""
+    f = D.Frame(cam)
+    step = batch or spp
+    done = 0
+    while done < spp:
+        s = min(step, spp - done)
+        f.render(ctx, s, bounces=bounces, sample0=sample0 + done)
+        done += s
+    return f.mean().contiguous()
+
+
+def make_dataset(ctx, n_poses=50, noisy_spp=8, clean_spp=256, seed=0, bounces=2):
+    """(noisy, clean) log1p radiance pairs [n, H, W, 3] on the GPU from orbit poses."""
+    base = ctx.camera()
+    noisy, clean = [], []
+    for pos, tgt, _ in orbit_poses(n_poses, seed=seed):
+        cam = camera_for(base, pos, tgt)
+        noisy.append(torch.log1p(render_mean(ctx, cam, noisy_spp, 0, bounces)))
+        clean.append(torch.log1p(render_mean(ctx, cam, clean_spp, 1 << 16, bounces, batch=64)))
+    return torch.stack(noisy), torch.stack(clean)
+
+
+def random_crops(noisy, clean, batch, crop, gen):
+    n, h, w, _ = noisy.shape
+    idx = torch.randint(0, n, (batch,), generator=gen)
+    ys = torch.randint(0, h - crop + 1, (batch,), generator=gen)
+    xs = torch.randint(0, w - crop + 1, (batch,), generator=gen)
+    flips = torch.randint(0, 2, (batch,), generator=gen)
+    xb, yb = [], []
+    for i, y, x, f in zip(idx.tolist(), ys.tolist(), xs.tolist(), flips.tolist()):
+        a = noisy[i, y:y + crop, x:x + crop]
+        b = clean[i, y:y + crop, x:x + crop]
+        if f:
+            a, b = a.flip(1), b.flip(1)
+        xb.append(a)
+        yb.append(b)
+    return torch.stack(xb).contiguous(), torch.stack(yb).contiguous()
+
+
+def train(noisy, clean, steps=2000, batch=16, crop=256, lr=1e-4, seed=42, val_frac=0.2, time_budget=None,
+          recompute=True, log=print):
+    n = noisy.shape[0]
+    perm = np.random.RandomState(seed).permutation(n)  # seeded split (ml/train.py:152-158)
+    n_val = max(1, int(n * val_frac))
+    val_idx, tr_idx = perm[:n_val].tolist(), perm[n_val:].tolist()
+    tr_n, tr_c = noisy[tr_idx], clean[tr_idx]
+    torch.manual_seed(seed)
+    model = UNet(seed=seed, recompute=recompute).cuda()
+    params = list(model.parameters())
+    opt = torch.optim.Adam(params, lr=lr)
+    scaler = LossScaler()
+    gen = torch.Generator().manual_seed(seed)
+    best, best_state, hist = float("inf"), None, []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        xb, yb = random_crops(tr_n, tr_c, batch, crop, gen)
+        loss = (model(xb) - yb).abs().mean()
+        (loss * scaler.scale).backward()
+        scaler.step(params, opt)
+        if (it + 1) % 100 == 0 or it + 1 == steps:
+            with torch.no_grad():
+                vl = float(np.mean([(model(noisy[i:i + 1]) - clean[i:i + 1]).abs().mean().item() for i in val_idx]))
+            hist.append((it + 1, float(loss), vl))
+            tag = ""
+            if vl < best:
+                best, tag = vl, "  *"
+                best_state = {k: v.detach().clone() for k, v in model.state_dict().items()}
+            log(f"step {it + 1:5d}/{steps}  train={float(loss):.4f}  val={vl:.4f}  scale={scaler.scale:g}{tag}")
+        if time_budget and time.perf_counter() - t0 > time_budget:
+            log(f"time budget reached after {it + 1} steps")
+            break
+    model.load_state_dict(best_state)
+    ckpt = {"model": {k: v.cpu() for k, v in best_state.items()}, "base": 32, "in_ch": 3, "arch": ARCH,
+            "optimizer": opt.state_dict(), "loss_scale": scaler.scale, "best_val": best, "history": hist}
+    return model, ckpt, time.perf_counter() - t0
+
+
+def load_model(path):
+    ckpt = torch.load(path, map_location="cpu", weights_only=True)
+    if ckpt.get("arch") != ARCH:
+        raise ValueError(f"checkpoint arch {ckpt.get('arch')!r} is not {ARCH!r}")
+    model = UNet().cuda()
+    model.load_state_dict(ckpt["model"])
+    return model.eval()
+
+
+def timed(fn):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t) * 1e3
+
+
+def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, log=print):
+    """PSNR/SSIM (8-bit sqrt-tonemapped domain, as tools/diff.py) of plain renders at growing spp
+    and of 8 spp + UNet against a ref_spp plain render; times to 40 dB. flags selects the
+    renderer used for the timed renders (default: sorted wavefront; both give the same bits)."""
+    gt = D.tonemap(ctx, render_mean(ctx, cam, ref_spp, 0, bounces, batch=256))
+    test0 = 1 << 24  # sample streams disjoint from the reference's
+    res = {"ref_spp": ref_spp, "plain": []}
+    # plain curve: progressive accumulation, timed per added chunk
+    f = D.Frame(cam)
+    total_ms, spp = 0.0, 0
+    for target in (8, 16, 32, 64, 128, 256, 512, 1024):
+        _, ms = timed(lambda: f.render(ctx, target - spp, bounces, sample0=test0 + spp, flags=flags))
+        total_ms += ms
+        spp = target
+        mse, ssim, mad = D.image_metrics(D.tonemap(ctx, f.mean()), gt)
+        res["plain"].append({"spp": spp, "ms": total_ms, "psnr": D.psnr_from_mse(mse), "ssim": ssim, "mad": mad})
+        log(f"plain {spp:5d} spp: {D.psnr_from_mse(mse):6.2f} dB  ssim {ssim:.4f}  {total_ms:8.1f} ms")
+    ms_per_spp = res["plain"][-1]["ms"] / 1024
+    # 8 spp + UNet (+ gated fallback)
+    fr = D.Frame(cam)
+    _, t_render = timed(lambda: fr.render(ctx, 8, bounces, sample0=test0 + (1 << 20), flags=flags))
+    D.denoise(model, fr.mean())  # warm-up: first-call allocations are not part of the steady state
+    den, t_den = timed(lambda: D.denoise(model, fr.mean()))
+    mse, ssim, mad = D.image_metrics(D.tonemap(ctx, den), gt)
+    res["unet"] = {"spp": 8, "render_ms": t_render, "denoise_ms": t_den, "psnr": D.psnr_from_mse(mse), "ssim": ssim}
+    mse8, ssim8, _ = D.image_metrics(D.tonemap(ctx, fr.mean()), gt)
+    res["noisy8"] = {"psnr": D.psnr_from_mse(mse8), "ssim": ssim8}
+    log(f"8 spp raw: {D.psnr_from_mse(mse8):.2f} dB; 8 spp + UNet: {res['unet']['psnr']:.2f} dB ssim {ssim:.4f} "
+        f"({t_render:.1f} + {t_den:.1f} ms)")
+    if fallback:
+        warm = D.Frame(cam)  # warm-up of the tile path on a throw-away frame
+        warm.render(ctx, 8, bounces, sample0=test0 + (1 << 22), flags=flags)
+        D.denoise_with_fallback(ctx, model, warm, bounces=bounces, sample0=test0 + (1 << 23))
+        (out, nt), t_fb = timed(lambda: D.denoise_with_fallback(ctx, model, fr, bounces=bounces,
+                                                                sample0=test0 + (1 << 21)))
+        mse, ssim, _ = D.image_metrics(D.tonemap(ctx, out), gt)
+        res["unet_fallback"] = {"tiles": nt, "ms": t_render + t_fb, "psnr": D.psnr_from_mse(mse), "ssim": ssim}
+        log(f"8 spp + UNet + gated fallback ({nt} tiles): {D.psnr_from_mse(mse):.2f} dB ({t_render + t_fb:.1f} ms)")
+    # plain spp reaching 40 dB: MSE ~ 1/spp between the bracketing measurements
+    pts = res["plain"]
+    spp40 = None
+    for a, b in zip(pts, pts[1:]):
+        if a["psnr"] < 40.0 <= b["psnr"]:
+            spp40 = a["spp"] * 10 ** ((40.0 - a["psnr"]) / 10.0)
+            break
+    if spp40 is None and pts[0]["psnr"] >= 40.0:
+        spp40 = pts[0]["spp"] * 10 ** ((40.0 - pts[0]["psnr"]) / 10.0)
+    res["plain_spp_40db"] = spp40
+    res["plain_ms_40db"] = None if spp40 is None else spp40 * ms_per_spp
+    # the cheapest UNet variant that reaches 40 dB (plain UNet first, then with the fallback)
+    res["unet"]["ms"] = res["unet"]["render_ms"] + res["unet"]["denoise_ms"]
+    cands = [(res[k]["ms"], k) for k in ("unet", "unet_fallback") if k in res and res[k]["psnr"] >= 40.0]
+    res["unet_reaches_40db"] = bool(cands)
+    if cands:
+        res["unet_ms"], res["unet_variant_40db"] = min(cands)
+        if res["plain_ms_40db"]:
+            res["speedup_40db"] = res["plain_ms_40db"] / res["unet_ms"]
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="train the denoiser on tracer-generated pairs and evaluate it")
+    ap.add_argument("--poses", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--time-budget", type=float, default=None, help="seconds of training at most")
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--crop", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--width", type=int, default=1728)
+    ap.add_argument("--height", type=int, default=864)
+    ap.add_argument("--scene-seed", type=int, default=1)
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--out", default="denoiser.pt")
+    ap.add_argument("--report", default=None, help="write the evaluation as JSON here")
+    args = ap.parse_args(argv)
+    ctx = make_scene(1, args.scene_seed, args.width, args.height)
+    (data, t_data) = timed(lambda: make_dataset(ctx, args.poses))
+    print(f"dataset: {args.poses} poses x (8, 256) spp at {args.width}x{args.height} in {t_data / 1e3:.1f} s")
+    model, ckpt, t_train = train(*data, steps=args.steps, batch=args.batch, crop=args.crop, lr=args.lr,
+                                 time_budget=args.time_budget)
+    torch.save(ckpt, args.out)
+    print(f"training: {t_train:.1f} s, best val L1 {ckpt['best_val']:.4f}, saved {args.out}")
+    # held-out pose: an orbit pose from a different seed
+    (pos, tgt, _), = orbit_poses(1, seed=12345)
+    cam = camera_for(ctx.camera(), pos, tgt)
+    res = evaluate(ctx, model.eval(), cam, ref_spp=args.ref_spp)
+    res.update({"train_s": t_train, "dataset_s": t_data / 1e3, "steps": len(ckpt["history"]) and ckpt["history"][-1][0]})
+    print(json.dumps({k: v for k, v in res.items() if k != "plain"}))
+    if args.report:
+        pathlib.Path(args.report).write_text(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    main()
