@@ -1,11 +1,14 @@
-"""Benchmark of the plain path tracer on one MI355X.
+"""Benchmark of the plain path tracer and the denoiser on one MI355X.
 
-Workload: the 62-mesh / ~1.05M-triangle procedural scene, 1728x864, 2 bounces with
-sun + 2 point-light NEE, PCG32 streams. One step = one 8-spp frame rendered into device
-accumulators (inputs resident in HBM). Prints one JSON line with Msamples/s, the
-render kernel's roofline figures and the CPU oracle timed on the host beside it.
+Workload: the 62-mesh / ~1.05M-triangle procedural scene, 1728x864, 64 spp, 2 bounces with
+sun + 2 point-light NEE, PCG32 streams, ray sorting between bounces. One step = one 64-spp
+frame rendered into device accumulators (inputs resident in HBM). Prints one JSON line with
+Msamples/s, the traversal kernel's roofline figures (HIP events around every traversal
+launch), the CPU oracle timed on the host beside it and, when the trained denoiser checkpoint
+is present, the time to 40 dB PSNR of 8 spp + UNet against the plain tracer on the 16-mesh
+scene at 1728x864 (error against a 4096-spp plain render).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--spp 8]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--spp 64]
 """
 import argparse
 import json
@@ -22,8 +25,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = "ms to 40 dB PSNR @1728x864: 8spp+UNet vs plain spp; Msamples/s, 2 bounces"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes (layout-independent minimum): 24 B box + 8 B links per node visit,
-# 3 x 12 B vertices per triangle test, 40 B (normals + material) per closest hit.
+# 3 x 12 B vertices per triangle test, 40 B (normals + material) per closest hit. The
+# traversal kernel's share is the node and triangle bytes; the hit bytes are read by shading.
 BYTES_NODE, BYTES_TRI, BYTES_HIT = 32, 36, 40
+DENOISER_CKPT = ROOT / "luminagi-cudaref_amd" / "ml" / "weights" / "unet_log1p_c32.pt"
 
 
 def cpu_model():
@@ -46,12 +51,31 @@ def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1):
     rows = (cam.height + row_step - 1) // row_step
     samples = rows * cam.width * spp
     c = out["counters"]
-    bytes_per_sample = (BYTES_NODE * c["node_visits"] + BYTES_TRI * c["tri_tests"]
-                        + BYTES_HIT * c["closest_hits"]) / samples
+    trace_bytes = (BYTES_NODE * c["node_visits"] + BYTES_TRI * c["tri_tests"]) / samples
+    bytes_per_sample = trace_bytes + BYTES_HIT * c["closest_hits"] / samples
     t0 = time.perf_counter()
     orc.bvh_build(scene["tris"])
     bvh_s = time.perf_counter() - t0
-    return samples / dt / 1e6, bytes_per_sample, c, dt, samples, bvh_s
+    return samples / dt / 1e6, bytes_per_sample, trace_bytes, c, dt, samples, bvh_s
+
+
+def denoise_section(ckpt, bounces):
+    """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864 (sorted renderer)."""
+    from lgi import train as T
+    ctx = T.make_scene(1, 1, 1728, 864)
+    model = T.load_model(ckpt)
+    r = T.evaluate(ctx, model, T.held_out_camera(ctx), ref_spp=4096, bounces=bounces, log=lambda *a: None)
+    keep = ("plain_spp_40db", "plain_ms_40db", "unet_reaches_40db", "unet_ms", "unet_variant_40db",
+            "speedup_40db", "plain_spp_same_psnr", "plain_ms_same_psnr", "speedup_same_psnr")
+    out = {k: (round(r[k], 3) if isinstance(r[k], float) else r[k]) for k in keep if k in r}
+    out["unet"] = {k: round(v, 3) for k, v in r["unet"].items()}
+    out["noisy8_psnr"] = round(r["noisy8"]["psnr"], 3)
+    if "unet_fallback" in r:
+        out["unet_fallback"] = {k: round(v, 3) for k, v in r["unet_fallback"].items()}
+    out["plain_curve"] = [(p["spp"], round(p["ms"], 2), round(p["psnr"], 3)) for p in r["plain"]]
+    out["workload"] = "16-mesh procedural scene (seed 1), 1728x864, 2 bounces, held-out orbit pose; error vs 4096 spp"
+    out["checkpoint"] = os.path.relpath(ckpt, ROOT)
+    return out
 
 
 def main():
@@ -59,14 +83,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp", type=int, default=8)
+    ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--bounces", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-row-step", type=int, default=2)
-    ap.add_argument("--cpu-spp", type=int, default=8)
+    ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--mode", choices=["sorted", "inline"], default="sorted",
                     help="sorted wavefront renderer (default) or the inline megakernel; same image bits")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
+    ap.add_argument("--ckpt", default=str(DENOISER_CKPT), help="denoiser checkpoint for the 40 dB measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -113,11 +139,15 @@ def main():
         ctx.accumulate(acc, comp, args.spp, args.bounces, flags=flags, sample0=sample0, d_sumsq=sumsq)
         sample0 += args.spp
     torch.cuda.synchronize()
-    kernel_ms = []
+    kernel_ms, trace_ms, trace_launches = [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.accumulate(acc, comp, args.spp, args.bounces, flags=flags, sample0=sample0, d_sumsq=sumsq)
-        kernel_ms.append(ctx.last_render_ms())  # HIP events on the context stream, per launch
+        # HIP events on the context stream: whole render call, and every traversal launch in it
+        kernel_ms.append(ctx.last_render_ms())
+        tm, tl = ctx.last_trace_ms()
+        trace_ms.append(tm)
+        trace_launches.append(tl)
         sample0 += args.spp
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -125,6 +155,7 @@ def main():
     samples_per_step = n_px * args.spp
     msps = samples_per_step / (wall / args.steps) / 1e6
     avg_kernel_ms = float(np.mean(kernel_ms))
+    avg_trace_ms = float(np.mean(trace_ms))
 
     result = {
         "metric": METRIC, "value": round(msps, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
@@ -135,29 +166,36 @@ def main():
                                                                                       W, H, args.spp, args.bounces),
                    "triangles": int(len(scene["tris"])), "width": W, "height": H, "spp_per_step": args.spp,
                    "bounces": args.bounces, "scene_seed": args.seed, "renderer": args.mode},
-        "render_kernel_ms": round(avg_kernel_ms, 3),
+        "render_ms": round(avg_kernel_ms, 3),
+        "trace_kernel_ms_per_frame": round(avg_trace_ms, 3),
+        "trace_launches_per_frame": int(np.median(trace_launches)),
         "lbvh_build_ms": round(float(np.median(build_ms[1:])), 3),
+        "triangles": int(len(scene["tris"])),
     }
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
-        cpu_msps, bps, counters, cpu_s, cpu_samples, bvh_s = cpu_baseline(scene, cam, args.bounces,
-                                                                           args.cpu_row_step, threads, args.cpu_spp)
-        algo_bytes = bps * samples_per_step
-        achieved = algo_bytes / (avg_kernel_ms * 1e-3) / 1e9
-        # HBM traffic per frame from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-        # this workload (calibrated against a plain copy); the timing here is live.
+        cpu_msps, bps, trace_bps, counters, cpu_s, cpu_samples, bvh_s = cpu_baseline(
+            scene, cam, args.bounces, args.cpu_row_step, threads, args.cpu_spp)
+        # dominant kernel = BVH traversal: its algorithmic bytes per frame over its summed
+        # launch time per frame (HIP events, live)
+        algo_bytes = trace_bps * samples_per_step
+        achieved = algo_bytes / (avg_trace_ms * 1e-3) / 1e9
         traffic, counters_src = None, None
-        rel = "profiles/render_%s_config2.json" % args.mode
+        rel = "profiles/trace_%s_config2.json" % args.mode
         prof = ROOT / rel
-        if prof.exists() and args.spp == 8 and args.bounces == 2 and args.seed == 1:
+        if prof.exists():
             p = json.loads(prof.read_text())
-            traffic, counters_src = int(p["traffic_bytes"]), {
-                "file": rel, "profiled_frame_ms": round(p["avg_ns"] / 1e6, 3),
-                "l2_hit_rate": round(p["l2_hit_rate"], 4), "kernels": p.get("kernels")}
+            if (p.get("spp"), p.get("bounces"), p.get("scene_seed")) == (args.spp, args.bounces, args.seed):
+                traffic = int(p["traffic_bytes_per_frame"])
+                counters_src = {"file": rel, "profiled_trace_ms_per_frame": round(p["trace_ns_per_frame"] / 1e6, 3),
+                                "l2_hit_rate": round(p["l2_hit_rate"], 4), "fetch_factor": round(p["fetch_factor"], 4),
+                                "write_factor": round(p["write_factor"], 4)}
         result["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                              "algorithmic_bytes": int(algo_bytes), "bytes_per_sample": round(bps, 1),
-                              "counters": counters_src}
+                              "kernel": "k_wf_trace (all traversal launches of one frame)" if args.mode == "sorted"
+                              else "k_render", "algorithmic_bytes": int(algo_bytes),
+                              "trace_bytes_per_sample": round(trace_bps, 1),
+                              "frame_bytes_per_sample": round(bps, 1), "counters": counters_src}
         result["cpu_baseline"] = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads,
                                   "kind": "port",
                                   "sample": "CPU oracle, same scene and PCG32 streams, %d spp on every %dth row "
@@ -165,6 +203,11 @@ def main():
                                                                           cpu_s, cpu_model()),
                                   "bvh_build_ms_1thread": round(bvh_s * 1e3, 1)}
         result["oracle_counters_per_sample"] = {k: round(v / cpu_samples, 2) for k, v in counters.items()}
+        # rays per sample (closest-hit + shadow) from the oracle's deterministic counts
+        rays = (counters["rays"] + counters["shadow_rays"]) / cpu_samples
+        result["mrays_per_s"] = round(msps * rays, 1)
+    if not args.no_denoise and pathlib.Path(args.ckpt).exists():
+        result["ms_to_40db"] = denoise_section(args.ckpt, args.bounces)
     print(json.dumps(result), flush=True)
 
 

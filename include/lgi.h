@@ -164,6 +164,10 @@ LGI_API lgi_status lgi_gbuffer(lgi_ctx* ctx, const lgi_camera* cam, uint8_t* d_a
                                uint8_t* d_nrm, uint8_t* d_wp);
 /* Kernel time of the last render call (ms, HIP events). */
 LGI_API lgi_status lgi_last_render_ms(const lgi_ctx* ctx, float* ms);
+/* Measurement hook, no reference counterpart: summed HIP-event time of the BVH traversal
+ * kernel launches of the last render call (for the sorted renderer every closest-hit and
+ * shadow-ray launch; for the inline renderer the one megakernel launch) and their count. */
+LGI_API lgi_status lgi_last_trace_ms(const lgi_ctx* ctx, float* ms, int32_t* launches);
 
 /* ---- many poses / selected tiles in one persistent launch ----
  * No reference counterpart beyond the --batch loop (src/main.cpp:158-189), which re-uploads the
@@ -222,6 +226,17 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
                                                int32_t c0, const void* src1, int32_t c1, int32_t n, int32_t h,
                                                int32_t w_, float* dw, float* db, float* scratch,
                                                int64_t scratch_floats, void* stream);
+/* Convolution tile-shape selection (no reference counterpart; the role of cuDNN's benchmark
+ * mode under the reference's PyTorch UNet). mode 1 (default): the first call of each layer
+ * shape times every tiling variant that fits it on the caller's buffers and stream (it
+ * synchronizes that stream once) and later calls reuse the fastest; calls under stream
+ * capture and mode 0 use a fixed choice. force_variant(v >= 0) makes every call use variant v
+ * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
+ * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, masked, relu, variant},
+ * us[5*i..] = microseconds per launch of each variant (0 = does not fit); *n = entries. */
+LGI_API lgi_status lgi_unet_conv_tuning(int32_t mode);
+LGI_API lgi_status lgi_unet_conv_force_variant(int32_t variant);
+LGI_API lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32_t* n);
 LGI_API lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w_, int32_t c, void* out,
                                         void* stream);
 LGI_API lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int32_t h, int32_t w_,

@@ -154,6 +154,7 @@ lgi_status run_queue(lgi_ctx* ctx, const std::vector<lgi_work_item>& items, cons
     for (const void* p : {(const void*)d_items, (const void*)d_cams, (const void*)counter})
         key.append((const char*)&p, sizeof(p));
     const int grid = std::max(1, std::min<int>(kQueueGrid, (int)items.size()));
+    ctx->kev_used = 0;
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (use_graph && ctx->graph_exec && ctx->graph_key == key) {
         LGI_HIP(ctx, hipGraphLaunch((hipGraphExec_t)ctx->graph_exec, ctx->stream));
@@ -199,6 +200,21 @@ lgi_status check_render_ready(lgi_ctx* ctx, const lgi_camera& cam) {
 
 }  // namespace
 
+lgi_status lgi_mark_trace(lgi_ctx* ctx, bool begin) {
+    constexpr int kPairs = 64;
+    if (ctx->kev.empty()) {
+        ctx->kev.resize(2 * kPairs, nullptr);
+        for (auto& e : ctx->kev) LGI_HIP(ctx, hipEventCreate(&e));
+    }
+    if (ctx->kev_used >= kPairs) {
+        ctx->kev_overflow = true;
+        return LGI_OK;
+    }
+    LGI_HIP(ctx, hipEventRecord(ctx->kev[2 * ctx->kev_used + (begin ? 0 : 1)], ctx->stream));
+    if (!begin) ++ctx->kev_used;
+    return LGI_OK;
+}
+
 extern "C" {
 
 const char* lgi_version(void) { return "lgi 0.1 (gfx950)"; }
@@ -225,6 +241,8 @@ void lgi_destroy(lgi_ctx* ctx) {
             if (e.first) (void)hipFree(e.first);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        for (auto e : ctx->kev)
+            if (e) (void)hipEventDestroy(e);
         if (ctx->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->graph_exec);
         if (ctx->cap_stream) (void)hipStreamDestroy((hipStream_t)ctx->cap_stream);
     }
@@ -393,10 +411,33 @@ lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t
                                     make_float3(r.bb_max[0], r.bb_max[1], r.bb_max[2]));
     }
     if ((st = attach_spill(ctx, P, lgi_grid_threads(cam.width, cam.height))) != LGI_OK) return st;
+    ctx->kev_used = 0;
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     lgi_launch_render(P, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());
     LGI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return LGI_OK;
+}
+
+lgi_status lgi_last_trace_ms(const lgi_ctx* ctx_c, float* ms, int32_t* launches) {
+    lgi_ctx* ctx = const_cast<lgi_ctx*>(ctx_c);
+    if (!ctx || !ms || !launches) return LGI_ERR_INVALID_ARG;
+    if (!ctx->gpu_ready) return lgi_set_err(ctx, LGI_ERR_NO_BVH, "nothing rendered yet");
+    LGI_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    if (ctx->kev_used == 0) {  // inline renderer: the megakernel is the traversal kernel
+        LGI_HIP(ctx, hipEventElapsedTime(ms, ctx->ev0, ctx->ev1));
+        *launches = 1;
+        return LGI_OK;
+    }
+    if (ctx->kev_overflow) return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "more traversal launches than event pairs");
+    float total = 0.f;
+    for (int i = 0; i < ctx->kev_used; ++i) {
+        float t = 0.f;
+        LGI_HIP(ctx, hipEventElapsedTime(&t, ctx->kev[2 * i], ctx->kev[2 * i + 1]));
+        total += t;
+    }
+    *ms = total;
+    *launches = ctx->kev_used;
     return LGI_OK;
 }
 

@@ -85,6 +85,10 @@ struct lgi_ctx {
     float last_build_ms = 0.f;
     float last_render_ms = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // event pairs around each traversal-kernel launch of the last wavefront render
+    std::vector<hipEvent_t> kev;
+    int kev_used = 0;           // pairs recorded by the last render (0: inline renderer)
+    bool kev_overflow = false;  // more launches than pairs: the sum covers the first kev.size()/2
     void* cap_stream = nullptr;  // hipStream_t used for graph capture
     void* graph_exec = nullptr;  // hipGraphExec_t of the last captured batch launch
     std::string graph_key;       // inputs the captured graph was built for
@@ -106,6 +110,8 @@ lgi_status lgi_set_err(lgi_ctx* ctx, lgi_status st, const std::string& msg);
 
 // Kernel-side entry points implemented in the .hip files.
 lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx);
+// Records the start (begin=true) or end event of the next traversal launch on ctx->stream.
+lgi_status lgi_mark_trace(lgi_ctx* ctx, bool begin);
 // Stable LSD radix sort of (keys, vals) on `bits` (multiple of 16) low key bits, in place;
 // scratch must hold lgi_sort_scratch_bytes(n).
 size_t lgi_sort_scratch_bytes(int64_t n);

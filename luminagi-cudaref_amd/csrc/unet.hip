@@ -9,14 +9,20 @@
 // Layout: NHWC, BF16 activations, weights [Cout][tap][Cin] (K = tap*Cin + ci contiguous per
 // output channel), FP32 bias / master weights / weight gradients.
 //
-// Convolution tiling: one 256-thread block computes an 8x16-pixel x TN-channel output tile.
-// Per 32-channel input chunk it stages the 10x18-pixel input halo and the TN x 9 x 32 weight
-// slice in LDS, then each wave (two output rows) issues 9 taps x 2 x (TN/16) MFMAs. The
-// channel concat of the decoder is fused into the halo gather (two source tensors), and the
-// ReLU backward mask is fused into the gather of the data-gradient pass.
+// Convolution tiling: a 256-thread block computes a (rows x 16 columns) pixel x TN-channel
+// output tile. Per 32-channel input chunk it stages the input halo and the TN x 9 x 32 weight
+// slice in LDS, then its four waves issue the MFMAs from LDS. Two tilings exist (k_conv: 8
+// rows, fragments re-read per tap; k_conv_rb: 16 or 32 rows, register-blocked over kernel
+// rows), and each layer shape is timed once to pick one. The channel concat of the decoder is
+// fused into the halo gather (two source tensors), and the ReLU backward mask is fused into
+// the gather of the data-gradient pass.
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <atomic>
 #include <cstdint>
+#include <map>
+#include <mutex>
 
 #include "lgi.h"
 
@@ -59,11 +65,14 @@ __device__ __forceinline__ uint4 relu_mask(uint4 v, uint4 m) {
     return *reinterpret_cast<const uint4*>(&x);
 }
 
-template <int TN, int KS>
+template <int TN, int KS, bool MASK>
 __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
     constexpr int kTaps = KS * KS;
     constexpr int kPad = KS / 2;
     constexpr int kHalH = kTH + KS - 1, kHalW = kTW + KS - 1;
+    constexpr int kHalo = kHalH * kHalW * 4;   // 16-byte loads of one chunk's halo
+    constexpr int kWLd = TN * kTaps * 4;       // 16-byte loads of one chunk's weights
+    constexpr int kHIt = (kHalo + kBlock - 1) / kBlock, kWIt = (kWLd + kBlock - 1) / kBlock;
     __shared__ __attribute__((aligned(16))) bf16 s_in[kHalH * kHalW * kCK];
     __shared__ __attribute__((aligned(16))) bf16 s_w[TN * kWRow];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -78,34 +87,68 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
     for (int mi = 0; mi < 2; ++mi)
         for (int ni = 0; ni < TN / 16; ++ni) acc[mi][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int g = lane >> 4, r16 = lane & 15;
-    for (int ci0 = 0; ci0 < Cin; ci0 += kCK) {
+    // Software pipeline: the global loads of chunk c + 1 are issued into registers before the
+    // MFMAs of chunk c and written to LDS after them, so their latency hides behind the math.
+    uint4 hv[kHIt], hm[MASK ? kHIt : 1], wv[kWIt];
+    bool masked = false;
+    auto load_chunk = [&](int ci0) {
         // source tensor of this chunk (chunks never straddle the concat boundary)
         const bool first = ci0 < a.C0;
         const bf16* src = first ? a.src0 : a.src1;
         const bf16* msk = first ? a.mask0 : a.mask1;
         const int Cs = first ? a.C0 : a.C1;
         const int cc0 = first ? ci0 : ci0 - a.C0;
-        // stage the input halo: kHalH x kHalW pixels x 32 channels, 16 B per thread-load
-        for (int t = tid; t < kHalH * kHalW * 4; t += kBlock) {
-            const int px = t >> 2, part = t & 3;
-            const int hy = px / kHalW, hx = px % kHalW;
-            const int gy = y0 + hy - kPad, gx = x0 + hx - kPad;
-            uint4 v = zero16();
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
-                const int64_t off = (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8;
-                v = *reinterpret_cast<const uint4*>(src + off);
-                if (msk) v = relu_mask(v, *reinterpret_cast<const uint4*>(msk + off));
+        masked = MASK && msk != nullptr;
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            hv[i] = zero16();
+            if (MASK) hm[i] = zero16();
+            if (t < kHalo) {
+                const int px = t >> 2, part = t & 3;
+                const int hy = px / kHalW, hx = px % kHalW;
+                const int gy = y0 + hy - kPad, gx = x0 + hx - kPad;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                    const int64_t off = (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8;
+                    hv[i] = *reinterpret_cast<const uint4*>(src + off);
+                    if (MASK && msk) hm[i] = *reinterpret_cast<const uint4*>(msk + off);
+                }
             }
-            *reinterpret_cast<uint4*>(&s_in[px * kCK + part * 8]) = v;
         }
-        // stage weights: TN output channels x taps x 32 input channels
-        for (int t = tid; t < TN * kTaps * 4; t += kBlock) {
-            const int co = t / (kTaps * 4), r = t % (kTaps * 4), tap = r >> 2, part = r & 3;
-            const uint4 v = *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * kTaps + tap) * Cin + ci0 +
-                                                              part * 8);
-            *reinterpret_cast<uint4*>(&s_w[co * kWRow + tap * kCK + part * 8]) = v;
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            wv[i] = zero16();
+            if (t < kWLd) {
+                const int co = t / (kTaps * 4), r = t % (kTaps * 4), tap = r >> 2, part = r & 3;
+                wv[i] = *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * kTaps + tap) * Cin + ci0 +
+                                                        part * 8);
+            }
+        }
+    };
+    load_chunk(0);
+    for (int ci0 = 0; ci0 < Cin; ci0 += kCK) {
+        // registers -> LDS: kHalH x kHalW pixels x 32 channels, TN x taps x 32 weights
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kHalo) {
+                const int px = t >> 2, part = t & 3;
+                uint4 v = hv[i];
+                if (MASK && masked) v = relu_mask(v, hm[MASK ? i : 0]);
+                *reinterpret_cast<uint4*>(&s_in[px * kCK + part * 8]) = v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kWLd) {
+                const int co = t / (kTaps * 4), r = t % (kTaps * 4), tap = r >> 2, part = r & 3;
+                *reinterpret_cast<uint4*>(&s_w[co * kWRow + tap * kCK + part * 8]) = wv[i];
+            }
         }
         __syncthreads();
+        if (ci0 + kCK < Cin) load_chunk(ci0 + kCK);
         for (int tap = 0; tap < kTaps; ++tap) {
             const int ky = tap / KS, kx = tap % KS;
             bf16x8 A[2], B[TN / 16];
@@ -135,6 +178,105 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
                 const int x = x0 + 4 * g + j;
                 if (x >= a.W) continue;
                 float v = acc[mi][ni][j] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
+            }
+        }
+    }
+}
+
+// Register-blocked 3x3 convolution. Each wave owns RW output rows x 16 columns x 16*NB output
+// channels; waves are stacked along rows (4 per block). For one kernel column kx the wave
+// keeps the 3 x NB weight fragments of that column in registers and walks the RW + 2 input
+// rows of its halo once: every input-row fragment read from LDS feeds up to 3 kernel rows x
+// NB output fragments (3*NB MFMAs), instead of the NB of the simple tiling above. The halo
+// keeps 40 channels of pitch per pixel (80 B) and the weight rows 296, so the 16-byte reads
+// of a 16-lane group fall on distinct banks.
+constexpr int kCKP = kCK + 8;
+
+template <int RW, int NB>
+__global__ __launch_bounds__(kBlock, RW * NB >= 32 ? 1 : 2) void k_conv_rb(ConvArgs a) {
+    constexpr int kWaves = kBlock / 64;
+    constexpr int TH = RW * kWaves;     // output rows per block
+    constexpr int TN = 16 * NB;         // output channels per block
+    constexpr int HH = TH + 2, HWd = kTW + 2;
+    __shared__ __attribute__((aligned(16))) bf16 s_in[HH * HWd * kCKP];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[TN * kWRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + TH - 1) / TH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * TH, x0 = tx * kTW;
+    const int co0 = blockIdx.y * TN;
+    const int Cin = a.C0 + a.C1;
+    const int g = lane >> 4, r16 = lane & 15;
+    f32x4 acc[RW][NB];
+    for (int r = 0; r < RW; ++r)
+        for (int ni = 0; ni < NB; ++ni) acc[r][ni] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ci0 = 0; ci0 < Cin; ci0 += kCK) {
+        const bool first = ci0 < a.C0;
+        const bf16* src = first ? a.src0 : a.src1;
+        const bf16* msk = first ? a.mask0 : a.mask1;
+        const int Cs = first ? a.C0 : a.C1;
+        const int cc0 = first ? ci0 : ci0 - a.C0;
+        for (int t = tid; t < HH * HWd * 4; t += kBlock) {
+            const int px = t >> 2, part = t & 3;
+            const int hy = px / HWd, hx = px % HWd;
+            const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+            uint4 v = zero16();
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                const int64_t off = (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8;
+                v = *reinterpret_cast<const uint4*>(src + off);
+                if (msk) v = relu_mask(v, *reinterpret_cast<const uint4*>(msk + off));
+            }
+            *reinterpret_cast<uint4*>(&s_in[px * kCKP + part * 8]) = v;
+        }
+        for (int t = tid; t < TN * 9 * 4; t += kBlock) {
+            const int co = t / 36, r = t % 36, tap = r >> 2, part = r & 3;
+            const uint4 v =
+                *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * 9 + tap) * Cin + ci0 + part * 8);
+            *reinterpret_cast<uint4*>(&s_w[co * kWRow + tap * kCK + part * 8]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+            bf16x8 B[3][NB];
+#pragma unroll
+            for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                for (int ni = 0; ni < NB; ++ni)
+                    B[ky][ni] = *reinterpret_cast<const bf16x8*>(
+                        &s_w[(ni * 16 + r16) * kWRow + (ky * 3 + kx) * kCK + g * 8]);
+#pragma unroll
+            for (int ir = 0; ir < RW + 2; ++ir) {
+                const bf16x8 A =
+                    *reinterpret_cast<const bf16x8*>(&s_in[((wave * RW + ir) * HWd + r16 + kx) * kCKP + g * 8]);
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky) {
+                    const int r = ir - ky;  // output row fed by this input row through kernel row ky
+                    if (r < 0 || r >= RW) continue;
+#pragma unroll
+                    for (int ni = 0; ni < NB; ++ni)
+                        acc[r][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[ky][ni], acc[r][ni], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < RW; ++r) {
+        const int y = y0 + wave * RW + r;
+        if (y >= a.H) continue;
+        for (int ni = 0; ni < NB; ++ni) {
+            const int co = co0 + ni * 16 + r16;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+            bf16* out = co < a.Cs ? a.out0 : a.out1;
+            const int oc = co < a.Cs ? co : co - a.Cs;
+            const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + 4 * g + j;
+                if (x >= a.W) continue;
+                float v = acc[r][ni][j] + b;
                 if (a.relu) v = fmaxf(v, 0.0f);
                 out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
             }
@@ -442,23 +584,125 @@ __global__ void k_tail_reduce(const float* partial, int blocks, float* dw, float
     else db[o] = s;
 }
 
+// ------------------------------------------------------------------ tile-shape selection
+// Convolution variants: 0 = 8x16-pixel tile x 32/64 channels (k_conv), 1..4 = register-blocked
+// k_conv_rb<RW, NB>. Which one is fastest depends on the layer (pixels, channels, how many
+// blocks fill 256 CUs), so the first call of each shape times every variant that fits it on
+// the caller's own buffers and stream, and later calls reuse the winner (the role cuDNN's
+// benchmark mode plays for the reference's PyTorch UNet). Under stream capture, or with
+// tuning off, the shape heuristic below is used.
+enum ConvVariant { kVarSimple = 0, kVarRb4x4, kVarRb8x2, kVarRb4x2, kVarRb8x4, kNumVariants };
+constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8};
+constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4};
+
+struct TuneEntry {
+    int variant;
+    float us[kNumVariants];
+};
+std::mutex g_tune_mu;
+std::map<std::array<int, 9>, TuneEntry> g_tune;
+std::atomic<int> g_tune_mode{1};
+std::atomic<int> g_force_variant{-1};
+
+bool variant_fits(int v, const ConvArgs& a, int taps) {
+    if (v == kVarSimple) return true;
+    if (taps != 9 || v < 0 || v >= kNumVariants) return false;
+    return a.Cout % (16 * kVarNB[v]) == 0;
+}
+
+int heuristic_variant(const ConvArgs& a, int taps) {
+    (void)a;
+    (void)taps;
+    return kVarSimple;
+}
+
+void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
+    if (v == kVarSimple) {
+        const int tiles = cdiv(a.W, kTW) * cdiv(a.H, kTH) * a.N;
+        const bool tn64 = (a.Cout % 64 == 0) && (a.Cs % 64 == 0);
+        const bool m = a.mask0 || a.mask1;
+        const dim3 g64(tiles, a.Cout / 64), g32(tiles, a.Cout / 32);
+        if (taps == 9) {
+            if (tn64 && m) hipLaunchKernelGGL((k_conv<64, 3, true>), g64, dim3(kBlock), 0, st, a);
+            else if (tn64) hipLaunchKernelGGL((k_conv<64, 3, false>), g64, dim3(kBlock), 0, st, a);
+            else if (m) hipLaunchKernelGGL((k_conv<32, 3, true>), g32, dim3(kBlock), 0, st, a);
+            else hipLaunchKernelGGL((k_conv<32, 3, false>), g32, dim3(kBlock), 0, st, a);
+        } else {
+            if (tn64) hipLaunchKernelGGL((k_conv<64, 1, true>), g64, dim3(kBlock), 0, st, a);
+            else hipLaunchKernelGGL((k_conv<32, 1, true>), g32, dim3(kBlock), 0, st, a);
+        }
+        return;
+    }
+    const int th = kVarRW[v] * (kBlock / 64), tn = 16 * kVarNB[v];
+    const dim3 grid(cdiv(a.W, kTW) * cdiv(a.H, th) * a.N, a.Cout / tn);
+    switch (v) {
+        case kVarRb4x4: hipLaunchKernelGGL((k_conv_rb<4, 4>), grid, dim3(kBlock), 0, st, a); break;
+        case kVarRb8x2: hipLaunchKernelGGL((k_conv_rb<8, 2>), grid, dim3(kBlock), 0, st, a); break;
+        case kVarRb4x2: hipLaunchKernelGGL((k_conv_rb<4, 2>), grid, dim3(kBlock), 0, st, a); break;
+        case kVarRb8x4: hipLaunchKernelGGL((k_conv_rb<8, 4>), grid, dim3(kBlock), 0, st, a); break;
+        default: break;
+    }
+}
+
+// Times every fitting variant (one warm-up and three timed launches each) and returns the
+// fastest; us[] receives the per-launch microseconds (0 for variants that do not fit).
+int tune_variant(const ConvArgs& a, int taps, hipStream_t st, float* us) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return -1;
+    }
+    int best = -1;
+    for (int v = 0; v < kNumVariants; ++v) {
+        us[v] = 0.f;
+        if (!variant_fits(v, a, taps)) continue;
+        launch_variant(v, a, taps, st);
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3; ++r) launch_variant(v, a, taps, st);
+        (void)hipEventRecord(e1, st);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) continue;
+        us[v] = ms * 1000.f / 3.f;
+        if (best < 0 || us[v] < us[best]) best = v;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
 lgi_status conv_launch(const ConvArgs& a, int taps, hipStream_t st) {
     const int Cin = a.C0 + a.C1;
     if (a.N <= 0 || a.H <= 0 || a.W <= 0 || Cin <= 0 || a.Cout <= 0) return LGI_ERR_INVALID_ARG;
     if (a.C0 % kCK || a.C1 % kCK || a.Cout % 32 || a.Cs % 32 || a.Cs > a.Cout || (a.C1 && !a.src1))
         return LGI_ERR_INVALID_ARG;
     if (a.Cs < a.Cout && !a.out1) return LGI_ERR_INVALID_ARG;
-    const int tiles = cdiv(a.W, kTW) * cdiv(a.H, kTH) * a.N;
-    const bool tn64 = (a.Cout % 64 == 0) && (a.Cs % 64 == 0);
-    if (taps == 9) {
-        if (tn64) hipLaunchKernelGGL((k_conv<64, 3>), dim3(tiles, a.Cout / 64), dim3(kBlock), 0, st, a);
-        else hipLaunchKernelGGL((k_conv<32, 3>), dim3(tiles, a.Cout / 32), dim3(kBlock), 0, st, a);
-    } else if (taps == 1) {
-        if (tn64) hipLaunchKernelGGL((k_conv<64, 1>), dim3(tiles, a.Cout / 64), dim3(kBlock), 0, st, a);
-        else hipLaunchKernelGGL((k_conv<32, 1>), dim3(tiles, a.Cout / 32), dim3(kBlock), 0, st, a);
+    if (taps != 9 && taps != 1) return LGI_ERR_INVALID_ARG;
+    int v = g_force_variant.load();
+    if (v >= 0) {
+        if (!variant_fits(v, a, taps)) return LGI_ERR_UNSUPPORTED;
     } else {
-        return LGI_ERR_INVALID_ARG;
+        v = heuristic_variant(a, taps);
+        if (taps == 9 && g_tune_mode.load() == 1) {
+            const std::array<int, 9> key{a.N, a.H, a.W, a.C0, a.C1, a.Cout, a.Cs, a.mask0 != nullptr, a.relu};
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) {
+                v = it->second.variant;
+            } else {
+                // timing needs a synchronize, which a capturing stream does not allow
+                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+                    TuneEntry e{};
+                    const int best = tune_variant(a, taps, st, e.us);
+                    if (best >= 0) {
+                        e.variant = v = best;
+                        g_tune.emplace(key, e);
+                    }
+                }
+            }
+        }
     }
+    launch_variant(v, a, taps, st);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
@@ -522,6 +766,34 @@ lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_
                            0);
     }
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_conv_tuning(int32_t mode) {
+    if (mode != 0 && mode != 1) return LGI_ERR_INVALID_ARG;
+    g_tune_mode.store(mode);
+    return LGI_OK;
+}
+
+lgi_status lgi_unet_conv_force_variant(int32_t variant) {
+    if (variant < -1 || variant >= kNumVariants) return LGI_ERR_INVALID_ARG;
+    g_force_variant.store(variant);
+    return LGI_OK;
+}
+
+lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32_t* n) {
+    if (!n || cap < 0 || (cap > 0 && (!rows || !us))) return LGI_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int i = 0;
+    for (const auto& kv : g_tune) {
+        if (i < cap) {
+            for (int k = 0; k < 9; ++k) rows[i * 10 + k] = kv.first[k];
+            rows[i * 10 + 9] = kv.second.variant;
+            for (int v = 0; v < kNumVariants; ++v) us[i * kNumVariants + v] = kv.second.us[v];
+        }
+        ++i;
+    }
+    *n = i;
+    return LGI_OK;
 }
 
 lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {

@@ -431,6 +431,9 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                     (void*)W.rng, (void*)W.T, (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    ctx->kev_used = 0;
+    ctx->kev_overflow = false;
+    lgi_status ms;
     for (int s0 = 0; s0 < P.spp; s0 += S_max) {
         const int S = std::min(S_max, P.spp - s0);
         W.P.sample0 = P.sample0 + s0;
@@ -440,8 +443,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         QRay* other = qb;
         for (int b = 0; b <= P.bounces && n > 0; ++b) {
             LGI_HIP(ctx, hipMemsetAsync(counters, 0, 64, st));
+            if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_trace<false>, dim3(trace_grid(n)), dim3(kBlock), 0, st, ctx->dev.tnodes,
                                ctx->dev.triv, cur, n, hits, (uint8_t*)nullptr, counters + 0, spill);
+            if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sq_tmp,
                                sflag, other, keys_tmp, nflag);
             // order-preserving compaction of shadow rays and continuations (no global atomics)
@@ -455,8 +460,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             if (ns > 0) {
                 hipLaunchKernelGGL(k_wf_compact, dim3(cdiv(nslots, kBlock)), dim3(kBlock), 0, st, sq_tmp,
                                    (const uint32_t*)nullptr, sflag, nslots, sq, (uint32_t*)nullptr, (int32_t*)nullptr);
+                if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
                 hipLaunchKernelGGL(k_wf_trace<true>, dim3(trace_grid(ns)), dim3(kBlock), 0, st, ctx->dev.tnodes,
                                    ctx->dev.triv, sq, ns, (float4*)nullptr, W.nee, counters + 3, spill);
+                if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
             }
             hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
             if (b == P.bounces || nn == 0) break;

@@ -90,6 +90,7 @@ SIGNATURES = {
     "lgi_tonemap": (C.c_int, [_P, _P, C.c_int32, C.c_float, _P]),
     "lgi_gbuffer": (C.c_int, [_P, C.POINTER(Camera), _P, _P, _P]),
     "lgi_last_render_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "lgi_last_trace_ms": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "lgi_render_batch": (C.c_int, [_P, C.POINTER(PoseJob), C.c_int32, C.c_int32, C.c_uint32, _P, _P, _P,
                                     C.c_int32]),
     "lgi_render_tiles": (C.c_int, [_P, C.POINTER(Camera), _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
@@ -104,6 +105,9 @@ SIGNATURES = {
                                             C.c_int32, _P, _P, _P, _P]),
     "lgi_unet_conv3x3_bwd_weight": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,
                                               C.c_int32, _P, _P, _P, C.c_int64, _P]),
+    "lgi_unet_conv_tuning": (C.c_int, [C.c_int32]),
+    "lgi_unet_conv_force_variant": (C.c_int, [C.c_int32]),
+    "lgi_unet_conv_tune_table": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_unet_maxpool_fwd": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "lgi_unet_maxpool_bwd": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "lgi_unet_upsample_fwd": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P, _P]),

@@ -196,6 +196,12 @@ class Context:
         self._c(lib().lgi_last_render_ms(self._h, C.byref(ms)))
         return ms.value
 
+    def last_trace_ms(self):
+        """(summed traversal-kernel ms, launches) of the last render call, from HIP events."""
+        ms, n = C.c_float(), C.c_int32()
+        self._c(lib().lgi_last_trace_ms(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
 
 def _dp(x):
     if x is None:

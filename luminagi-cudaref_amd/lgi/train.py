@@ -28,8 +28,8 @@ class LossScaler:
     """Dynamic loss scaling: halve on inf/nan gradients (step skipped), double every `interval`
     clean steps."""
 
-    def __init__(self, scale=2.0 ** 12, interval=200):
-        self.scale, self.interval, self.clean = scale, interval, 0
+    def __init__(self, scale=2.0 ** 12, interval=200, max_scale=2.0 ** 24):
+        self.scale, self.interval, self.clean, self.max_scale = scale, interval, 0, max_scale
 
     def step(self, params, opt):
         grads = [p.grad for p in params if p.grad is not None]
@@ -45,7 +45,7 @@ class LossScaler:
         opt.zero_grad(set_to_none=True)
         self.clean += 1
         if self.clean % self.interval == 0:
-            self.scale *= 2.0
+            self.scale = min(self.scale * 2.0, self.max_scale)
         return True
 
 
@@ -97,7 +97,9 @@ def random_crops(noisy, clean, batch, crop, gen):
 
 
 def train(noisy, clean, steps=2000, batch=16, crop=256, lr=1e-4, seed=42, val_frac=0.2, time_budget=None,
-          recompute=True, log=print):
+          recompute=True, cosine=False, log=print):
+    """Adam on L1(log1p) with dynamic loss scaling; with cosine=True the learning rate follows a
+    cosine from lr down to lr/20 over `steps`. Returns (model at best validation, checkpoint, s)."""
     n = noisy.shape[0]
     perm = np.random.RandomState(seed).permutation(n)  # seeded split (ml/train.py:152-158)
     n_val = max(1, int(n * val_frac))
@@ -115,6 +117,9 @@ def train(noisy, clean, steps=2000, batch=16, crop=256, lr=1e-4, seed=42, val_fr
         xb, yb = random_crops(tr_n, tr_c, batch, crop, gen)
         loss = (model(xb) - yb).abs().mean()
         (loss * scaler.scale).backward()
+        if cosine:
+            for g in opt.param_groups:
+                g["lr"] = lr * (0.05 + 0.475 * (1.0 + math.cos(math.pi * it / steps)))
         scaler.step(params, opt)
         if (it + 1) % 100 == 0 or it + 1 == steps:
             with torch.no_grad():
@@ -134,6 +139,15 @@ def train(noisy, clean, steps=2000, batch=16, crop=256, lr=1e-4, seed=42, val_fr
     return model, ckpt, time.perf_counter() - t0
 
 
+def export_checkpoint(ckpt):
+    """Inference checkpoint: reference keys {"model", "base", "in_ch"} plus "arch"; 3x3 conv
+    weights stored as BF16 (the convolution kernels round them to BF16 anyway, so the network's
+    output is unchanged), biases and the 1x1 tail in FP32; no optimizer state."""
+    model = {k: (v.to(torch.bfloat16) if k.startswith("weights.") else v) for k, v in ckpt["model"].items()}
+    return {"model": model, "base": ckpt["base"], "in_ch": ckpt["in_ch"], "arch": ckpt["arch"],
+            "best_val": ckpt.get("best_val")}
+
+
 def load_model(path):
     ckpt = torch.load(path, map_location="cpu", weights_only=True)
     if ckpt.get("arch") != ARCH:
@@ -149,6 +163,17 @@ def timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return out, (time.perf_counter() - t) * 1e3
+
+
+def plain_spp_for(pts, psnr):
+    """spp at which the plain curve reaches `psnr`, from MSE ~ 1/spp between the two bracketing
+    measurements (None when the curve does not bracket it)."""
+    if pts and pts[0]["psnr"] >= psnr:
+        return pts[0]["spp"] * 10 ** ((psnr - pts[0]["psnr"]) / 10.0)
+    for a, b in zip(pts, pts[1:]):
+        if a["psnr"] < psnr <= b["psnr"]:
+            return a["spp"] * 10 ** ((psnr - a["psnr"]) / 10.0)
+    return None
 
 
 def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, log=print):
@@ -189,15 +214,7 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, l
         mse, ssim, _ = D.image_metrics(D.tonemap(ctx, out), gt)
         res["unet_fallback"] = {"tiles": nt, "ms": t_render + t_fb, "psnr": D.psnr_from_mse(mse), "ssim": ssim}
         log(f"8 spp + UNet + gated fallback ({nt} tiles): {D.psnr_from_mse(mse):.2f} dB ({t_render + t_fb:.1f} ms)")
-    # plain spp reaching 40 dB: MSE ~ 1/spp between the bracketing measurements
-    pts = res["plain"]
-    spp40 = None
-    for a, b in zip(pts, pts[1:]):
-        if a["psnr"] < 40.0 <= b["psnr"]:
-            spp40 = a["spp"] * 10 ** ((40.0 - a["psnr"]) / 10.0)
-            break
-    if spp40 is None and pts[0]["psnr"] >= 40.0:
-        spp40 = pts[0]["spp"] * 10 ** ((40.0 - pts[0]["psnr"]) / 10.0)
+    spp40 = plain_spp_for(res["plain"], 40.0)
     res["plain_spp_40db"] = spp40
     res["plain_ms_40db"] = None if spp40 is None else spp40 * ms_per_spp
     # the cheapest UNet variant that reaches 40 dB (plain UNet first, then with the fallback)
@@ -208,7 +225,19 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, l
         res["unet_ms"], res["unet_variant_40db"] = min(cands)
         if res["plain_ms_40db"]:
             res["speedup_40db"] = res["plain_ms_40db"] / res["unet_ms"]
+    # plain time to the PSNR that 8 spp + UNet reaches
+    spp_eq = plain_spp_for(res["plain"], res["unet"]["psnr"])
+    if spp_eq is not None:
+        res["plain_spp_same_psnr"] = spp_eq
+        res["plain_ms_same_psnr"] = spp_eq * ms_per_spp
+        res["speedup_same_psnr"] = res["plain_ms_same_psnr"] / res["unet"]["ms"]
     return res
+
+
+def held_out_camera(ctx):
+    """The evaluation pose: an orbit pose drawn from a seed the training poses do not use."""
+    (pos, tgt, _), = orbit_poses(1, seed=12345)
+    return camera_for(ctx.camera(), pos, tgt)
 
 
 def main(argv=None):
@@ -225,19 +254,28 @@ def main(argv=None):
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--out", default="denoiser.pt")
     ap.add_argument("--report", default=None, help="write the evaluation as JSON here")
+    ap.add_argument("--cosine", action="store_true", help="cosine learning-rate decay over --steps")
+    ap.add_argument("--export", default=None, help="also write the compact inference checkpoint here")
+    ap.add_argument("--eval-only", default=None, metavar="CKPT", help="skip training; evaluate this checkpoint")
     args = ap.parse_args(argv)
     ctx = make_scene(1, args.scene_seed, args.width, args.height)
-    (data, t_data) = timed(lambda: make_dataset(ctx, args.poses))
-    print(f"dataset: {args.poses} poses x (8, 256) spp at {args.width}x{args.height} in {t_data / 1e3:.1f} s")
-    model, ckpt, t_train = train(*data, steps=args.steps, batch=args.batch, crop=args.crop, lr=args.lr,
-                                 time_budget=args.time_budget)
-    torch.save(ckpt, args.out)
-    print(f"training: {t_train:.1f} s, best val L1 {ckpt['best_val']:.4f}, saved {args.out}")
-    # held-out pose: an orbit pose from a different seed
-    (pos, tgt, _), = orbit_poses(1, seed=12345)
-    cam = camera_for(ctx.camera(), pos, tgt)
-    res = evaluate(ctx, model.eval(), cam, ref_spp=args.ref_spp)
-    res.update({"train_s": t_train, "dataset_s": t_data / 1e3, "steps": len(ckpt["history"]) and ckpt["history"][-1][0]})
+    extra = {}
+    if args.eval_only:
+        model = load_model(args.eval_only)
+    else:
+        (data, t_data) = timed(lambda: make_dataset(ctx, args.poses))
+        print(f"dataset: {args.poses} poses x (8, 256) spp at {args.width}x{args.height} in {t_data / 1e3:.1f} s")
+        model, ckpt, t_train = train(*data, steps=args.steps, batch=args.batch, crop=args.crop, lr=args.lr,
+                                     time_budget=args.time_budget, cosine=args.cosine)
+        del data
+        torch.save(ckpt, args.out)
+        if args.export:
+            torch.save(export_checkpoint(ckpt), args.export)
+        print(f"training: {t_train:.1f} s, best val L1 {ckpt['best_val']:.4f}, saved {args.out}")
+        extra = {"train_s": t_train, "dataset_s": t_data / 1e3,
+                 "steps": len(ckpt["history"]) and ckpt["history"][-1][0]}
+    res = evaluate(ctx, model.eval(), held_out_camera(ctx), ref_spp=args.ref_spp)
+    res.update(extra)
     print(json.dumps({k: v for k, v in res.items() if k != "plain"}))
     if args.report:
         pathlib.Path(args.report).write_text(json.dumps(res, indent=1))

@@ -42,6 +42,36 @@ def _check(t, dtype, channels=None):
         assert t.shape[-1] == channels, (t.shape, channels)
 
 
+# ------------------------------------------------------------------ tile-shape selection
+CONV_VARIANTS = {0: "8x16 px, 32/64 ch", 1: "16x16 px, 64 ch, rb", 2: "32x16 px, 32 ch, rb",
+                 3: "16x16 px, 32 ch, rb", 4: "32x16 px, 64 ch, rb"}
+
+
+def conv_tuning(enabled=True):
+    """Time the tiling variants on the first call of each conv shape (default) or use a fixed
+    choice."""
+    _call("lgi_unet_conv_tuning", int(bool(enabled)))
+
+
+def force_conv_variant(variant):
+    """Every conv call uses `variant` (None restores per-shape selection)."""
+    _call("lgi_unet_conv_force_variant", -1 if variant is None else int(variant))
+
+
+def conv_tune_table():
+    """[(shape dict, chosen variant, {variant: us per launch})] of the shapes timed so far."""
+    import numpy as np
+    n = C.c_int32()
+    _call("lgi_unet_conv_tune_table", None, None, 0, C.byref(n))
+    cap = n.value
+    rows = np.zeros((max(cap, 1), 10), np.int32)
+    us = np.zeros((max(cap, 1), len(CONV_VARIANTS)), np.float32)
+    _call("lgi_unet_conv_tune_table", C.c_void_p(rows.ctypes.data), C.c_void_p(us.ctypes.data), cap, C.byref(n))
+    keys = ("n", "h", "w", "c0", "c1", "cout", "cout_split", "masked", "relu")
+    return [(dict(zip(keys, map(int, r[:9]))), int(r[9]), {v: float(t) for v, t in enumerate(u) if t > 0})
+            for r, u in zip(rows[:min(cap, n.value)], us)]
+
+
 # ------------------------------------------------------------------ ops
 def conv3x3_fwd(src0, src1, w_bf16, bias, relu):
     n, h, w, c0 = src0.shape

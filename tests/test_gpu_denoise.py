@@ -61,3 +61,71 @@ def test_training_loop_runs_and_checkpoint_round_trips(gpu, tmp_path):
     with torch.no_grad():
         assert torch.equal(model(x), m2(x))
     assert set(ckpt) >= {"model", "base", "in_ch", "arch", "optimizer", "loss_scale"}
+
+
+CKPT = __import__("pathlib").Path(__file__).resolve().parent.parent / "luminagi-cudaref_amd" / "ml" / "weights" / \
+    "unet_log1p_c32.pt"
+needs_ckpt = pytest.mark.skipif(not CKPT.exists(), reason="trained denoiser checkpoint not present")
+
+
+@needs_ckpt
+def test_trained_denoiser_gains_over_the_plain_tracer(gpu):
+    """What the learned part buys: on the held-out pose (864x432, error against 1024 spp),
+    8 spp + UNet must beat both its own 8-spp input and the plain tracer at 4x the samples."""
+    import torch
+    from lgi import denoise as D
+    from lgi import train
+    ctx = train.make_scene(1, 1, 864, 432)
+    cam = train.held_out_camera(ctx)
+    gt = D.tonemap(ctx, train.render_mean(ctx, cam, 1024, 0, batch=256))
+    model = train.load_model(CKPT)
+
+    def psnr(hdr):
+        return D.psnr_from_mse(D.image_metrics(D.tonemap(ctx, hdr), gt)[0])
+
+    noisy = train.render_mean(ctx, cam, 8, 1 << 24)
+    plain32 = train.render_mean(ctx, cam, 32, 1 << 25)
+    p8, p32, pu = psnr(noisy), psnr(plain32), psnr(D.denoise(model, noisy))
+    # an untrained network of the same architecture gains nothing
+    pr = psnr(D.denoise(__import__("lgi.unet", fromlist=["UNet"]).UNet(seed=0).cuda().eval(), noisy))
+    print(f"held-out 864x432: 8 spp {p8:.2f} dB, 32 spp {p32:.2f} dB, 8 spp + UNet {pu:.2f} dB, "
+          f"8 spp + untrained UNet {pr:.2f} dB", flush=True)
+    assert pu > p8 + 6.0
+    assert pu > p32 + 1.0
+    assert pr < pu - 6.0
+
+
+@needs_ckpt
+def test_trained_weights_per_layer_error_vs_fp64(gpu):
+    """Per-level error of the BF16 MFMA forward of the trained weights on a lit 64x128 crop of a
+    real 8-spp frame (log1p radiance) against the FP64 CPU forward of the same weights, and
+    against the FP64 forward that rounds its stored activations to BF16 as the GPU network
+    does (which separates the kernels' own error from the network's BF16 sensitivity)."""
+    import torch
+    from lgi import train
+    from oracle import unet_ref
+    ctx = train.make_scene(1, 1, 1728, 864)
+    cam = train.held_out_camera(ctx)
+    x = torch.log1p(train.render_mean(ctx, cam, 8, 1 << 24))[640:704, 960:1088].unsqueeze(0).contiguous()
+    assert float((x.mean(-1) > 0).float().mean()) > 0.99  # the crop is lit
+    model = train.load_model(CKPT)
+    taps, r64, r16 = {}, {}, {}
+    with torch.no_grad():
+        y = model(x, taps=taps)
+        ref = unet_ref.forward(model.state_dict(), x.cpu(), taps=r64)
+        ref16 = unet_ref.forward(model.state_dict(), x.cpu(), taps=r16, bf16_activations=True)
+
+    def rel_l2(a, b):
+        a, b = a.detach().double().cpu(), b.detach().double().cpu()
+        return float((a - b).norm() / b.norm().clamp_min(1e-30))
+
+    xd = x.double().cpu()
+    rows = {k: (rel_l2(taps[k], r64[k]), rel_l2(taps[k], r16[k])) for k in r64}
+    rows["residual (output - input)"] = (rel_l2(y - x, ref - xd), rel_l2(y - x, ref16 - xd))
+    rows["output"] = (rel_l2(y, ref), rel_l2(y, ref16))
+    print("  level                      rel-L2 vs FP64   vs FP64 with BF16 activations")
+    for k, (e64, e16) in rows.items():
+        print(f"  {k:26s} {e64:.2e}         {e16:.2e}", flush=True)
+    assert all(e16 < 1e-2 for _, e16 in rows.values())
+    assert all(e64 < 3e-2 for k, (e64, _) in rows.items() if not k.startswith("residual"))
+    assert rows["residual (output - input)"][0] < 6e-2 and rows["output"][0] < 5e-3

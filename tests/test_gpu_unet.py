@@ -40,8 +40,21 @@ CONV_CASES = [  # n, h, w, c0, c1, cout, relu
 ]
 
 
+@pytest.fixture(params=[None, 0, 1, 2, 3, 4], ids=lambda v: "tuned" if v is None else f"variant{v}")
+def conv_variant(request, gpu):
+    """Every tiling variant forced in turn (None = per-shape selection by timing)."""
+    from lgi import unet
+    v = request.param
+    cout = request.node.callspec.params["case"][5]
+    if v in (1, 4) and cout % 64:
+        pytest.skip("64-channel tiling needs cout % 64 == 0")
+    unet.force_conv_variant(v)
+    yield v
+    unet.force_conv_variant(None)
+
+
 @pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
-def test_conv3x3_forward(gpu, case):
+def test_conv3x3_forward(conv_variant, case):
     from lgi import unet
     n, h, w, c0, c1, cout, relu = case
     x0 = bf16_rand(n, h, w, c0, seed=1)
@@ -51,12 +64,12 @@ def test_conv3x3_forward(gpu, case):
     y = unet.conv3x3_fwd(x0.cuda(), None if x1 is None else x1.cuda(), wt.cuda(), b.cuda(), relu)
     ref = ref_conv_nhwc(x0, x1, wt, b, relu)
     e = rel(y, ref)
-    print(f"conv fwd {case}: rel err {e:.2e}", flush=True)
+    print(f"conv fwd {case} variant {conv_variant}: rel err {e:.2e}", flush=True)
     assert y.shape == ref.shape and e < BF16_TOL
 
 
 @pytest.mark.parametrize("case", CONV_CASES, ids=lambda c: "x".join(map(str, c)))
-def test_conv3x3_backward(gpu, case):
+def test_conv3x3_backward(conv_variant, case):
     from lgi import unet
     n, h, w, c0, c1, cout, relu = case
     x0 = bf16_rand(n, h, w, c0, seed=5)
@@ -78,7 +91,7 @@ def test_conv3x3_backward(gpu, case):
     errs = {"dx0": rel(dx0, xs[0].grad), "dw": rel(dw, wd.grad), "db": rel(db, bd.grad)}
     if c1:
         errs["dx1"] = rel(dx1, xs[1].grad)
-    print(f"conv bwd {case}: {errs}", flush=True)
+    print(f"conv bwd {case} variant {conv_variant}: {errs}", flush=True)
     assert errs["dx0"] < BF16_TOL and errs.get("dx1", 0) < BF16_TOL
     assert errs["dw"] < 1e-2 and errs["db"] < 1e-2   # FP32 outputs of BF16 products
 
@@ -249,3 +262,31 @@ def test_conv_gradients_by_finite_differences(gpu):
     an_p = float((Xr.grad.double() * Vp.double()).sum())
     print(f"finite differences: maxpool {fd_p:.6g} vs {an_p:.6g}", flush=True)
     assert abs(fd_p - an_p) <= 1e-2 * max(1.0, abs(an_p))
+
+
+def test_conv_variant_unsupported_shape_is_an_error(gpu):
+    from lgi import unet
+    from lgi._abi import LgiError
+    x = torch.zeros(1, 8, 16, 32, dtype=torch.bfloat16, device="cuda")
+    w = torch.zeros(96, 9, 32, dtype=torch.bfloat16, device="cuda")
+    unet.force_conv_variant(1)
+    try:
+        with pytest.raises(LgiError, match="UNSUPPORTED"):
+            unet.conv3x3_fwd(x, None, w, torch.zeros(96, device="cuda"), True)
+    finally:
+        unet.force_conv_variant(None)
+
+
+def test_conv_tuning_table_records_each_shape_once(gpu):
+    from lgi import unet
+    unet.conv_tuning(True)
+    x = bf16_rand(1, 40, 48, 64, seed=11).cuda()
+    w = bf16_rand(64, 9, 64, scale=0.05, seed=12).cuda()
+    b = torch.zeros(64, device="cuda")
+    y1 = unet.conv3x3_fwd(x, None, w, b, True)
+    y2 = unet.conv3x3_fwd(x, None, w, b, True)
+    rows = [r for r in unet.conv_tune_table() if (r[0]["h"], r[0]["w"], r[0]["c0"], r[0]["cout"]) == (40, 48, 64, 64)]
+    assert len(rows) == 1
+    shape, chosen, times = rows[0]
+    assert set(times) == {0, 1, 2, 3, 4} and chosen == min(times, key=times.get)
+    assert torch.equal(y1, y2)
