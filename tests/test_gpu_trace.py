@@ -118,6 +118,35 @@ def test_sorted_wavefront_matches_inline_and_oracle(gpu, flags):
     assert_close_images(wave[0].cpu().numpy(), o["sum"], f"sorted flags={flags}")
 
 
+def test_converged_means_agree_across_rng_policies(gpu):
+    """4096-spp means of the PCG32 sorted renderer and of the reference's xorshift32 seeding
+    (src/pathtracer.cu:345-364) agree within 3 standard errors of their difference: the RNG
+    choice changes the noise, not the estimator. Independent sample sets, per-pixel
+    luminance, 64x32 pixels."""
+    import lgi
+    ctx, _, s = make_pair(seed=3)
+    cam = small_camera(s["camera"], 64, 32)
+    lw = np.array([0.2126, 0.7152, 0.0722])
+    stats = []
+    for flags in (lgi.RENDER_SORTED, lgi.RNG_XORSHIFT_REF):
+        g = gpu_accumulate(ctx, cam, 4096, 2, flags=flags)
+        n = 4096.0
+        m = (g[0].cpu().numpy().reshape(-1, 3) / n) @ lw
+        var = np.maximum(g[2].cpu().numpy() / n - m * m, 0.0) * n / (n - 1)
+        stats.append((m, var / n))
+    (m0, v0), (m1, v1) = stats
+    lit = (v0 + v1) > 0
+    z = np.abs(m0 - m1)[lit] / np.sqrt(v0 + v1)[lit]
+    print(f"converged 4096 spp: {lit.sum()} lit pixels, max |z| {z.max():.2f}, "
+          f"frac |z|>3 {np.mean(z > 3):.4f}, mean z^2 {np.mean(z * z):.3f}", flush=True)
+    assert lit.sum() > 0.3 * lit.size
+    assert np.mean(z > 3) < 0.01             # ~0.27% expected beyond 3 sigma
+    assert 0.7 < np.mean(z * z) < 1.3        # chi-square per pixel ~ 1 when unbiased
+    # image-wide means agree within 3 standard errors of the mean over pixels
+    d = (m0 - m1)[lit]
+    assert abs(d.mean()) < 3 * np.sqrt((v0 + v1)[lit].sum()) / lit.sum()
+
+
 def test_sorted_rejects_xorshift(gpu):
     import lgi
     ctx, _, s = make_pair(seed=6, w=32, h=16)
