@@ -58,21 +58,23 @@ struct Hit {
 // Traversal stack: the top kLds entries live in LDS as [depth][thread] (lanes at different
 // depths still hit distinct banks), deeper entries in a per-thread global slot laid out as
 // [depth - kLds][thread] (coalesced). The reference tree has at most 30 internal levels (one
-// split per Morton bit), so kLds + kSpill = 32 entries can never overflow; 16 LDS entries
+// split per Morton bit), so kStackMax = 32 entries in total can never overflow; 16 LDS entries
 // (16 KB per 256-thread block) keep 8 waves per SIMD resident.
-struct TravStack {
+template <int kLds>
+struct TravStackT {
     int* lds;          // s_stack + threadIdx.x, stride kBlockThreads
     int* spill;        // spill + global thread id, stride spill_stride
     int spill_stride;
     static constexpr int kBlockThreads = 256;
     LGI_D void push(int sp, int v) const {
-        if (sp < kStackLds) lds[sp * kBlockThreads] = v;
-        else spill[(sp - kStackLds) * spill_stride] = v;
+        if (sp < kLds) lds[sp * kBlockThreads] = v;
+        else spill[(sp - kLds) * spill_stride] = v;
     }
     LGI_D int pop(int sp) const {
-        return sp < kStackLds ? lds[sp * kBlockThreads] : spill[(sp - kStackLds) * spill_stride];
+        return sp < kLds ? lds[sp * kBlockThreads] : spill[(sp - kLds) * spill_stride];
     }
 };
+using TravStack = TravStackT<kStackLds>;
 
 // Shading normal at the closest hit (src/pathtracer.cu:149-155).
 LGI_D V3 hit_normal(const TriS* tris, const Hit& h, int& mat) {

@@ -26,8 +26,12 @@ struct TraceParams {
 };
 
 // Traversal stack split: entries [0, kStackLds) in LDS, [kStackLds, 32) in global spill slots.
+// Spill areas are sized for kStackMax - kStackLdsMin entries so that kernels holding fewer
+// stack entries in LDS (to make room for the top BVH levels) fit the same allocation.
+constexpr int kStackMax = 32;
 constexpr int kStackLds = 16;
-constexpr int kStackSpill = 16;
+constexpr int kStackLdsMin = 8;
+constexpr int kStackSpill = kStackMax - kStackLdsMin;
 
 // Threads of the pixel-grid kernels (16x16-pixel blocks) for a W x H image.
 inline int64_t lgi_grid_threads(int w, int h) { return (int64_t)((w + 15) / 16) * ((h + 15) / 16) * 256; }
