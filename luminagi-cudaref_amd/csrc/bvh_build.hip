@@ -12,9 +12,9 @@
 //      first key != last key), i.e. the reference leaf rule (src/bvh.cpp:92)
 //   6. BFS numbering (src/bvh.cpp:124-162): sort reference nodes by (depth, first index);
 //      within a level, left-to-right order is range order
-//   7. bottom-up AABB refit with per-node atomic arrival counters; leaf boxes from the
-//      triangle vertices in order, parent box = left box then strict-union right box
-//      (src/bvh.cpp:96-103, 115-117)
+//   7. bottom-up AABB refit, one launch per level from the deepest up (no atomics); leaf
+//      boxes from the triangle vertices in order, parent box = left box then strict-union
+//      right box (src/bvh.cpp:96-103, 115-117)
 //   8. traversal layout: TNode (child boxes in the parent), TriV (v0, e1, e2), TriS.
 #include <algorithm>
 #include <cstring>
