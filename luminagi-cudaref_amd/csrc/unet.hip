@@ -419,42 +419,62 @@ __global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* dy, const bf16
 }
 
 // ------------------------------------------------------------------ pool / upsample
+// Elementwise layers move 8 channels (16 bytes) per thread; C is a multiple of 8.
+__device__ __forceinline__ void unpack8(const uint4& v, float f[8]) {
+    const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+    for (int k = 0; k < 8; ++k) f[k] = (float)e[k];
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    bf16x8 e;
+    for (int k = 0; k < 8; ++k) e[k] = (bf16)f[k];
+    return *reinterpret_cast<const uint4*>(&e);
+}
+
 // MaxPool2d(2): out[n,y,x,c] = max over the 2x2 window (first maximum in row-major order).
 __global__ __launch_bounds__(kBlock) void k_pool_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int Ho = H / 2, Wo = W / 2;
-    if (i >= (int64_t)N * Ho * Wo * C) return;
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    if (i >= (int64_t)N * Ho * Wo * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const int64_t p = i / C8;
     const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
     const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
-    float m = (float)in[base];
-    const float v1 = (float)in[base + C], v2 = (float)in[base + (int64_t)W * C], v3 = (float)in[base + (int64_t)W * C + C];
-    if (v1 > m || v1 != v1) m = v1;
-    if (v2 > m || v2 != v2) m = v2;
-    if (v3 > m || v3 != v3) m = v3;
-    out[i] = (bf16)m;
+    const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
+    float m[8], v[8];
+    unpack8(*reinterpret_cast<const uint4*>(in + base), m);
+    for (int k = 1; k < 4; ++k) {
+        unpack8(*reinterpret_cast<const uint4*>(in + base + o[k]), v);
+        for (int e = 0; e < 8; ++e)
+            if (v[e] > m[e] || v[e] != v[e]) m[e] = v[e];
+    }
+    *reinterpret_cast<uint4*>(out + p * C + c) = pack8(m);
 }
 
 // Gradient to the first maximum of each window (the pooled input is re-read, not stored).
 __global__ __launch_bounds__(kBlock) void k_pool_bwd(const bf16* in, const bf16* dout, int N, int H, int W, int C,
                                                      bf16* din) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int Ho = H / 2, Wo = W / 2;
-    if (i >= (int64_t)N * Ho * Wo * C) return;
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    if (i >= (int64_t)N * Ho * Wo * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const int64_t p = i / C8;
     const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
     const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
     const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
-    float m = (float)in[base];
-    int arg = 0;
+    float m[8], v[8];
+    int arg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unpack8(*reinterpret_cast<const uint4*>(in + base), m);
     for (int k = 1; k < 4; ++k) {
-        const float v = (float)in[base + o[k]];
-        if (v > m || v != v) { m = v; arg = k; }
+        unpack8(*reinterpret_cast<const uint4*>(in + base + o[k]), v);
+        for (int e = 0; e < 8; ++e)
+            if (v[e] > m[e] || v[e] != v[e]) { m[e] = v[e]; arg[e] = k; }
     }
-    const bf16 g = dout[i];
-    for (int k = 0; k < 4; ++k) din[base + o[k]] = k == arg ? g : (bf16)0.0f;
+    float g[8], z[8];
+    unpack8(*reinterpret_cast<const uint4*>(dout + p * C + c), g);
+    for (int k = 0; k < 4; ++k) {
+        for (int e = 0; e < 8; ++e) z[e] = arg[e] == k ? g[e] : 0.0f;
+        *reinterpret_cast<uint4*>(din + base + o[k]) = pack8(z);
+    }
 }
 
 // Bilinear 2x (align_corners = False): source coordinate (d + 0.5) / 2 - 0.5 clamped at 0.
@@ -468,30 +488,36 @@ __device__ __forceinline__ void lin_src(int d, int in, int& i0, int& i1, float& 
 
 __global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int Ho = 2 * H, Wo = 2 * W;
-    if (i >= (int64_t)N * Ho * Wo * C) return;
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
+    const int Ho = 2 * H, Wo = 2 * W, C8 = C / 8;
+    if (i >= (int64_t)N * Ho * Wo * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const int64_t p = i / C8;
     const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
     int y0, y1, x0, x1;
     float ly, lx;
     lin_src(y, H, y0, y1, ly);
     lin_src(x, W, x0, x1, lx);
-    auto at = [&](int yy, int xx) { return (float)in[(((int64_t)n * H + yy) * W + xx) * C + c]; };
-    const float v = (1.f - ly) * ((1.f - lx) * at(y0, x0) + lx * at(y0, x1)) + ly * ((1.f - lx) * at(y1, x0) + lx * at(y1, x1));
-    out[i] = (bf16)v;
+    auto at = [&](int yy, int xx, float f[8]) {
+        unpack8(*reinterpret_cast<const uint4*>(in + (((int64_t)n * H + yy) * W + xx) * C + c), f);
+    };
+    float a[8], b[8], cc[8], d[8], v[8];
+    at(y0, x0, a); at(y0, x1, b); at(y1, x0, cc); at(y1, x1, d);
+    for (int e = 0; e < 8; ++e)
+        v[e] = (1.f - ly) * ((1.f - lx) * a[e] + lx * b[e]) + ly * ((1.f - lx) * cc[e] + lx * d[e]);
+    *reinterpret_cast<uint4*>(out + p * C + c) = pack8(v);
 }
 
 // Adjoint of k_up_fwd as a gather: each input pixel sums the weighted gradients of the
 // (at most 4 x 4) output pixels that read it.
 __global__ __launch_bounds__(kBlock) void k_up_bwd(const bf16* dout, int N, int H, int W, int C, bf16* din) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= (int64_t)N * H * W * C) return;
-    const int c = (int)(i % C);
-    const int64_t p = i / C;
+    const int C8 = C / 8;
+    if (i >= (int64_t)N * H * W * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const int64_t p = i / C8;
     const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
     const int Ho = 2 * H, Wo = 2 * W;
-    float acc = 0.f;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, g[8];
     for (int oy = max(0, 2 * y - 2); oy <= min(Ho - 1, 2 * y + 2); ++oy) {
         int y0, y1;
         float ly;
@@ -504,19 +530,24 @@ __global__ __launch_bounds__(kBlock) void k_up_bwd(const bf16* dout, int N, int 
             lin_src(ox, W, x0, x1, lx);
             float wx = (x0 == x ? 1.f - lx : 0.f) + (x1 == x ? lx : 0.f);
             if (wx == 0.f) continue;
-            acc += wy * wx * (float)dout[(((int64_t)n * Ho + oy) * Wo + ox) * C + c];
+            unpack8(*reinterpret_cast<const uint4*>(dout + (((int64_t)n * Ho + oy) * Wo + ox) * C + c), g);
+            const float w = wy * wx;
+            for (int e = 0; e < 8; ++e) acc[e] += w * g[e];
         }
     }
-    din[i] = (bf16)acc;
+    *reinterpret_cast<uint4*>(din + p * C + c) = pack8(acc);
 }
 
 // ------------------------------------------------------------------ input / tail
-// log1p radiance (FP32, NHWC with 3 channels) -> BF16 NHWC padded to 32 channels.
+// log1p radiance (FP32, NHWC with 3 channels) -> BF16 NHWC padded to 32 channels; one pixel
+// (64 bytes out) per thread.
 __global__ __launch_bounds__(kBlock) void k_pack_input(const float* x, int64_t npx, bf16* out) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= npx * 32) return;
-    const int c = (int)(i % 32);
-    out[i] = c < 3 ? (bf16)x[(i / 32) * 3 + c] : (bf16)0.0f;
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npx) return;
+    float f[8] = {x[p * 3], x[p * 3 + 1], x[p * 3 + 2], 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint4* o = reinterpret_cast<uint4*>(out + p * 32);
+    o[0] = pack8(f);
+    o[1] = o[2] = o[3] = zero16();
 }
 
 // out = x + W_t * feat + b_t (1x1 conv, 32 -> 3, FP32 math).
@@ -797,8 +828,8 @@ lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32
 }
 
 lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
-    if (!in || !out || h % 2 || w % 2) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * c;
+    if (!in || !out || h % 2 || w % 2 || c % 8) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * (c / 8);
     hipLaunchKernelGGL(k_pool_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n,
                        h, w, c, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
@@ -806,24 +837,24 @@ lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w,
 
 lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int32_t h, int32_t w, int32_t c,
                                 void* din, void* stream) {
-    if (!in || !dout || !din || h % 2 || w % 2) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * c;
+    if (!in || !dout || !din || h % 2 || w % 2 || c % 8) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * (c / 8);
     hipLaunchKernelGGL(k_pool_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in,
                        (const bf16*)dout, n, h, w, c, (bf16*)din);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_upsample_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
-    if (!in || !out) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * 2 * h * 2 * w * c;
+    if (!in || !out || c % 8) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * 2 * h * 2 * w * (c / 8);
     hipLaunchKernelGGL(k_up_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n, h,
                        w, c, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_upsample_bwd(const void* dout, int32_t n, int32_t h, int32_t w, int32_t c, void* din, void* stream) {
-    if (!dout || !din) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * h * w * c;
+    if (!dout || !din || c % 8) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)n * h * w * (c / 8);
     hipLaunchKernelGGL(k_up_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)dout, n,
                        h, w, c, (bf16*)din);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
@@ -831,7 +862,7 @@ lgi_status lgi_unet_upsample_bwd(const void* dout, int32_t n, int32_t h, int32_t
 
 lgi_status lgi_unet_pack_input(const float* x, int64_t npx, void* out, void* stream) {
     if (!x || !out) return LGI_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(k_pack_input, dim3(cdiv(npx * 32, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, x, npx,
+    hipLaunchKernelGGL(k_pack_input, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, x, npx,
                        (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }

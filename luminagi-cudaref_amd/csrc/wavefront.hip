@@ -18,6 +18,8 @@
 // (pixel, sample), same expression trees, same light order).
 #include "trace_device.h"
 
+#include <climits>
+
 using namespace lgi;
 
 namespace {
@@ -125,7 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
     uint32_t cbase = 0, cend = 0;
     bool exhausted = false;
     bool active = false;
-    int qi = 0, node = 0, sp = 0, best_tri = -1, slot = 0;
+    // node: >= 0 internal node to visit, < 0 leaf (~first triangle), kNone when the stack is
+    // exhausted; leaf: a parked leaf (0 = none)
+    constexpr int kNone = INT_MIN;
+    int qi = 0, node = 0, sp = 0, leaf = 0, best_tri = -1, slot = 0;
     float best_t = 0.f, best_u = 0.f, best_v = 0.f;
     Ray r;
     for (;;) {
@@ -161,6 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                         best_t = tmax;
                         best_tri = -1;
                         node = 0;
+                        leaf = 0;
                         sp = 0;
                         active = true;
                     } else if (!kAny) {
@@ -183,9 +189,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
             continue;
         }
         if (!active) continue;
-        // one traversal step
-        bool done = false;
-        if (node >= 0) {
+        // Phase 1: internal nodes. A lane that reaches a leaf parks it and keeps traversing
+        // (speculatively) until every lane still in this loop holds a parked leaf, so the
+        // wave then tests triangles with most lanes busy instead of alternating one node step
+        // with one whole leaf per iteration.
+        while (node >= 0) {
             const TNode nd = (kTop > 0 && node < kTop) ? s_top[node] : tnodes[node];
             float ta, tb;
             const bool ha = slab(r, nd.b0xy.x, nd.b0xy.y, nd.b0xy.z, nd.b0xy.w, nd.bz.x, nd.bz.y, best_t, ta);
@@ -200,14 +208,23 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 node = nd.ch.x;
             } else if (hb) {
                 node = nd.ch.y;
-            } else if (sp == 0) {
-                done = true;
             } else {
-                --sp;
-                node = stk.pop(sp);
+                node = sp > 0 ? stk.pop(--sp) : kNone;
             }
-        } else {
-            int ti = ~node;
+            if (node < 0 && node != kNone && leaf == 0) {
+                leaf = node;
+                node = sp > 0 ? stk.pop(--sp) : kNone;
+            }
+            if (__ballot(leaf == 0) == 0) break;
+        }
+        // Phase 2: the parked leaf, then the current node for as long as it is a leaf.
+        while (leaf != 0 || (node < 0 && node != kNone)) {
+            if (leaf == 0) {
+                leaf = node;
+                node = sp > 0 ? stk.pop(--sp) : kNone;
+            }
+            int ti = ~leaf;
+            leaf = 0;
             for (;;) {
                 const TriV t = triv[ti];
                 float tt, uu, vv;
@@ -218,14 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 if (__float_as_int(t.v0.w) != 0) break;
                 ++ti;
             }
-            if ((kAny && best_tri >= 0) || sp == 0) {
-                done = true;
-            } else {
-                --sp;
-                node = stk.pop(sp);
+            if (kAny && best_tri >= 0) {
+                node = kNone;
+                sp = 0;
+                break;
             }
         }
-        if (done) {
+        if (node == kNone && leaf == 0) {
             if (kAny) {
                 if (best_tri >= 0) nee[slot] = 2;
             } else {

@@ -199,12 +199,17 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, l
     _, t_render = timed(lambda: fr.render(ctx, 8, bounces, sample0=test0 + (1 << 20), flags=flags))
     D.denoise(model, fr.mean())  # warm-up: first-call allocations are not part of the steady state
     den, t_den = timed(lambda: D.denoise(model, fr.mean()))
+    graph = D.DenoiseGraph(model, cam.height, cam.width)
+    graph(fr.mean())
+    den_g, t_den_g = timed(lambda: graph(fr.mean()))
+    assert torch.equal(den_g, den), "graph replay differs from the eager forward"
     mse, ssim, mad = D.image_metrics(D.tonemap(ctx, den), gt)
-    res["unet"] = {"spp": 8, "render_ms": t_render, "denoise_ms": t_den, "psnr": D.psnr_from_mse(mse), "ssim": ssim}
+    res["unet"] = {"spp": 8, "render_ms": t_render, "denoise_ms": t_den_g, "denoise_eager_ms": t_den,
+                   "psnr": D.psnr_from_mse(mse), "ssim": ssim}
     mse8, ssim8, _ = D.image_metrics(D.tonemap(ctx, fr.mean()), gt)
     res["noisy8"] = {"psnr": D.psnr_from_mse(mse8), "ssim": ssim8}
     log(f"8 spp raw: {D.psnr_from_mse(mse8):.2f} dB; 8 spp + UNet: {res['unet']['psnr']:.2f} dB ssim {ssim:.4f} "
-        f"({t_render:.1f} + {t_den:.1f} ms)")
+        f"({t_render:.1f} + {t_den_g:.1f} ms; eager denoise {t_den:.1f} ms)")
     if fallback:
         warm = D.Frame(cam)  # warm-up of the tile path on a throw-away frame
         warm.render(ctx, 8, bounces, sample0=test0 + (1 << 22), flags=flags)

@@ -68,6 +68,23 @@ CKPT = __import__("pathlib").Path(__file__).resolve().parent.parent / "luminagi-
 needs_ckpt = pytest.mark.skipif(not CKPT.exists(), reason="trained denoiser checkpoint not present")
 
 
+def test_denoise_graph_replays_match_eager(gpu):
+    """The captured UNet graph gives the eager forward's bits on every replay, with eager work
+    and synchronisations between the replays."""
+    import torch
+    from lgi import denoise as D
+    from lgi.unet import UNet
+    model = UNet(seed=3).cuda().eval()
+    g = D.DenoiseGraph(model, 64, 96)
+    for k in range(4):
+        x = torch.rand(64, 96, 3, device="cuda", generator=torch.Generator("cuda").manual_seed(k)) * 4.0
+        want = D.denoise(model, x)
+        got = g(x).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(got, want), f"replay {k}"
+        _ = (x * 2).sum().item()  # eager work between replays
+
+
 @needs_ckpt
 def test_trained_denoiser_gains_over_the_plain_tracer(gpu):
     """What the learned part buys: on the held-out pose (864x432, error against 1024 spp),
