@@ -215,6 +215,13 @@ LGI_API lgi_status lgi_write_png(const char* path, const uint8_t* rgb, int32_t w
 LGI_API lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1,
                                         const void* w, const float* bias, int32_t cout, int32_t n,
                                         int32_t h, int32_t w_, int32_t relu, void* out, void* stream);
+/* First layer for inference: y = relu?(conv3x3(x) + bias) with x the FP32 3-channel NHWC
+ * input (read directly, no padded copy), w [32][9][cin_w] BF16 of which channels 0..2 are used,
+ * y (n, h, w, 32) BF16. Equals lgi_unet_conv3x3_fwd on lgi_unet_pack_input(x) up to FP32
+ * summation order (the 27 products of a pixel are one MFMA row). */
+LGI_API lgi_status lgi_unet_conv3x3_first_fwd(const float* x, const void* w, int32_t cin_w, const float* bias,
+                                              int32_t n, int32_t h, int32_t w_, int32_t relu, void* out,
+                                              void* stream);
 /* dx = conv3x3^T(dy * (ymask > 0)); channels [0, cin_split) -> dx0, the rest -> dx1.
  * ymask may be null (no ReLU); wt_scratch holds cout*9*cin BF16. */
 LGI_API lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout,

@@ -284,6 +284,72 @@ __global__ __launch_bounds__(kBlock, RW * NB >= 32 ? 1 : 2) void k_conv_rb(ConvA
     }
 }
 
+// First convolution straight from the FP32 3-channel input: the 3x3x3 patch of a pixel is
+// one K = 27 (+5 zero) row, so a 16-pixel x 16-channel output fragment is a single MFMA
+// instead of the 9 taps x 32 (mostly zero) channels of the packed-input path. Output tile
+// 8 x 16 pixels x 32 channels; weights [32][9][cin_w] BF16 (channels 0..2 used).
+__global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__ x, const bf16* __restrict__ wk,
+                                                       int cin_w, const float* __restrict__ bias, int N, int H, int W,
+                                                       int relu, bf16* __restrict__ out) {
+    __shared__ bf16 s_in[kHH * kHW * 4];  // halo pixels x 4 (3 channels + pad)
+    __shared__ __attribute__((aligned(16))) bf16 s_w[32 * 40];  // [co][k], k = tap * 3 + c, 40 = padded row
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (W + kTW - 1) / kTW, tiles_y = (H + kTH - 1) / kTH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * kTH, x0 = tx * kTW;
+    for (int t = tid; t < kHH * kHW; t += kBlock) {
+        const int hy = t / kHW, hx = t % kHW;
+        const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+            const float* px = x + (((int64_t)n * H + gy) * W + gx) * 3;
+            v0 = px[0]; v1 = px[1]; v2 = px[2];
+        }
+        s_in[t * 4 + 0] = (bf16)v0; s_in[t * 4 + 1] = (bf16)v1; s_in[t * 4 + 2] = (bf16)v2;
+        s_in[t * 4 + 3] = (bf16)0.f;
+    }
+    for (int t = tid; t < 32 * 32; t += kBlock) {
+        const int co = t >> 5, k = t & 31;
+        s_w[co * 40 + k] = k < 27 ? wk[((int64_t)co * 9 + k / 3) * cin_w + k % 3] : (bf16)0.f;
+    }
+    __syncthreads();
+    const int g = lane >> 4, r16 = lane & 15;
+    f32x4 acc[2][2];
+    for (int mi = 0; mi < 2; ++mi) {
+        bf16x8 A;
+        for (int i = 0; i < 8; ++i) {
+            const int k = 8 * g + i;
+            bf16 v = (bf16)0.f;
+            if (k < 27) {
+                const int tap = k / 3, c = k % 3, ky = tap / 3, kx = tap % 3;
+                v = s_in[((wave * 2 + mi + ky) * kHW + r16 + kx) * 4 + c];
+            }
+            A[i] = v;
+        }
+        for (int ni = 0; ni < 2; ++ni) {
+            const bf16x8 B = *reinterpret_cast<const bf16x8*>(&s_w[(ni * 16 + r16) * 40 + g * 8]);
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+    }
+    for (int mi = 0; mi < 2; ++mi) {
+        const int y = y0 + wave * 2 + mi;
+        if (y >= H) continue;
+        for (int ni = 0; ni < 2; ++ni) {
+            const int co = ni * 16 + r16;
+            const float b = bias ? bias[co] : 0.0f;
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x0 + 4 * g + j;
+                if (xx >= W) continue;
+                float v = acc[mi][ni][j] + b;
+                if (relu) v = fmaxf(v, 0.0f);
+                out[(((int64_t)n * H + y) * W + xx) * 32 + co] = (bf16)v;
+            }
+        }
+    }
+}
+
 // Wt[ci][tap][co] = W[co][taps-1-tap][ci]: weights of the data-gradient convolution.
 __global__ __launch_bounds__(kBlock) void k_flip_weights(const bf16* w, int Cout, int Cin, int taps, bf16* wt) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -750,6 +816,15 @@ lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, 
     a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu;
     if (!src0 || !w || !out) return LGI_ERR_INVALID_ARG;
     return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_conv3x3_first_fwd(const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,
+                                      int32_t h, int32_t wd, int32_t relu, void* out, void* stream) {
+    if (!x || !w || !out || cin_w < 3 || n <= 0 || h <= 0 || wd <= 0) return LGI_ERR_INVALID_ARG;
+    const int tiles = cdiv(wd, kTW) * cdiv(h, kTH) * n;
+    hipLaunchKernelGGL(k_conv_first, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w, cin_w,
+                       bias, n, h, wd, relu, (bf16*)out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout, int32_t cin,

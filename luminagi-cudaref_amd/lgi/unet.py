@@ -86,6 +86,17 @@ def conv3x3_fwd(src0, src1, w_bf16, bias, relu):
     return out
 
 
+def conv3x3_first_fwd(x, w_bf16, bias, relu=True):
+    """First layer straight from the FP32 3-channel input x [N,H,W,3] -> BF16 [N,H,W,32]."""
+    n, h, w, c = x.shape
+    _check(x, torch.float32, 3)
+    assert w_bf16.shape[0] == 32 and w_bf16.dtype == torch.bfloat16 and w_bf16.is_contiguous()
+    out = torch.empty((n, h, w, 32), dtype=torch.bfloat16, device=x.device)
+    _call("lgi_unet_conv3x3_first_fwd", _p(x), _p(w_bf16), w_bf16.shape[2], _p(bias), n, h, w, int(relu), _p(out),
+          _stream())
+    return out
+
+
 def conv3x3_bwd(dy, y_relu, w_bf16, src0, src1, need_dx=True):
     """Returns (dx0, dx1, dw fp32 [cout,9,cin], db fp32)."""
     n, h, w, cout = dy.shape
@@ -245,8 +256,21 @@ class UNet(torch.nn.Module):
         self.tail_w = torch.nn.Parameter(torch.randn(3, 32, generator=g) * (1.0 / 32) ** 0.5 * 0.1)
         self.tail_b = torch.nn.Parameter(torch.zeros(3))
         self.recompute = recompute
+        self._wcache = {}  # name -> ((version, data_ptr), BF16 copy) for inference
+
+    def _wb(self, name):
+        """BF16 copy of a 3x3 weight, rebuilt only when the FP32 weight has changed."""
+        w = self.weights[name]
+        key = (w._version, w.data_ptr())
+        hit = self._wcache.get(name)
+        if hit is None or hit[0] != key:
+            hit = (key, w.detach().to(torch.bfloat16).contiguous())
+            self._wcache[name] = hit
+        return hit[1]
 
     def conv(self, name, a, b=None, relu=True):
+        if not torch.is_grad_enabled():
+            return conv3x3_fwd(a, b, self._wb(name), self.biases[name], relu)
         return Conv3x3Fn.apply(a, b, self.weights[name], self.biases[name], relu)
 
     def block(self, prefix, a, b=None):
@@ -261,10 +285,14 @@ class UNet(torch.nn.Module):
 
     def forward(self, x, taps=None):
         """taps (optional dict) receives every level's activation for per-layer error reports."""
-        h = pack_input(x.contiguous())
+        x = x.contiguous()
         skips = []
         for i in range(len(ENC)):
-            h = self._block(f"enc{i}", h)
+            if i == 0 and not torch.is_grad_enabled():
+                # inference: the first conv reads the 3-channel input directly
+                h = self.conv("enc0b", conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"]))
+            else:
+                h = self._block(f"enc{i}", pack_input(x) if i == 0 else h)
             skips.append(h)
             if taps is not None:
                 taps[f"enc{i}"] = h
@@ -276,7 +304,7 @@ class UNet(torch.nn.Module):
             h = self._block(f"dec{i}", UpsampleFn.apply(h), skips[i])
             if taps is not None:
                 taps[f"dec{i}"] = h
-        return TailFn.apply(h, x.contiguous(), self.tail_w, self.tail_b)
+        return TailFn.apply(h, x, self.tail_w, self.tail_b)
 
     def state_for_checkpoint(self):
         return {k: v.detach().cpu() for k, v in self.state_dict().items()}

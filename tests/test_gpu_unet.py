@@ -290,3 +290,24 @@ def test_conv_tuning_table_records_each_shape_once(gpu):
     shape, chosen, times = rows[0]
     assert set(times) == {0, 1, 2, 3, 4} and chosen == min(times, key=times.get)
     assert torch.equal(y1, y2)
+
+
+def test_first_layer_kernel_matches_packed_path_and_fp64(gpu):
+    """The inference first layer (FP32 3-channel input read directly, one K=27 MFMA row per
+    pixel) against the packed-input 3x3 conv and the FP64 reference, ragged tile edges."""
+    from lgi import unet
+    g = torch.Generator().manual_seed(21)
+    x = torch.log1p(torch.rand(2, 20, 37, 3, generator=g) * 3.0)
+    w = torch.zeros(32, 9, 32)
+    w[:, :, :3] = torch.randn(32, 9, 3, generator=g) * (2.0 / 27) ** 0.5
+    wb = w.to(torch.bfloat16)
+    b = torch.randn(32, generator=g) * 0.1
+    y1 = unet.conv3x3_first_fwd(x.cuda(), wb.cuda(), b.cuda(), True)
+    y0 = unet.conv3x3_fwd(unet.pack_input(x.cuda()), None, wb.cuda(), b.cuda(), True)
+    xb = torch.zeros(2, 20, 37, 32, dtype=torch.float64)
+    xb[..., :3] = x.to(torch.bfloat16).double()
+    ref = ref_conv_nhwc(xb, None, wb, b, True)
+    e1, e0 = rel(y1, ref), rel(y0, ref)
+    d = (y1.float() - y0.float()).abs().max().item() / y0.float().abs().max().item()
+    print(f"first layer: rel err {e1:.2e} (packed path {e0:.2e}), max diff between paths {d:.2e}", flush=True)
+    assert e1 < BF16_TOL and d < 1e-2
