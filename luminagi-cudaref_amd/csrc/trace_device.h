@@ -103,28 +103,45 @@ LGI_D Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
 // the shadow ray, its tmax and the contribution albedo * (color * I) to add if unoccluded.
 // A point light at distance >= radius has attenuation exactly 0, so it is reported as not
 // contributing (its shadow ray would only ever add +0).
+// Direction to light li from the shading point (li 0 = sun, then the point lights) and, for a
+// point light, its distance (src/pathtracer.cu:236-262).
+LGI_D void light_dir(const TraceParams& P, int li, V3 Pt, V3& toLight, float& dist) {
+    if (li == 0) {
+        toLight = normalize(mul(v3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]), -1.0f));
+        dist = 0.0f;
+        return;
+    }
+    const lgi_point_light L = P.points[li - 1];
+    V3 toLightVec = sub(v3(L.pos[0], L.pos[1], L.pos[2]), Pt);
+    dist = lgi_sqrt(dot(toLightVec, toLightVec));
+    toLight = mul(toLightVec, lgi_div(1.0f, dist));
+}
+
+// Shadow ray of a NEE candidate: from Pt + N * 1e-3 toward the light, tmax 1e30 for the sun and
+// dist - 1e-3 for a point light.
+LGI_D void shadow_ray(int li, V3 Pt, V3 N, V3 toLight, float dist, Ray& shadow, float& tmax) {
+    const float kShadowBias = 1e-3f;
+    shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
+    tmax = li == 0 ? 1e30f : dist - kShadowBias;
+}
+
 LGI_D bool nee_candidate(const TraceParams& P, int li, V3 Pt, V3 N, V3 albedo, Ray& shadow, float& tmax,
                          V3& contrib) {
-    const float kShadowBias = 1e-3f;
+    V3 toLight;
+    float dist;
+    light_dir(P, li, Pt, toLight, dist);
+    float ndotl = fmaxf(0.0f, dot(N, toLight));
     if (li == 0) {
-        V3 toLight = normalize(mul(v3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]), -1.0f));
-        float ndotl = fmaxf(0.0f, dot(N, toLight));
         if (!(ndotl > 0.0f)) return false;
-        shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
-        tmax = 1e30f;
+        shadow_ray(li, Pt, N, toLight, dist, shadow, tmax);
         float I = P.sun_intensity * ndotl;
         contrib = had(albedo, mul(v3(P.sun_color[0], P.sun_color[1], P.sun_color[2]), I));
         return true;
     }
     const lgi_point_light L = P.points[li - 1];
-    V3 toLightVec = sub(v3(L.pos[0], L.pos[1], L.pos[2]), Pt);
-    float dist = lgi_sqrt(dot(toLightVec, toLightVec));
-    V3 toLight = mul(toLightVec, lgi_div(1.0f, dist));
-    float ndotl = fmaxf(0.0f, dot(N, toLight));
     if (ndotl <= 0.0f || dist >= L.radius * 3.0f) return false;
     if (dist >= L.radius) return false;  // atten == 0 exactly
-    shadow = make_ray(add(Pt, mul(N, kShadowBias)), toLight);
-    tmax = dist - kShadowBias;
+    shadow_ray(li, Pt, N, toLight, dist, shadow, tmax);
     float r = L.radius;
     float atten = fmaxf(0.0f, 1.0f - lgi_div(dist, r));
     atten = atten * atten;

@@ -43,8 +43,21 @@ struct WfParams {
     float* T;             // throughput per path (3)
     float* Tb;            // throughput before the bounce update (3)
     float* L;             // radiance per path (3)
-    float* contrib;       // per (path, light): contribution if unoccluded (3)
-    uint8_t* nee;         // per (path, light): 0 none, 1 candidate, 2 occluded
+    float* contrib;       // per (queue slot, light): contribution if unoccluded (3)
+    uint8_t* nee;         // per (queue slot, light): 0 none, 1 candidate, 2 occluded
+    float4* vP;           // per queue slot: shading point (xyz)
+    float4* vN;           // per queue slot: facing shading normal (xyz)
+};
+
+// Shadow rays are not stored: the any-hit kernel reads the id j = slot * nl + light of each
+// candidate and rebuilds the ray from the slot's shading point and normal with the same
+// expressions the shading kernel evaluated (light_dir, shadow_ray).
+struct ShadowSrc {
+    const uint32_t* ids;  // compacted candidate ids
+    const float4* vP;
+    const float4* vN;
+    int nl;
+    TraceParams P;        // light parameters
 };
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
@@ -110,7 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
 template <bool kAny, int kLds, int kTop>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
-                                                     float4* hits, uint8_t* nee, uint32_t* counter, int* spill) {
+                                                     float4* hits, uint8_t* nee, uint32_t* counter, int* spill,
+                                                     ShadowSrc shadow) {
     __shared__ int s_stack[kLds * kBlock];
     __shared__ TNode s_top[kTop > 0 ? kTop : 1];
     if (kTop > 0) {
@@ -154,7 +168,28 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 uint32_t idx = 0xFFFFFFFFu;
                 if (rank < avail) idx = cbase + rank;
                 else if (nb != 0xFFFFFFFFu && rank - avail < nend - nb) idx = nb + (rank - avail);
-                if (idx != 0xFFFFFFFFu) {
+                if (idx != 0xFFFFFFFFu && kAny) {
+                    const uint32_t j = shadow.ids[idx];
+                    const int si = (int)(j / (uint32_t)shadow.nl), li = (int)(j % (uint32_t)shadow.nl);
+                    const float4 p4 = shadow.vP[si], n4 = shadow.vN[si];
+                    const V3 Pt = v3(p4.x, p4.y, p4.z), N = v3(n4.x, n4.y, n4.z);
+                    V3 toLight;
+                    float dist, tmax;
+                    light_dir(shadow.P, li, Pt, toLight, dist);
+                    Ray sr;
+                    shadow_ray(li, Pt, N, toLight, dist, sr, tmax);
+                    qi = (int)idx;
+                    slot = (int)j;
+                    r.o = sr.o;
+                    r.d = sr.d;
+                    r.inv = v3(lgi_div(1.0f, r.d.x), lgi_div(1.0f, r.d.y), lgi_div(1.0f, r.d.z));
+                    best_t = tmax;
+                    best_tri = -1;
+                    node = 0;
+                    leaf = 0;
+                    sp = 0;
+                    active = true;
+                } else if (idx != 0xFFFFFFFFu) {
                     const QRay e = q[idx];
                     qi = (int)idx;
                     slot = __float_as_int(e.o.w);
@@ -256,8 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
 // For each traced path: NEE candidates -> shadow queue, Russian roulette + cosine bounce ->
 // next queue with its sort key (src/pathtracer.cu:295-331, same order of RNG draws).
 __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n, int bounce,
-                                                     QRay* sq, uint32_t* sflag, QRay* nq, uint32_t* nkeys,
-                                                     uint32_t* nflag) {
+                                                     uint32_t* sflag, QRay* nq, uint32_t* nkeys, uint32_t* nflag) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
@@ -285,25 +319,23 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
     }
     if (i >= n) return;
-    // NEE candidates into fixed slots i*nl + li (compacted afterwards by a scan)
+    // NEE candidates: flags and contributions in fixed slots i*nl + li (the candidates are
+    // compacted afterwards by a scan); the shading point and normal for the shadow rays
+    bool any = false;
     for (int li = 0; li < W.nl; ++li) {
         Ray shadow;
         float tmax = 0.f;
         V3 c = v3(0.f, 0.f, 0.f);
         const bool cand = live && nee_candidate(W.P, li, Pt, N, albedo, shadow, tmax, c);
-        if (live) {
-            const int64_t k = (int64_t)p * W.nl + li;
-            W.nee[k] = cand ? 1 : 0;
-            if (cand) { W.contrib[3 * k] = c.x; W.contrib[3 * k + 1] = c.y; W.contrib[3 * k + 2] = c.z; }
-        }
         const int64_t j = (int64_t)i * W.nl + li;
-        QRay sr;
-        sr.o = make_float4(shadow.o.x, shadow.o.y, shadow.o.z,
-                           __int_as_float(cand ? (int)((int64_t)p * W.nl + li) : -1));
-        sr.d = make_float4(shadow.d.x, shadow.d.y, shadow.d.z, tmax);
-        if (cand) sq[j] = sr;
-        else sq[j].o.w = __int_as_float(-1);
+        W.nee[j] = cand ? 1 : 0;
+        if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
         sflag[j] = cand ? 1u : 0u;
+        any |= cand;
+    }
+    if (any) {
+        W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, 0.f);
+        W.vN[i] = make_float4(N.x, N.y, N.z, 0.f);
     }
     // continuation
     bool cont = live && bounce < W.P.bounces;
@@ -357,6 +389,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const ui
     }
 }
 
+// Candidate ids to their scanned positions (order-preserving).
+__global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, const uint32_t* off, int64_t n,
+                                                           uint32_t* ids) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n || nee[j] != 1) return;
+    ids[off[j]] = (uint32_t)j;
+}
+
 // L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313).
 __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q, const float4* hits, int n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -365,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q
     if (p < 0 || __float_as_int(hits[i].w) < 0) return;
     V3 color = v3(0.f, 0.f, 0.f);
     for (int li = 0; li < W.nl; ++li) {
-        const int64_t k = (int64_t)p * W.nl + li;
+        const int64_t k = (int64_t)i * W.nl + li;
         if (W.nee[k] == 1) color = add(color, v3(W.contrib[3 * k], W.contrib[3 * k + 1], W.contrib[3 * k + 2]));
     }
     float* L = W.L + 3 * (int64_t)p;
@@ -411,9 +451,9 @@ static_assert(kTraceLdsStack >= kStackLdsMin, "spill area sized for kStackMax - 
 
 template <bool kAny>
 void launch_trace(int grid, hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
-                  float4* hits, uint8_t* nee, uint32_t* counter, int* spill) {
+                  float4* hits, uint8_t* nee, uint32_t* counter, int* spill, const ShadowSrc& shadow) {
     hipLaunchKernelGGL((k_wf_trace<kAny, kTraceLdsStack, kTraceTopNodes>), dim3(grid), dim3(kBlock), 0, st, tn, n_tn,
-                       tv, q, n, hits, nee, counter, spill);
+                       tv, q, n, hits, nee, counter, spill, shadow);
 }
 
 int trace_grid(int n) {
@@ -440,8 +480,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     QRay* qa = (QRay*)S_(50, cap * sizeof(QRay));
     QRay* qb = (QRay*)S_(51, cap * sizeof(QRay));
     float4* hits = (float4*)S_(52, cap * sizeof(float4));
-    QRay* sq_tmp = (QRay*)S_(53, cap * nl * sizeof(QRay));
-    QRay* sq = (QRay*)S_(64, cap * nl * sizeof(QRay));
+    uint32_t* sids = (uint32_t*)S_(53, cap * nl * 4);
     uint32_t* sflag = (uint32_t*)S_(65, cap * nl * 4);
     uint32_t* nflag = (uint32_t*)S_(66, cap * 4);
     uint32_t* keys_tmp = (uint32_t*)S_(67, cap * 4);
@@ -464,11 +503,14 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     W.T = (float*)S_(59, paths * 12);
     W.Tb = (float*)S_(60, paths * 12);
     W.L = (float*)S_(61, paths * 12);
-    W.contrib = (float*)S_(62, paths * nl * 12);
-    W.nee = (uint8_t*)S_(63, paths * nl);
-    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sq_tmp, (void*)sq, (void*)sflag, (void*)nflag,
-                    (void*)keys_tmp, (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters,
-                    (void*)W.rng, (void*)W.T, (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee})
+    W.contrib = (float*)S_(62, cap * nl * 12);
+    W.nee = (uint8_t*)S_(63, cap * nl);
+    W.vP = (float4*)S_(64, cap * 16);
+    W.vN = (float4*)S_(70, cap * 16);
+    ShadowSrc shadow{sids, W.vP, W.vN, nl, P};
+    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)sflag, (void*)nflag, (void*)keys_tmp,
+                    (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.rng, (void*)W.T,
+                    (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
     LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
@@ -486,10 +528,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             LGI_HIP(ctx, hipMemsetAsync(counters, 0, 64, st));
             if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
             launch_trace<false>(trace_grid(n), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n, hits,
-                                (uint8_t*)nullptr, counters + 0, spill);
+                                (uint8_t*)nullptr, counters + 0, spill, shadow);
             if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
-            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sq_tmp,
-                               sflag, other, keys_tmp, nflag);
+            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sflag,
+                               other, keys_tmp, nflag);
             // order-preserving compaction of shadow rays and continuations (no global atomics)
             const int64_t nslots = (int64_t)n * nl;
             lgi_exclusive_scan(st, sflag, (int)nslots, counters + 1, scan_scratch);
@@ -499,11 +541,11 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             LGI_HIP(ctx, hipStreamSynchronize(st));
             const int ns = (int)hc[1], nn = (int)hc[2];
             if (ns > 0) {
-                hipLaunchKernelGGL(k_wf_compact, dim3(cdiv(nslots, kBlock)), dim3(kBlock), 0, st, sq_tmp,
-                                   (const uint32_t*)nullptr, sflag, nslots, sq, (uint32_t*)nullptr, (int32_t*)nullptr);
+                hipLaunchKernelGGL(k_wf_compact_ids, dim3(cdiv(nslots, kBlock)), dim3(kBlock), 0, st,
+                                   (const uint8_t*)W.nee, (const uint32_t*)sflag, nslots, sids);
                 if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
-                launch_trace<true>(trace_grid(ns), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, sq, ns,
-                                   (float4*)nullptr, W.nee, counters + 3, spill);
+                launch_trace<true>(trace_grid(ns), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr, ns,
+                                   (float4*)nullptr, W.nee, counters + 3, spill, shadow);
                 if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
             }
             hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
