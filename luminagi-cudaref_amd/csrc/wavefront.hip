@@ -290,8 +290,32 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
 // ------------------------------------------------------------------ shading
 // For each traced path: NEE candidates -> shadow queue, Russian roulette + cosine bounce ->
 // next queue with its sort key (src/pathtracer.cu:295-331, same order of RNG draws).
+// Block-wide exclusive scan of one value per thread (256 threads; every thread must call).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; ++w) pre += s_w[w];
+    return pre + x - v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Shading also counts, per 256-slot block, the NEE candidates and continuations it produced
+// (blk_s, blk_n); a scan of those counts places each block's output, and the compaction
+// kernels rank entries within their block, so no scan runs over the per-slot flags.
 __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n, int bounce,
-                                                     uint32_t* sflag, QRay* nq, uint32_t* nkeys, uint32_t* nflag) {
+                                                     QRay* nq, uint32_t* nkeys, uint8_t* ncont, uint32_t* blk_s,
+                                                     uint32_t* blk_n) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
@@ -318,83 +342,104 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         float* Tb = W.Tb + 3 * (int64_t)p;
         Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
     }
-    if (i >= n) return;
-    // NEE candidates: flags and contributions in fixed slots i*nl + li (the candidates are
-    // compacted afterwards by a scan); the shading point and normal for the shadow rays
-    bool any = false;
-    for (int li = 0; li < W.nl; ++li) {
-        Ray shadow;
-        float tmax = 0.f;
-        V3 c = v3(0.f, 0.f, 0.f);
-        const bool cand = live && nee_candidate(W.P, li, Pt, N, albedo, shadow, tmax, c);
-        const int64_t j = (int64_t)i * W.nl + li;
-        W.nee[j] = cand ? 1 : 0;
-        if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
-        sflag[j] = cand ? 1u : 0u;
-        any |= cand;
-    }
-    if (any) {
-        W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, 0.f);
-        W.vN[i] = make_float4(N.x, N.y, N.z, 0.f);
-    }
-    // continuation
-    bool cont = live && bounce < W.P.bounces;
-    Ray next;
-    if (cont) {
-        float* T = W.T + 3 * (int64_t)p;
-        V3 Tv = v3(T[0], T[1], T[2]);
-        Rng rng;
-        rng.pcg = true;
-        const uint32_t pix = (uint32_t)(p % W.npx);
-        const uint64_t stream = ((uint64_t)W.P.seed << 32) | pix;
-        rng.g.inc = (stream << 1u) | 1u;
-        rng.g.state = W.rng[p];
-        if (bounce >= 1 && (W.P.flags & LGI_RENDER_NO_RR) == 0) {
-            float qv = fminf(0.95f, fmaxf(Tv.x, fmaxf(Tv.y, Tv.z)));
-            if (rng.next() > qv) cont = false;
-            else Tv = mul(Tv, lgi_div(1.0f, qv));
+    // NEE candidates: flags and contributions in fixed slots i*nl + li (compacted afterwards);
+    // the shading point and normal for the shadow rays
+    uint32_t ncand = 0;
+    bool cont = false;
+    if (i < n) {
+        for (int li = 0; li < W.nl; ++li) {
+            Ray shadow;
+            float tmax = 0.f;
+            V3 c = v3(0.f, 0.f, 0.f);
+            const bool cand = live && nee_candidate(W.P, li, Pt, N, albedo, shadow, tmax, c);
+            const int64_t j = (int64_t)i * W.nl + li;
+            W.nee[j] = cand ? 1 : 0;
+            if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
+            ncand += cand ? 1u : 0u;
+        }
+        if (ncand) {
+            W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, 0.f);
+            W.vN[i] = make_float4(N.x, N.y, N.z, 0.f);
+        }
+        // continuation
+        cont = live && bounce < W.P.bounces;
+        Ray next;
+        if (cont) {
+            float* T = W.T + 3 * (int64_t)p;
+            V3 Tv = v3(T[0], T[1], T[2]);
+            Rng rng;
+            rng.pcg = true;
+            const uint32_t pix = (uint32_t)(p % W.npx);
+            const uint64_t stream = ((uint64_t)W.P.seed << 32) | pix;
+            rng.g.inc = (stream << 1u) | 1u;
+            rng.g.state = W.rng[p];
+            if (bounce >= 1 && (W.P.flags & LGI_RENDER_NO_RR) == 0) {
+                float qv = fminf(0.95f, fmaxf(Tv.x, fmaxf(Tv.y, Tv.z)));
+                if (rng.next() > qv) cont = false;
+                else Tv = mul(Tv, lgi_div(1.0f, qv));
+            }
+            if (cont) {
+                V3 nd = sample_cos_hemi(N, rng);
+                next = make_ray(add(Pt, mul(N, 1e-3f)), nd);
+                Tv = had(Tv, albedo);
+                T[0] = Tv.x; T[1] = Tv.y; T[2] = Tv.z;
+            }
+            W.rng[p] = rng.g.state;
         }
         if (cont) {
-            V3 nd = sample_cos_hemi(N, rng);
-            next = make_ray(add(Pt, mul(N, 1e-3f)), nd);
-            Tv = had(Tv, albedo);
-            T[0] = Tv.x; T[1] = Tv.y; T[2] = Tv.z;
+            QRay nr;
+            nr.o = make_float4(next.o.x, next.o.y, next.o.z, __int_as_float(p));
+            nr.d = make_float4(next.d.x, next.d.y, next.d.z, 1e30f);
+            nq[i] = nr;
+            nkeys[i] = ray_key(W, next.o, next.d);
         }
-        W.rng[p] = rng.g.state;
+        ncont[i] = cont ? 1 : 0;
     }
-    if (cont) {
-        QRay nr;
-        nr.o = make_float4(next.o.x, next.o.y, next.o.z, __int_as_float(p));
-        nr.d = make_float4(next.d.x, next.d.y, next.d.z, 1e30f);
-        nq[i] = nr;
-        nkeys[i] = ray_key(W, next.o, next.d);
-    } else {
-        nq[i].o.w = __int_as_float(-1);
-    }
-    nflag[i] = cont ? 1u : 0u;
-}
-
-// Scatter flagged entries to their scanned positions (order-preserving compaction).
-__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint32_t* src_key, const uint32_t* off,
-                                                       int64_t n, QRay* dst, uint32_t* dst_key, int32_t* dst_idx) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const QRay e = src[i];
-    if (__float_as_int(e.o.w) < 0) return;
-    const uint32_t o = off[i];
-    dst[o] = e;
-    if (dst_key) {
-        dst_key[o] = src_key[i];
-        dst_idx[o] = (int32_t)o;
+    // per-block totals
+    __shared__ uint32_t s_c[2][kBlock / 64];
+    const uint32_t ws = wave_sum(ncand), wn = wave_sum(cont ? 1u : 0u);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s_c[0][wave] = ws; s_c[1][wave] = wn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t a0 = 0, a1 = 0;
+        for (int w = 0; w < kBlock / 64; ++w) { a0 += s_c[0][w]; a1 += s_c[1][w]; }
+        blk_s[blockIdx.x] = a0;
+        blk_n[blockIdx.x] = a1;
     }
 }
 
-// Candidate ids to their scanned positions (order-preserving).
-__global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, const uint32_t* off, int64_t n,
+// Continuations of block b (flag ncont) to blk_off[b] + their rank in the block, with their
+// sort keys and identity indices (order-preserving).
+__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint32_t* src_key, const uint8_t* flag,
+                                                       int n, const uint32_t* blk_off, QRay* dst, uint32_t* dst_key,
+                                                       int32_t* dst_idx) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
+    const uint32_t r = block_excl_scan(f, s_w);
+    if (!f) return;
+    const uint32_t o = blk_off[blockIdx.x] + r;
+    dst[o] = src[i];
+    dst_key[o] = src_key[i];
+    dst_idx[o] = (int32_t)o;
+}
+
+// NEE candidate ids j = slot * nl + light of block b to blk_off[b] + their rank in the block
+// (slot-major, lights in order).
+__global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, int nl, int n, const uint32_t* blk_off,
                                                            uint32_t* ids) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n || nee[j] != 1) return;
-    ids[off[j]] = (uint32_t)j;
+    __shared__ uint32_t s_w[kBlock / 64];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t c = 0;
+    if (i < n)
+        for (int li = 0; li < nl; ++li) c += nee[(int64_t)i * nl + li] == 1 ? 1u : 0u;
+    uint32_t o = blk_off[blockIdx.x] + block_excl_scan(c, s_w);
+    if (!c) return;
+    for (int li = 0; li < nl; ++li) {
+        const int64_t j = (int64_t)i * nl + li;
+        if (nee[j] == 1) ids[o++] = (uint32_t)j;
+    }
 }
 
 // L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313).
@@ -481,13 +526,15 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     QRay* qb = (QRay*)S_(51, cap * sizeof(QRay));
     float4* hits = (float4*)S_(52, cap * sizeof(float4));
     uint32_t* sids = (uint32_t*)S_(53, cap * nl * 4);
-    uint32_t* sflag = (uint32_t*)S_(65, cap * nl * 4);
-    uint32_t* nflag = (uint32_t*)S_(66, cap * 4);
+    const int64_t nblk = cdiv(cap, kBlock);
+    uint32_t* blk_s = (uint32_t*)S_(65, nblk * 4 + 64);
+    uint32_t* blk_n = (uint32_t*)S_(66, nblk * 4 + 64);
+    uint8_t* ncont = (uint8_t*)S_(71, cap + 64);
     uint32_t* keys_tmp = (uint32_t*)S_(67, cap * 4);
     uint32_t* keys = (uint32_t*)S_(54, cap * 4);
     int32_t* idx = (int32_t*)S_(55, cap * 4);
     void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
-    void* scan_scratch = S_(68, lgi_scan_scratch_bytes(cap * nl));
+    void* scan_scratch = S_(68, lgi_scan_scratch_bytes(nblk));
     uint32_t* counters = (uint32_t*)S_(57, 64);
     int* spill = (int*)S_(69, (size_t)kStackSpill * kTraceGridMax * kBlock * 4);
     if (!spill) return lgi_set_err(ctx, LGI_ERR_HIP, "stack spill allocation failed");
@@ -508,7 +555,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     W.vP = (float4*)S_(64, cap * 16);
     W.vN = (float4*)S_(70, cap * 16);
     ShadowSrc shadow{sids, W.vP, W.vN, nl, P};
-    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)sflag, (void*)nflag, (void*)keys_tmp,
+    for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)blk_s, (void*)blk_n, (void*)ncont,
+                    (void*)keys_tmp,
                     (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.rng, (void*)W.T,
                     (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
@@ -530,19 +578,19 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             launch_trace<false>(trace_grid(n), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n, hits,
                                 (uint8_t*)nullptr, counters + 0, spill, shadow);
             if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
-            hipLaunchKernelGGL(k_wf_shade, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n, b, sflag,
-                               other, keys_tmp, nflag);
-            // order-preserving compaction of shadow rays and continuations (no global atomics)
-            const int64_t nslots = (int64_t)n * nl;
-            lgi_exclusive_scan(st, sflag, (int)nslots, counters + 1, scan_scratch);
-            lgi_exclusive_scan(st, nflag, n, counters + 2, scan_scratch);
+            const int nb = cdiv(n, kBlock);
+            hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n, b, other, keys_tmp, ncont,
+                               blk_s, blk_n);
+            // block offsets of the shadow-ray candidates and of the continuations
+            lgi_exclusive_scan(st, blk_s, nb, counters + 1, scan_scratch);
+            lgi_exclusive_scan(st, blk_n, nb, counters + 2, scan_scratch);
             uint32_t hc[4];
             LGI_HIP(ctx, hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
             LGI_HIP(ctx, hipStreamSynchronize(st));
             const int ns = (int)hc[1], nn = (int)hc[2];
             if (ns > 0) {
-                hipLaunchKernelGGL(k_wf_compact_ids, dim3(cdiv(nslots, kBlock)), dim3(kBlock), 0, st,
-                                   (const uint8_t*)W.nee, (const uint32_t*)sflag, nslots, sids);
+                hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n,
+                                   (const uint32_t*)blk_s, sids);
                 if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
                 launch_trace<true>(trace_grid(ns), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr, ns,
                                    (float4*)nullptr, W.nee, counters + 3, spill, shadow);
@@ -551,8 +599,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
             if (b == P.bounces || nn == 0) break;
             // compact the continuations, sort them by origin/direction key, gather in key order
-            hipLaunchKernelGGL(k_wf_compact, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, other, keys_tmp, nflag,
-                               (int64_t)n, cur, keys, idx);
+            hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont, n,
+                               (const uint32_t*)blk_n, cur, keys, idx);
             lgi_sort_pairs(st, keys, idx, nn, 32, sort_scratch);
             hipLaunchKernelGGL(k_wf_gather, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, cur, idx, other, nn);
             std::swap(cur, other);
