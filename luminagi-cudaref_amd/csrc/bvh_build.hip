@@ -109,9 +109,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* data, int n, cons
 }
 
 // ------------------------------------------------------------------ radix sort
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int n, int shift,
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int n, const uint32_t* n_dev, int shift,
                                                        uint32_t* counts, int nb) {
     __shared__ uint32_t h[256];
+    if (n_dev) n = min(n, (int)*n_dev);
     h[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * kTile;
@@ -126,9 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int
 // Stable scatter: within a tile, element order is (round j, thread); ranks among equal
 // digits come from a 64-lane match built from 8 ballots plus per-wave digit counts.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int32_t* vin,
-                                                          uint32_t* kout, int32_t* vout, int n, int shift,
-                                                          const uint32_t* offs, int nb) {
+                                                          uint32_t* kout, int32_t* vout, int n, const uint32_t* n_dev,
+                                                          int shift, const uint32_t* offs, int nb) {
     __shared__ uint32_t s_base[256];
+    if (n_dev) n = min(n, (int)*n_dev);
     __shared__ uint32_t s_wc[kBlock / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     s_base[tid] = offs[(size_t)tid * nb + blockIdx.x];
@@ -462,19 +464,20 @@ struct SortBufs {
 
 // Stable LSD radix sort of (keys, vals) over `bits` low bits (a multiple of 16, so the
 // even number of ping-pong passes leaves the result in buffer 0).
-void radix_sort(hipStream_t st, SortBufs& b, int n, int bits) {
+// n_dev (nullable): the key count on the device, at most n; the grids are sized for n.
+void radix_sort(hipStream_t st, SortBufs& b, int n, int bits, const uint32_t* n_dev = nullptr) {
     if (n <= 1) return;
     const int nb = ceil_div(n, kTile);
     const int ncount = 256 * nb;
     const int nsb = ceil_div(ncount, kTile);
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, st, b.k[cur], n, shift, b.counts, nb);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, st, b.k[cur], n, n_dev, shift, b.counts, nb);
         hipLaunchKernelGGL(k_scan_tiles, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, b.sums, nsb, (uint32_t*)nullptr);
         hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, st, b.k[cur], b.v[cur], b.k[cur ^ 1],
-                           b.v[cur ^ 1], n, shift, b.counts, nb);
+                           b.v[cur ^ 1], n, n_dev, shift, b.counts, nb);
         cur ^= 1;
     }
 }
@@ -495,7 +498,8 @@ size_t lgi_sort_scratch_bytes(int64_t n) {
     return 4 * (4 * (size_t)n + ncount + ceil_div(ncount, kTile) + 64);
 }
 
-void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch) {
+void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
+                    const uint32_t* n_dev) {
     SortBufs b;
     char* p = (char*)scratch;
     const int nb = ceil_div(n, kTile);
@@ -506,7 +510,7 @@ void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bi
     b.v[1] = (int32_t*)p; p += 4 * (size_t)n;
     b.counts = (uint32_t*)p; p += 4 * ncount;
     b.sums = (uint32_t*)p;
-    radix_sort(st, b, n, bits);
+    radix_sort(st, b, n, bits, n_dev);
 }
 
 size_t lgi_scan_scratch_bytes(int64_t n) { return 4 * ((size_t)ceil_div(n, kTile) + 64); }

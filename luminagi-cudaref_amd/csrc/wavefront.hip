@@ -124,7 +124,8 @@ template <bool kAny, int kLds, int kTop>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
                                                      float4* hits, uint8_t* nee, uint32_t* counter, int* spill,
-                                                     ShadowSrc shadow) {
+                                                     ShadowSrc shadow, const uint32_t* n_dev) {
+    if (n_dev) n = min(n, (int)*n_dev);
     __shared__ int s_stack[kLds * kBlock];
     __shared__ TNode s_top[kTop > 0 ? kTop : 1];
     if (kTop > 0) {
@@ -313,9 +314,10 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 // Shading also counts, per 256-slot block, the NEE candidates and continuations it produced
 // (blk_s, blk_n); a scan of those counts places each block's output, and the compaction
 // kernels rank entries within their block, so no scan runs over the per-slot flags.
-__global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n, int bounce,
-                                                     QRay* nq, uint32_t* nkeys, uint8_t* ncont, uint32_t* blk_s,
-                                                     uint32_t* blk_n) {
+__global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n,
+                                                     const uint32_t* n_dev, int bounce, QRay* nq, uint32_t* nkeys,
+                                                     uint8_t* ncont, uint32_t* blk_s, uint32_t* blk_n) {
+    if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
@@ -412,9 +414,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
 // Continuations of block b (flag ncont) to blk_off[b] + their rank in the block, with their
 // sort keys and identity indices (order-preserving).
 __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint32_t* src_key, const uint8_t* flag,
-                                                       int n, const uint32_t* blk_off, QRay* dst, uint32_t* dst_key,
-                                                       int32_t* dst_idx) {
+                                                       int n, const uint32_t* n_dev, const uint32_t* blk_off, QRay* dst,
+                                                       uint32_t* dst_key, int32_t* dst_idx) {
     __shared__ uint32_t s_w[kBlock / 64];
+    if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
     const uint32_t r = block_excl_scan(f, s_w);
@@ -427,9 +430,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const ui
 
 // NEE candidate ids j = slot * nl + light of block b to blk_off[b] + their rank in the block
 // (slot-major, lights in order).
-__global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, int nl, int n, const uint32_t* blk_off,
-                                                           uint32_t* ids) {
+__global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, int nl, int n, const uint32_t* n_dev,
+                                                           const uint32_t* blk_off, uint32_t* ids) {
     __shared__ uint32_t s_w[kBlock / 64];
+    if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t c = 0;
     if (i < n)
@@ -443,7 +447,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, i
 }
 
 // L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313).
-__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q, const float4* hits, int n) {
+__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q, const float4* hits, int n,
+                                                       const uint32_t* n_dev) {
+    if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int p = __float_as_int(q[i].o.w);
@@ -459,9 +465,16 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q
     L[0] = Lv.x; L[1] = Lv.y; L[2] = Lv.z;
 }
 
-__global__ __launch_bounds__(kBlock) void k_wf_gather(const QRay* src, const int32_t* idx, QRay* dst, int n) {
+__global__ __launch_bounds__(kBlock) void k_wf_gather(const QRay* src, const int32_t* idx, QRay* dst, int n,
+                                                      const uint32_t* n_dev) {
+    if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_wf_fill(uint32_t* p, int n, uint32_t v) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Kahan-add the S samples of every pixel in sample order (same as the inline kernel).
@@ -496,9 +509,10 @@ static_assert(kTraceLdsStack >= kStackLdsMin, "spill area sized for kStackMax - 
 
 template <bool kAny>
 void launch_trace(int grid, hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
-                  float4* hits, uint8_t* nee, uint32_t* counter, int* spill, const ShadowSrc& shadow) {
+                  float4* hits, uint8_t* nee, uint32_t* counter, int* spill, const ShadowSrc& shadow,
+                  const uint32_t* n_dev) {
     hipLaunchKernelGGL((k_wf_trace<kAny, kTraceLdsStack, kTraceTopNodes>), dim3(grid), dim3(kBlock), 0, st, tn, n_tn,
-                       tv, q, n, hits, nee, counter, spill, shadow);
+                       tv, q, n, hits, nee, counter, spill, shadow, n_dev);
 }
 
 int trace_grid(int n) {
@@ -535,7 +549,9 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     int32_t* idx = (int32_t*)S_(55, cap * 4);
     void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
     void* scan_scratch = S_(68, lgi_scan_scratch_bytes(nblk));
-    uint32_t* counters = (uint32_t*)S_(57, 64);
+    constexpr int kCounters = 8;  // per bounce
+    if (P.bounces > 64) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "too many bounces");
+    uint32_t* counters = (uint32_t*)S_(57, (size_t)kCounters * 4 * 65);
     int* spill = (int*)S_(69, (size_t)kStackSpill * kTraceGridMax * kBlock * 4);
     if (!spill) return lgi_set_err(ctx, LGI_ERR_HIP, "stack spill allocation failed");
     WfParams W;
@@ -561,54 +577,58 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                     (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
-    LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
+    // No host round trip inside a frame: queue lengths stay on the device (each kernel reads its
+    // count and the launches are sized for the capacity), so a frame can be captured in a graph.
+    // Timing events are skipped while the stream is being captured.
+    hipStreamCaptureStatus cap_status = hipStreamCaptureStatusNone;
+    LGI_HIP(ctx, hipStreamIsCapturing(st, &cap_status));
+    const bool timed = cap_status == hipStreamCaptureStatusNone;
+    if (timed) LGI_HIP(ctx, hipEventRecord(ctx->ev0, st));
     ctx->kev_used = 0;
     ctx->kev_overflow = false;
     lgi_status ms;
+    auto mark = [&](bool begin) { return timed ? lgi_mark_trace(ctx, begin) : LGI_OK; };
     for (int s0 = 0; s0 < P.spp; s0 += S_max) {
         const int S = std::min(S_max, P.spp - s0);
         W.P.sample0 = P.sample0 + s0;
-        int n = S * padded;
-        hipLaunchKernelGGL(k_wf_primary, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, S, tiles_x, padded, qa);
+        const int n_max = S * padded;
+        const int nb = cdiv(n_max, kBlock);
+        hipLaunchKernelGGL(k_wf_fill, dim3(cdiv(kCounters * (P.bounces + 1), kBlock)), dim3(kBlock), 0, st, counters,
+                           kCounters * (P.bounces + 1), 0u);
+        hipLaunchKernelGGL(k_wf_primary, dim3(cdiv(n_max, kBlock)), dim3(kBlock), 0, st, W, S, tiles_x, padded, qa);
         QRay* cur = qa;
         QRay* other = qb;
-        for (int b = 0; b <= P.bounces && n > 0; ++b) {
-            LGI_HIP(ctx, hipMemsetAsync(counters, 0, 64, st));
-            if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
-            launch_trace<false>(trace_grid(n), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n, hits,
-                                (uint8_t*)nullptr, counters + 0, spill, shadow);
-            if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
-            const int nb = cdiv(n, kBlock);
-            hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n, b, other, keys_tmp, ncont,
-                               blk_s, blk_n);
+        const uint32_t* n_dev = nullptr;  // bounce 0: all n_max entries
+        for (int b = 0; b <= P.bounces; ++b) {
+            uint32_t* cnt = counters + kCounters * b;  // [0] chunk cursor, [1] shadow rays, [2] continuations, [3] cursor
+            if ((ms = mark(true)) != LGI_OK) return ms;
+            launch_trace<false>(trace_grid(n_max), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, hits,
+                                (uint8_t*)nullptr, cnt + 0, spill, shadow, n_dev);
+            if ((ms = mark(false)) != LGI_OK) return ms;
+            hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
+                               keys_tmp, ncont, blk_s, blk_n);
             // block offsets of the shadow-ray candidates and of the continuations
-            lgi_exclusive_scan(st, blk_s, nb, counters + 1, scan_scratch);
-            lgi_exclusive_scan(st, blk_n, nb, counters + 2, scan_scratch);
-            uint32_t hc[4];
-            LGI_HIP(ctx, hipMemcpyAsync(hc, counters, 16, hipMemcpyDeviceToHost, st));
-            LGI_HIP(ctx, hipStreamSynchronize(st));
-            const int ns = (int)hc[1], nn = (int)hc[2];
-            if (ns > 0) {
-                hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n,
-                                   (const uint32_t*)blk_s, sids);
-                if ((ms = lgi_mark_trace(ctx, true)) != LGI_OK) return ms;
-                launch_trace<true>(trace_grid(ns), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr, ns,
-                                   (float4*)nullptr, W.nee, counters + 3, spill, shadow);
-                if ((ms = lgi_mark_trace(ctx, false)) != LGI_OK) return ms;
-            }
-            hipLaunchKernelGGL(k_wf_resolve, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, W, cur, hits, n);
-            if (b == P.bounces || nn == 0) break;
+            lgi_exclusive_scan(st, blk_s, nb, cnt + 1, scan_scratch);
+            lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
+            hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n_max, n_dev,
+                               (const uint32_t*)blk_s, sids);
+            if ((ms = mark(true)) != LGI_OK) return ms;
+            launch_trace<true>(trace_grid(n_max * nl), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr,
+                               n_max * nl, (float4*)nullptr, W.nee, cnt + 3, spill, shadow, cnt + 1);
+            if ((ms = mark(false)) != LGI_OK) return ms;
+            hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev);
+            if (b == P.bounces) break;
             // compact the continuations, sort them by origin/direction key, gather in key order
-            hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont, n,
-                               (const uint32_t*)blk_n, cur, keys, idx);
-            lgi_sort_pairs(st, keys, idx, nn, 32, sort_scratch);
-            hipLaunchKernelGGL(k_wf_gather, dim3(cdiv(nn, kBlock)), dim3(kBlock), 0, st, cur, idx, other, nn);
+            hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont,
+                               n_max, n_dev, (const uint32_t*)blk_n, cur, keys, idx);
+            lgi_sort_pairs(st, keys, idx, n_max, 32, sort_scratch, cnt + 2);
+            hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, idx, other, n_max, cnt + 2);
             std::swap(cur, other);
-            n = nn;
+            n_dev = cnt + 2;
         }
         hipLaunchKernelGGL(k_wf_accum, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, st, W, S);
     }
     LGI_HIP(ctx, hipGetLastError());
-    LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));
+    if (timed) LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));
     return LGI_OK;
 }

@@ -110,6 +110,50 @@ class DenoiseGraph:
         return self.y
 
 
+class FrameGraph:
+    """One frame of the 8-spp + UNet path captured in a single HIP graph: clear the
+    accumulators, render `spp` samples with the sorted renderer (from `sample0`), mean radiance,
+    UNet, tonemap. A replay re-renders the same sample indices, so it reproduces the frame bit
+    for bit; the outputs (`hdr`, `rgb`) are static tensors overwritten by each replay.
+
+    The render needs its device buffers allocated and the UNet its convolution tiles chosen
+    before capture, so the constructor runs the frame once eagerly first. The context is moved
+    to the capture stream for the capture and back afterwards."""
+
+    def __init__(self, ctx, model, cam, spp=8, bounces=2, sample0=0, flags=2):
+        self.frame = Frame(cam)
+        h, w = cam.height, cam.width
+        args = (spp, bounces, sample0, flags)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        prev = ctx.stream
+        try:
+            with torch.cuda.stream(side):
+                ctx.set_stream(side.cuda_stream)
+                for _ in range(2):
+                    self._frame(ctx, model, h, w, *args)
+            torch.cuda.current_stream().wait_stream(side)
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+                self.hdr, self.rgb = self._frame(ctx, model, h, w, *args)
+        finally:
+            ctx.set_stream(prev)
+
+    def _frame(self, ctx, model, h, w, spp, bounces, sample0, flags):
+        f = self.frame
+        f.sum.zero_()
+        f.comp.zero_()
+        f.sumsq.zero_()
+        ctx.accumulate(f.sum, f.comp, spp, bounces, flags=flags, sample0=sample0, d_sumsq=f.sumsq, camera=f.cam)
+        hdr = denoise(model, f.sum.view(h, w, 3) / float(spp))
+        return hdr, tonemap(ctx, hdr)
+
+    def replay(self):
+        self.graph.replay()
+        return self.hdr, self.rgb
+
+
 def _tile_sum(x, th, tw):
     h, w = x.shape
     pad = torch.zeros((th * 8, tw * 8), dtype=x.dtype, device=x.device)

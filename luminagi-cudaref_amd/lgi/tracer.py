@@ -49,6 +49,7 @@ class Context:
         h = C.c_void_p()
         _check(None, lib().lgi_create(C.byref(h), device, stream))
         self._h = h
+        self.stream = stream  # hipStream_t handle the context orders its work on (None: null stream)
 
     def close(self):
         h, self._h = getattr(self, "_h", None), None
@@ -73,6 +74,7 @@ class Context:
 
     def set_stream(self, stream):
         self._c(lib().lgi_set_stream(self._h, stream))
+        self.stream = stream
 
     # ------------------------------------------------------------------ scene
     def load_json(self, path, verbose=False):

@@ -206,10 +206,16 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, l
     mse, ssim, mad = D.image_metrics(D.tonemap(ctx, den), gt)
     res["unet"] = {"spp": 8, "render_ms": t_render, "denoise_ms": t_den_g, "denoise_eager_ms": t_den,
                    "psnr": D.psnr_from_mse(mse), "ssim": ssim}
+    # the whole 8-spp + UNet frame (clear, render, sort, accumulate, UNet, tonemap) as one graph
+    fg = D.FrameGraph(ctx, model, cam, spp=8, bounces=bounces, sample0=test0 + (1 << 20), flags=flags)
+    fg.replay()
+    (hdr_g, _), t_frame = timed(fg.replay)
+    assert torch.equal(hdr_g, den), "frame graph differs from the eager render + denoise"
+    res["unet"]["frame_graph_ms"] = t_frame
     mse8, ssim8, _ = D.image_metrics(D.tonemap(ctx, fr.mean()), gt)
     res["noisy8"] = {"psnr": D.psnr_from_mse(mse8), "ssim": ssim8}
     log(f"8 spp raw: {D.psnr_from_mse(mse8):.2f} dB; 8 spp + UNet: {res['unet']['psnr']:.2f} dB ssim {ssim:.4f} "
-        f"({t_render:.1f} + {t_den_g:.1f} ms; eager denoise {t_den:.1f} ms)")
+        f"({t_render:.1f} + {t_den_g:.1f} ms; eager denoise {t_den:.1f} ms; one-graph frame {t_frame:.1f} ms)")
     if fallback:
         warm = D.Frame(cam)  # warm-up of the tile path on a throw-away frame
         warm.render(ctx, 8, bounces, sample0=test0 + (1 << 22), flags=flags)
@@ -223,7 +229,7 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, l
     res["plain_spp_40db"] = spp40
     res["plain_ms_40db"] = None if spp40 is None else spp40 * ms_per_spp
     # the cheapest UNet variant that reaches 40 dB (plain UNet first, then with the fallback)
-    res["unet"]["ms"] = res["unet"]["render_ms"] + res["unet"]["denoise_ms"]
+    res["unet"]["ms"] = min(res["unet"]["render_ms"] + res["unet"]["denoise_ms"], res["unet"]["frame_graph_ms"])
     cands = [(res[k]["ms"], k) for k in ("unet", "unet_fallback") if k in res and res[k]["psnr"] >= 40.0]
     res["unet_reaches_40db"] = bool(cands)
     if cands:

@@ -85,6 +85,27 @@ def test_denoise_graph_replays_match_eager(gpu):
         _ = (x * 2).sum().item()  # eager work between replays
 
 
+def test_frame_graph_replays_match_eager(gpu):
+    """Render + UNet + tonemap captured as one graph: each replay equals the eager frame."""
+    import torch
+    from lgi import denoise as D
+    from lgi import train
+    from lgi.unet import UNet
+    ctx = train.make_scene(1, 2, 96, 64)
+    cam = ctx.camera()
+    model = UNet(seed=5).cuda().eval()
+    fg = D.FrameGraph(ctx, model, cam, spp=4, sample0=9)
+    f = D.Frame(cam)
+    f.render(ctx, 4, 2, sample0=9, flags=2)
+    want = D.denoise(model, f.mean())
+    for k in range(3):
+        hdr, rgb = fg.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(hdr, want), f"replay {k}"
+        assert torch.equal(rgb, D.tonemap(ctx, want))
+        f.render(ctx, 1, 2, sample0=100 + k)  # eager work between replays
+
+
 @needs_ckpt
 def test_trained_denoiser_gains_over_the_plain_tracer(gpu):
     """What the learned part buys: on the held-out pose (864x432, error against 1024 spp),
