@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, c
                                                           int shift, const uint32_t* offs, int nb) {
     __shared__ uint32_t s_base[256];
     if (n_dev) n = min(n, (int)*n_dev);
+    if (blockIdx.x * kTile >= n) return;  // whole block past the device-side count
     __shared__ uint32_t s_wc[kBlock / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     s_base[tid] = offs[(size_t)tid * nb + blockIdx.x];
