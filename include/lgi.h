@@ -189,7 +189,9 @@ typedef struct lgi_pose_job {
  * with the same jobs. */
 LGI_API lgi_status lgi_render_batch(lgi_ctx* ctx, const lgi_pose_job* jobs, int32_t n, int32_t bounces,
                                     uint32_t flags, float* d_sum, float* d_comp, float* d_sumsq, int32_t use_graph);
-/* Adds spp samples (from sample0) to the listed 8x8 tiles of one image (host tile list). */
+/* Adds spp samples (from sample0) to the listed, distinct 8x8 tiles of one image (host tile
+ * list). With LGI_RENDER_SORTED all samples of all listed tiles run in the wavefront renderer
+ * at once; otherwise one persistent-queue item per tile (one wave per tile). */
 LGI_API lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam, const int32_t* tiles, int32_t n_tiles,
                                     int32_t spp, int32_t sample0, int32_t bounces, uint32_t flags, float* d_sum,
                                     float* d_comp, float* d_sumsq);

@@ -577,15 +577,30 @@ lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam_in, const int32_
     const lgi_camera& cam = cam_in ? *cam_in : ctx->scene.camera;
     lgi_status st = check_render_ready(ctx, cam);
     if (st != LGI_OK) return st;
-    if (flags & (LGI_RENDER_SORTED | LGI_RNG_XORSHIFT_REF))
-        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "tile renders use the PCG32 inline path");
+    if (flags & LGI_RNG_XORSHIFT_REF)
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "tile renders use PCG32 per-sample streams");
     const int ntiles = ((cam.width + 7) / 8) * ((cam.height + 7) / 8);
+    std::vector<uint8_t> seen((size_t)ntiles, 0);
     std::vector<lgi_work_item> items;
     for (int i = 0; i < n_tiles; ++i) {
         if (tiles[i] < 0 || tiles[i] >= ntiles) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "tile index out of range");
+        if (seen[tiles[i]]++) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "tile listed twice");
         if (spp > 0) items.push_back({0, tiles[i], spp, sample0});
     }
     if (items.empty()) return LGI_OK;
+    if (flags & LGI_RENDER_SORTED) {
+        // all samples of all listed tiles in flight at once in the wavefront renderer
+        int32_t* d_tiles = (int32_t*)ctx->scratch_get(48, (size_t)n_tiles * 4);
+        if (!d_tiles) return lgi_set_err(ctx, LGI_ERR_HIP, "tile list allocation failed");
+        LGI_HIP(ctx, hipMemcpyAsync(d_tiles, tiles, (size_t)n_tiles * 4, hipMemcpyHostToDevice, ctx->stream));
+        TraceParams P = make_params(ctx, cam, spp, bounces, flags, sample0);
+        P.sum = d_sum;
+        P.comp = d_comp;
+        P.sumsq = d_sumsq;
+        const lgi_bvh_node& r = ctx->root;
+        return lgi_render_wavefront(ctx, P, make_float3(r.bb_min[0], r.bb_min[1], r.bb_min[2]),
+                                    make_float3(r.bb_max[0], r.bb_max[1], r.bb_max[2]), d_tiles, n_tiles);
+    }
     std::vector<DevCamera> cams{lgi_make_dev_camera(cam)};
     return run_queue(ctx, items, cams, cam, bounces, flags, d_sum, d_comp, d_sumsq, false);
 }

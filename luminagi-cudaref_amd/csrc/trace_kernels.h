@@ -45,4 +45,6 @@ void lgi_launch_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out
 void lgi_launch_render_queue(const TraceParams& P, const DevCamera* cams, const lgi_work_item* items, int n_items,
                              uint32_t* counter, int grid, hipStream_t st);
 // Sorted wavefront renderer (wavefront.hip); bb = scene box used by the ray sort key.
-lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max);
+// d_tiles (device, n_tiles distinct row-major 8x8 tile ids) restricts the render to those tiles.
+lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max,
+                                const int32_t* d_tiles = nullptr, int n_tiles = 0);

@@ -47,6 +47,7 @@ struct WfParams {
     uint8_t* nee;         // per (queue slot, light): 0 none, 1 candidate, 2 occluded
     float4* vP;           // per queue slot: shading point (xyz)
     float4* vN;           // per queue slot: facing shading normal (xyz)
+    const int32_t* tiles; // 8x8 tiles to render (row-major tile ids), or null for the whole image
 };
 
 // Shadow rays are not stored: the any-hit kernel reads the id j = slot * nl + light of each
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= (int64_t)S * padded) return;
     const int s = (int)(i / padded), j = (int)(i % padded);
-    const int tile = j >> 6, lane = j & 63;
+    const int tile = W.tiles ? W.tiles[j >> 6] : j >> 6, lane = j & 63;
     const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
     const DevCamera& c = W.P.cam;
     QRay r;
@@ -478,9 +479,19 @@ __global__ __launch_bounds__(kBlock) void k_wf_fill(uint32_t* p, int n, uint32_t
 }
 
 // Kahan-add the S samples of every pixel in sample order (same as the inline kernel).
-__global__ __launch_bounds__(kBlock) void k_wf_accum(WfParams W, int S) {
-    const int pix = blockIdx.x * kBlock + threadIdx.x;
-    if (pix >= W.npx) return;
+__global__ __launch_bounds__(kBlock) void k_wf_accum(WfParams W, int S, int n_tiles, int tiles_x) {
+    int pix;
+    if (W.tiles) {  // pixels of the listed tiles
+        const int k = blockIdx.x * kBlock + threadIdx.x;
+        if (k >= n_tiles * 64) return;
+        const int tile = W.tiles[k >> 6], lane = k & 63;
+        const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
+        if (px >= W.P.cam.width || py >= W.P.cam.height) return;
+        pix = py * W.P.cam.width + px;
+    } else {
+        pix = blockIdx.x * kBlock + threadIdx.x;
+        if (pix >= W.npx) return;
+    }
     float* sum = W.P.sum + 3 * (int64_t)pix;
     float* comp = W.P.comp + 3 * (int64_t)pix;
     float s0 = sum[0], s1 = sum[1], s2 = sum[2], c0 = comp[0], c1 = comp[1], c2 = comp[2];
@@ -524,12 +535,13 @@ int trace_grid(int n) {
 }  // namespace
 
 // Host driver of the wavefront renderer. Adds spp samples into P.sum/P.comp/P.sumsq.
-lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max) {
+lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max,
+                                const int32_t* d_tiles, int n_tiles) {
     hipStream_t st = ctx->stream;
     const int Wd = P.cam.width, Hd = P.cam.height;
     const int npx = Wd * Hd;
     const int tiles_x = (Wd + 7) / 8, tiles_y = (Hd + 7) / 8;
-    const int padded = tiles_x * tiles_y * 64;
+    const int padded = (d_tiles ? n_tiles : tiles_x * tiles_y) * 64;
     const int nl = 1 + P.n_points;
     // samples per pass: keep the path count under ~24M (a few GB of queues)
     const int max_paths = 24 << 20;
@@ -556,6 +568,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     if (!spill) return lgi_set_err(ctx, LGI_ERR_HIP, "stack spill allocation failed");
     WfParams W;
     W.P = P;
+    W.tiles = d_tiles;
     W.npx = npx;
     W.nl = nl;
     W.bb_min = bb_min;
@@ -626,7 +639,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             std::swap(cur, other);
             n_dev = cnt + 2;
         }
-        hipLaunchKernelGGL(k_wf_accum, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, st, W, S);
+        const int n_acc = d_tiles ? n_tiles * 64 : npx;
+        hipLaunchKernelGGL(k_wf_accum, dim3(cdiv(n_acc, kBlock)), dim3(kBlock), 0, st, W, S, n_tiles, tiles_x);
     }
     LGI_HIP(ctx, hipGetLastError());
     if (timed) LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));

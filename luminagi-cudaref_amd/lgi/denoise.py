@@ -179,9 +179,11 @@ def gate_tiles(noisy_mean, se, denoised, k=4.0, rel_floor=0.02, abs_floor=1e-3):
     return torch.nonzero(bad.reshape(-1)).flatten().to(torch.int32)
 
 
-def denoise_with_fallback(ctx, model, frame, extra_spp=248, bounces=2, sample0=1 << 22, k=4.0, rel_floor=0.02):
-    """Denoise frame; re-render gated tiles with extra_spp more samples and keep their plain mean.
-    Returns (radiance HxWx3, number of fallback tiles)."""
+def denoise_with_fallback(ctx, model, frame, extra_spp=248, bounces=2, sample0=1 << 22, k=4.0, rel_floor=0.02,
+                          flags=2):
+    """Denoise frame; re-render gated tiles with extra_spp more samples (sorted renderer on the
+    listed tiles by default) and keep their plain mean. Returns (radiance HxWx3, number of
+    fallback tiles)."""
     noisy = frame.mean()
     den = denoise(model, noisy)
     tiles = gate_tiles(noisy, frame.lum_se(), den, k=k, rel_floor=rel_floor)
@@ -189,7 +191,7 @@ def denoise_with_fallback(ctx, model, frame, extra_spp=248, bounces=2, sample0=1
         return den, 0
     cam = frame.cam
     ctx.render_tiles(tiles.cpu().numpy(), extra_spp, frame.sum, frame.comp, bounces, sample0=sample0,
-                     d_sumsq=frame.sumsq, camera=cam)
+                     d_sumsq=frame.sumsq, camera=cam, flags=flags)
     h, w = cam.height, cam.width
     th, tw = (h + 7) // 8, (w + 7) // 8
     flag = torch.zeros(th * tw, dtype=torch.bool, device=noisy.device)

@@ -60,7 +60,10 @@ def test_batch_graph_replays_match_eager(gpu):
         _ = (s * 2).sum().item()
 
 
-def test_tiles_render_only_listed_tiles(gpu):
+@pytest.mark.parametrize("flags", [0, 2], ids=["queue", "sorted"])
+def test_tiles_render_only_listed_tiles(gpu, flags):
+    """Tile renders (persistent queue, or the sorted wavefront renderer on the listed tiles)
+    equal a full render of the same samples on those tiles and touch nothing else."""
     import torch
     ctx = setup(w=64, h=40, seed=3)
     cam = ctx.camera()
@@ -70,7 +73,7 @@ def test_tiles_render_only_listed_tiles(gpu):
     ctx.accumulate(full[0], full[1], 6, 2, sample0=2, camera=cam)
     tiles = [0, 5, 17, 39]  # 8 x 5 tiles
     part = [torch.zeros(n * 3, device="cuda"), torch.zeros(n * 3, device="cuda")]
-    ctx.render_tiles(tiles, 6, part[0], part[1], 2, sample0=2, camera=cam)
+    ctx.render_tiles(tiles, 6, part[0], part[1], 2, sample0=2, camera=cam, flags=flags)
     torch.cuda.synchronize()
     f = full[0].cpu().numpy().reshape(H, W, 3)
     p = part[0].cpu().numpy().reshape(H, W, 3)
@@ -80,3 +83,15 @@ def test_tiles_render_only_listed_tiles(gpu):
         mask[ty * 8:(ty + 1) * 8, tx * 8:(tx + 1) * 8] = True
     assert np.array_equal(p[mask], f[mask])
     assert not p[~mask].any()
+
+
+def test_tiles_listed_twice_are_rejected(gpu):
+    import torch
+    import lgi
+    ctx = setup(w=32, h=16, seed=3)
+    cam = ctx.camera()
+    n = cam.width * cam.height
+    buf = [torch.zeros(n * 3, device="cuda"), torch.zeros(n * 3, device="cuda")]
+    with pytest.raises(lgi.LgiError) as e:
+        ctx.render_tiles([1, 1], 2, buf[0], buf[1], 1, camera=cam)
+    assert e.value.status == 1
