@@ -16,6 +16,8 @@
 // (one per Morton bit), so 32 entries never overflow.
 #include "trace_device.h"
 
+#include <climits>
+
 using namespace lgi;
 
 namespace {
@@ -28,14 +30,17 @@ __device__ __forceinline__ TravStack make_stack(const TraceParams& P, int* s_sta
     return TravStack{s_stack + threadIdx.x, P.spill + gtid, P.spill_stride};
 }
 
-// Closest hit (kAny = false) or any hit within (1e-6, tmax) (kAny = true).
+// Closest hit (kAny = false) or any hit within (1e-6, tmax) (kAny = true). While-while loop:
+// a lane that reaches a leaf parks it and keeps descending until every lane still walking
+// internal nodes holds a parked leaf, then the parked leaves are tested together. Leaves are
+// still tested in the order they are reached, so the result equals the plain loop's.
 template <bool kAny>
 __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, const TravStack& stk) {
+    constexpr int kNone = INT_MIN;  // stack exhausted
     Hit h{-1, tmax, 0.f, 0.f};
-    int node = 0;
-    int sp = 0;
-    for (;;) {
-        if (node >= 0) {
+    int node = 0, leaf = 0, sp = 0;
+    while (node != kNone || leaf != 0) {
+        while (node >= 0) {
             const TNode n = P.tnodes[node];
             float ta, tb;
             const bool ha = slab(r, n.b0xy.x, n.b0xy.y, n.b0xy.z, n.b0xy.w, n.bz.x, n.bz.y, h.t, ta);
@@ -46,12 +51,26 @@ __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, const Tr
                 stk.push(sp, farc);
                 ++sp;
                 node = nearc;
-                continue;
+            } else if (ha) {
+                node = n.ch.x;
+            } else if (hb) {
+                node = n.ch.y;
+            } else {
+                node = sp > 0 ? stk.pop(--sp) : kNone;
             }
-            if (ha) { node = n.ch.x; continue; }
-            if (hb) { node = n.ch.y; continue; }
-        } else {
-            int ti = ~node;
+            if (node < 0 && node != kNone && leaf == 0) {
+                leaf = node;
+                node = sp > 0 ? stk.pop(--sp) : kNone;
+            }
+            if (__ballot(leaf == 0) == 0) break;
+        }
+        while (leaf != 0 || (node < 0 && node != kNone)) {
+            if (leaf == 0) {
+                leaf = node;
+                node = sp > 0 ? stk.pop(--sp) : kNone;
+            }
+            int ti = ~leaf;
+            leaf = 0;
             for (;;) {
                 const TriV t = P.triv[ti];
                 float tt, uu, vv;
@@ -63,9 +82,6 @@ __device__ Hit traverse(const TraceParams& P, const Ray& r, float tmax, const Tr
                 ++ti;
             }
         }
-        if (sp == 0) break;
-        --sp;
-        node = stk.pop(sp);
     }
     return h;
 }
