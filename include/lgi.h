@@ -114,6 +114,13 @@ LGI_API const char* lgi_version(void);
 /* ---- scene (LoadSceneJSON, include/scene.h:58) ---- */
 LGI_API lgi_status lgi_scene_load_json(lgi_ctx* ctx, const char* path, int verbose);
 LGI_API lgi_status lgi_scene_save_json(lgi_ctx* ctx, const char* path);
+/* Binary scene file: the same scene model as bulk little-endian records (64-byte header,
+ * camera, sun, lights, materials with texture paths, 100-byte triangles). No reference
+ * counterpart: the reference reads only scene.json, whose DOM parse takes 12 s for 300 MB
+ * (README.md:46; binary format listed as a TODO at README.md:196). Errors: LGI_ERR_IO,
+ * LGI_ERR_PARSE (bad magic, truncated or inconsistent sizes), LGI_ERR_VERSION. */
+LGI_API lgi_status lgi_scene_load_bin(lgi_ctx* ctx, const char* path, int verbose);
+LGI_API lgi_status lgi_scene_save_bin(lgi_ctx* ctx, const char* path);
 LGI_API lgi_status lgi_scene_set(lgi_ctx* ctx, const lgi_camera* cam, const lgi_dir_light* sun,
                                  const lgi_dir_light* dirs, int32_t n_dirs,
                                  const lgi_point_light* points, int32_t n_points,

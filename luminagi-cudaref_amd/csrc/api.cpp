@@ -272,6 +272,21 @@ lgi_status lgi_scene_save_json(lgi_ctx* ctx, const char* path) {
     return scene_save_json(path, ctx->scene, ctx->err);
 }
 
+lgi_status lgi_scene_load_bin(lgi_ctx* ctx, const char* path, int verbose) {
+    if (!ctx || !path) return LGI_ERR_INVALID_ARG;
+    HostScene s;
+    lgi_status st = scene_load_bin(path, s, ctx->err, verbose != 0);
+    if (st != LGI_OK) return st;
+    ctx->scene = std::move(s);
+    scene_changed(ctx);
+    return LGI_OK;
+}
+
+lgi_status lgi_scene_save_bin(lgi_ctx* ctx, const char* path) {
+    if (!ctx || !path) return LGI_ERR_INVALID_ARG;
+    return scene_save_bin(path, ctx->scene, ctx->err);
+}
+
 lgi_status lgi_scene_set(lgi_ctx* ctx, const lgi_camera* cam, const lgi_dir_light* sun, const lgi_dir_light* dirs,
                          int32_t n_dirs, const lgi_point_light* points, int32_t n_points, const lgi_material* mats,
                          int32_t n_mats, const lgi_triangle* tris, int64_t n_tris) {

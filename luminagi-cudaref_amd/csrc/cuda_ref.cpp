@@ -1,7 +1,7 @@
 // cuda_ref.cpp — command-line renderer with the reference's argv surface, exit codes and
 // log lines (src/main.cpp:93-229), built on the liblgi C ABI.
 //
-//   cuda_ref <scene.json> [out.png] [-o out.png] [--spp N] [--bounces N] [--asset-root DIR]
+//   cuda_ref <scene.json | scene.lgis> [out.png] [-o out.png] [--spp N] [--bounces N] [--asset-root DIR]
 //            [--sort] [--gbuffer] [--cam-pos x y z --cam-target x y z] [--batch file]
 //            [--hdr out.f32] [--rng xorshift]
 // Exit codes: 1 usage, 2 scene load failure, 3 PNG write failure, 4 GPU failure.
@@ -79,7 +79,10 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "no HIP device\n");
         return 4;
     }
-    if (lgi_scene_load_json(ctx, jsonPath, 1) != LGI_OK) {
+    // a .lgis path is the binary scene file (lgi_scene_save_bin), anything else scene.json
+    const std::string sp(jsonPath);
+    const bool bin = sp.size() > 5 && sp.compare(sp.size() - 5, 5, ".lgis") == 0;
+    if ((bin ? lgi_scene_load_bin(ctx, jsonPath, 1) : lgi_scene_load_json(ctx, jsonPath, 1)) != LGI_OK) {
         lgi_destroy(ctx);
         return 2;
     }

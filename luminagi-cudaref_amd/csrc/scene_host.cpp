@@ -856,3 +856,127 @@ lgi_status parse_batch_file(const char* path, std::vector<lgi_batch_job>& jobs, 
     }
     return LGI_OK;
 }
+
+// ------------------------------------------------------------------ binary scene file
+// Little-endian container of the same scene model, written and read with bulk copies: a 64-byte
+// header, the camera and sun records, the directional and point lights, the materials (albedo,
+// emissive, texture path) and the 100-byte triangles, all as laid out in include/lgi.h.
+namespace {
+
+constexpr char kBinMagic[8] = {'L', 'G', 'I', 'S', 'C', 'N', '1', '\0'};
+constexpr uint32_t kBinVersion = 1;
+
+struct BinHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    uint64_t n_tris;
+    uint32_t n_mats, n_points, n_dirs, reserved;
+    uint64_t file_bytes;  // total size, checked on load
+    uint8_t pad[16];
+};
+static_assert(sizeof(BinHeader) == 64, "binary scene header");
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+lgi_status scene_save_bin(const char* path, const HostScene& s, std::string& err) {
+    BinHeader h{};
+    std::memcpy(h.magic, kBinMagic, 8);
+    h.version = kBinVersion;
+    h.header_bytes = sizeof(BinHeader);
+    h.n_tris = s.tris.size();
+    h.n_mats = (uint32_t)s.materials.size();
+    h.n_points = (uint32_t)s.points.size();
+    h.n_dirs = (uint32_t)s.dirs.size();
+    uint64_t size = sizeof(BinHeader) + sizeof(lgi_camera) + sizeof(lgi_dir_light) + h.n_dirs * sizeof(lgi_dir_light) +
+                    h.n_points * sizeof(lgi_point_light) + h.n_tris * sizeof(lgi_triangle);
+    for (const HostMaterial& m : s.materials) size += 24 + 4 + m.diffuse_tex.size();
+    h.file_bytes = size;
+    FileCloser fc{std::fopen(path, "wb")};
+    if (!fc.f) {
+        err = std::string("cannot open ") + path + " for writing";
+        return LGI_ERR_IO;
+    }
+    bool ok = std::fwrite(&h, sizeof h, 1, fc.f) == 1;
+    ok = ok && std::fwrite(&s.camera, sizeof(lgi_camera), 1, fc.f) == 1;
+    ok = ok && std::fwrite(&s.sun, sizeof(lgi_dir_light), 1, fc.f) == 1;
+    ok = ok && std::fwrite(s.dirs.data(), sizeof(lgi_dir_light), s.dirs.size(), fc.f) == s.dirs.size();
+    ok = ok && std::fwrite(s.points.data(), sizeof(lgi_point_light), s.points.size(), fc.f) == s.points.size();
+    for (const HostMaterial& m : s.materials) {
+        const uint32_t len = (uint32_t)m.diffuse_tex.size();
+        ok = ok && std::fwrite(m.albedo, 4, 3, fc.f) == 3 && std::fwrite(m.emissive, 4, 3, fc.f) == 3;
+        ok = ok && std::fwrite(&len, 4, 1, fc.f) == 1 && std::fwrite(m.diffuse_tex.data(), 1, len, fc.f) == len;
+    }
+    ok = ok && std::fwrite(s.tris.data(), sizeof(lgi_triangle), s.tris.size(), fc.f) == s.tris.size();
+    ok = std::fclose(fc.f) == 0 && ok;
+    fc.f = nullptr;
+    if (!ok) {
+        err = std::string("write to ") + path + " failed";
+        return LGI_ERR_IO;
+    }
+    return LGI_OK;
+}
+
+lgi_status scene_load_bin(const char* path, HostScene& out, std::string& err, bool verbose) {
+    FileCloser fc{std::fopen(path, "rb")};
+    if (!fc.f) {
+        err = std::string("cannot open ") + path;
+        return LGI_ERR_IO;
+    }
+    std::fseek(fc.f, 0, SEEK_END);
+    const long long fsize = std::ftell(fc.f);
+    std::fseek(fc.f, 0, SEEK_SET);
+    BinHeader h{};
+    auto bad = [&](const char* what) {
+        err = std::string(path) + ": " + what;
+        return LGI_ERR_PARSE;
+    };
+    if (fsize < (long long)sizeof h || std::fread(&h, sizeof h, 1, fc.f) != 1) return bad("truncated header");
+    if (std::memcmp(h.magic, kBinMagic, 8) != 0) return bad("not a binary scene file");
+    if (h.version != kBinVersion) {
+        err = std::string(path) + ": unsupported binary scene version " + std::to_string(h.version);
+        return LGI_ERR_VERSION;
+    }
+    if (h.header_bytes != sizeof h || h.file_bytes != (uint64_t)fsize) return bad("size mismatch");
+    const uint64_t fixed = sizeof h + sizeof(lgi_camera) + sizeof(lgi_dir_light);
+    if (h.n_tris > (uint64_t)fsize / sizeof(lgi_triangle) || h.n_points > (uint64_t)fsize ||
+        h.n_dirs > (uint64_t)fsize || h.n_mats > (uint64_t)fsize || fixed > (uint64_t)fsize)
+        return bad("counts exceed the file size");
+    HostScene s;
+    s.dirs.resize(h.n_dirs);
+    s.points.resize(h.n_points);
+    s.materials.resize(h.n_mats);
+    s.tris.resize(h.n_tris);
+    bool ok = std::fread(&s.camera, sizeof(lgi_camera), 1, fc.f) == 1;
+    ok = ok && std::fread(&s.sun, sizeof(lgi_dir_light), 1, fc.f) == 1;
+    ok = ok && std::fread(s.dirs.data(), sizeof(lgi_dir_light), h.n_dirs, fc.f) == h.n_dirs;
+    ok = ok && std::fread(s.points.data(), sizeof(lgi_point_light), h.n_points, fc.f) == h.n_points;
+    for (HostMaterial& m : s.materials) {
+        uint32_t len = 0;
+        ok = ok && std::fread(m.albedo, 4, 3, fc.f) == 3 && std::fread(m.emissive, 4, 3, fc.f) == 3;
+        ok = ok && std::fread(&len, 4, 1, fc.f) == 1 && len <= (uint64_t)fsize;
+        if (ok) {
+            m.diffuse_tex.resize(len);
+            ok = std::fread(&m.diffuse_tex[0], 1, len, fc.f) == len;
+        }
+    }
+    ok = ok && std::fread(s.tris.data(), sizeof(lgi_triangle), h.n_tris, fc.f) == h.n_tris;
+    if (!ok || std::ftell(fc.f) != fsize) return bad("truncated or inconsistent");
+    out = std::move(s);
+    if (verbose) {
+        std::printf("[scene] loaded %s\n", path);
+        std::printf("  camera: %dx%d, fov %.1f deg, pos (%.2f,%.2f,%.2f)\n", out.camera.width,
+                    out.camera.height, out.camera.fov_y_deg, out.camera.pos[0], out.camera.pos[1],
+                    out.camera.pos[2]);
+        std::printf("  %zu directional + %zu point lights, %zu materials, %zu triangles\n",
+                    out.dirs.size(), out.points.size(), out.materials.size(), out.tris.size());
+    }
+    return LGI_OK;
+}

@@ -29,6 +29,10 @@ struct HostScene {
 // reference's summary lines.
 lgi_status scene_load_json(const char* path, HostScene& out, std::string& err, bool verbose);
 lgi_status scene_save_json(const char* path, const HostScene& s, std::string& err);
+// Binary scene file (the same model in bulk little-endian records): LGI_ERR_IO, LGI_ERR_PARSE
+// (bad magic, truncated or inconsistent sizes) or LGI_ERR_VERSION.
+lgi_status scene_save_bin(const char* path, const HostScene& s, std::string& err);
+lgi_status scene_load_bin(const char* path, HostScene& out, std::string& err, bool verbose);
 
 // Procedural scenes. kind 1: ground quad + 16 meshes (icosphere L4 / 8x8 box / 64x32
 // torus), ~65k triangles. kind 2: ground + 62 meshes (geodesic spheres, tori, value-noise

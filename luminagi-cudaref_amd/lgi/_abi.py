@@ -71,6 +71,8 @@ SIGNATURES = {
     "lgi_version": (C.c_char_p, []),
     "lgi_scene_load_json": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "lgi_scene_save_json": (C.c_int, [_P, C.c_char_p]),
+    "lgi_scene_load_bin": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "lgi_scene_save_bin": (C.c_int, [_P, C.c_char_p]),
     "lgi_scene_set": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(DirLight), C.POINTER(DirLight), C.c_int32,
                                  _P, C.c_int32, C.POINTER(Material), C.c_int32, _P, C.c_int64]),
     "lgi_scene_generate": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_int32, C.c_int32]),

@@ -83,6 +83,13 @@ class Context:
     def save_json(self, path):
         self._c(lib().lgi_scene_save_json(self._h, str(path).encode()))
 
+    def load_bin(self, path, verbose=False):
+        """Binary scene file written by save_bin (bulk records, no text parsing)."""
+        self._c(lib().lgi_scene_load_bin(self._h, str(path).encode(), int(verbose)))
+
+    def save_bin(self, path):
+        self._c(lib().lgi_scene_save_bin(self._h, str(path).encode()))
+
     def generate(self, kind, seed, width, height):
         self._c(lib().lgi_scene_generate(self._h, kind, seed, width, height))
 
