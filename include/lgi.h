@@ -135,6 +135,12 @@ LGI_API lgi_status lgi_scene_get(const lgi_ctx* ctx, lgi_camera* cam, lgi_dir_li
                                  lgi_point_light* points, lgi_material* mats,
                                  lgi_triangle* tris /* current order (permuted after a build) */);
 LGI_API lgi_status lgi_scene_set_camera(lgi_ctx* ctx, const lgi_camera* cam);
+/* Directory the materials' diffuse_tex paths are relative to (the reference's --asset-root,
+ * src/main.cpp:118-141; default: the loaded scene file's grandparent directory). Null restores
+ * the default. Textures (PNG) load with the scene upload and reload before the next render
+ * after a change; unreadable ones are reported and leave the material with its constant albedo
+ * (src/pathtracer.cu:594-612). */
+LGI_API lgi_status lgi_set_asset_root(lgi_ctx* ctx, const char* dir);
 /* overwrite_camera, src/main.cpp:29-59 (no-op when |target-pos| < 1e-8). */
 LGI_API lgi_status lgi_camera_look_at(lgi_camera* cam, const float pos[3], const float target[3]);
 
@@ -214,6 +220,9 @@ LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_
 
 /* ---- image output (stbi_write_png call sites, src/main.cpp:176,214) ---- */
 LGI_API lgi_status lgi_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h);
+/* PNG -> RGBA8 rows (stbi_load(..., 4), src/pathtracer.cu:557): non-interlaced, depth 8 or 16,
+ * grey / grey+alpha / RGB / RGBA / palette. rgba may be null to query the size. */
+LGI_API lgi_status lgi_read_png(const char* path, int32_t* w, int32_t* h, uint8_t* rgba);
 
 /* ---- denoiser layers (UNet of ml/train.py:37-75; NHWC BF16 activations on device) ----
  * All pointers are device pointers; `stream` is a hipStream_t (null = default stream).

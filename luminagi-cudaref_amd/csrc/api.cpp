@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <new>
 
 #include "lgi_internal.h"
@@ -63,6 +64,55 @@ void scene_changed(lgi_ctx* ctx) {
     ctx->bvh_ready = false;
 }
 
+// Diffuse textures (src/pathtracer.cu:594-612): every material's texture is loaded from
+// asset_root + "/" + path, missing or unreadable files are reported and leave the material
+// with its constant albedo. Loaded texels go into one device buffer with a descriptor per
+// material, next to the 8-bit sRGB -> linear table the sampler uses.
+lgi_status upload_textures(lgi_ctx* ctx) {
+    DevScene& d = ctx->dev;
+    const HostScene& s = ctx->scene;
+    d.tex_loaded = d.tex_missing = 0;
+    ctx->tex_dirty = false;
+    const std::string& root = ctx->asset_root_set ? ctx->asset_root : ctx->scene_root;
+    bool any = false;
+    for (const HostMaterial& m : s.materials) any |= !m.diffuse_tex.empty();
+    if (!any) return LGI_OK;
+    std::vector<int4> desc(s.materials.size(), make_int4(0, 0, 0, 0));
+    std::vector<uint32_t> texels;
+    for (size_t i = 0; i < s.materials.size(); ++i) {
+        const std::string& rel = s.materials[i].diffuse_tex;
+        if (rel.empty()) continue;
+        const std::string full = root + "/" + rel;
+        int w = 0, h = 0;
+        std::vector<uint8_t> rgba;
+        std::string err;
+        if (read_png_rgba(full.c_str(), w, h, rgba, err) != LGI_OK ||
+            texels.size() + (size_t)w * h > (size_t)INT32_MAX) {
+            std::fprintf(stderr, "  texture not found: %s\n", full.c_str());
+            ++d.tex_missing;
+            continue;
+        }
+        desc[i] = make_int4((int)texels.size(), w, h, 1);
+        const size_t at = texels.size();
+        texels.resize(at + (size_t)w * h);
+        std::memcpy(&texels[at], rgba.data(), (size_t)w * h * 4);
+        ++d.tex_loaded;
+    }
+    std::vector<float> lut(256);
+    for (int i = 0; i < 256; ++i) {
+        const double c = i / 255.0;
+        lut[i] = (float)(c <= 0.04045 ? c / 12.92 : std::pow((c + 0.055) / 1.055, 2.4));
+    }
+    LGI_HIP(ctx, hipMalloc((void**)&d.srgb_lut, 256 * 4));
+    LGI_HIP(ctx, hipMemcpy(d.srgb_lut, lut.data(), 256 * 4, hipMemcpyHostToDevice));
+    LGI_HIP(ctx, hipMalloc((void**)&d.tex_desc, desc.size() * sizeof(int4)));
+    LGI_HIP(ctx, hipMemcpy(d.tex_desc, desc.data(), desc.size() * sizeof(int4), hipMemcpyHostToDevice));
+    LGI_HIP(ctx, hipMalloc((void**)&d.tex_texels, std::max<size_t>(1, texels.size()) * 4));
+    if (!texels.empty())
+        LGI_HIP(ctx, hipMemcpy(d.tex_texels, texels.data(), texels.size() * 4, hipMemcpyHostToDevice));
+    return LGI_OK;
+}
+
 lgi_status upload_scene(lgi_ctx* ctx) {
     lgi_status st = ensure_gpu(ctx);
     if (st != LGI_OK) return st;
@@ -89,6 +139,7 @@ lgi_status upload_scene(lgi_ctx* ctx) {
     LGI_HIP(ctx, hipMalloc((void**)&d.points, sizeof(lgi_point_light) * std::max(1, d.n_points)));
     if (d.n_points)
         LGI_HIP(ctx, hipMemcpy(d.points, s.points.data(), sizeof(lgi_point_light) * d.n_points, hipMemcpyHostToDevice));
+    if ((st = upload_textures(ctx)) != LGI_OK) return st;
     ctx->uploaded = true;
     return LGI_OK;
 }
@@ -106,6 +157,9 @@ TraceParams make_params(lgi_ctx* ctx, const lgi_camera& cam, int spp, int bounce
     P.triv = d.triv;
     P.tris = d.tris;
     P.mat_albedo = d.mat_albedo;
+    P.tex_texels = d.tex_texels;
+    P.tex_desc = d.tex_desc;
+    P.srgb_lut = d.srgb_lut;
     P.points = d.points;
     P.n_points = d.n_points;
     for (int k = 0; k < 3; ++k) {
@@ -193,6 +247,17 @@ lgi_status check_render_ready(lgi_ctx* ctx, const lgi_camera& cam) {
     if (!ctx) return LGI_ERR_INVALID_ARG;
     if (ctx->scene.tris.empty()) return lgi_set_err(ctx, LGI_ERR_NO_SCENE, "no scene");
     if (!ctx->bvh_ready) return lgi_set_err(ctx, LGI_ERR_NO_BVH, "lgi_bvh_build has not run");
+    if (ctx->tex_dirty) {  // the asset root changed since the upload: reload the textures only
+        DevScene& d = ctx->dev;
+        for (void* p : {(void*)d.tex_texels, (void*)d.tex_desc, (void*)d.srgb_lut})
+            if (p) LGI_HIP(ctx, hipFree(p));
+        d.tex_texels = nullptr;
+        d.tex_desc = nullptr;
+        d.srgb_lut = nullptr;
+        lgi_status st = upload_textures(ctx);
+        if (st != LGI_OK) return st;
+        ctx->tex_dirty = false;
+    }
     if (cam.width <= 0 || cam.height <= 0 || (int64_t)cam.width * cam.height > (int64_t(1) << 30))
         return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "bad image size");
     return LGI_OK;
@@ -263,7 +328,18 @@ lgi_status lgi_scene_load_json(lgi_ctx* ctx, const char* path, int verbose) {
     lgi_status st = scene_load_json(path, s, ctx->err, verbose != 0);
     if (st != LGI_OK) return st;
     ctx->scene = std::move(s);
+    // texture paths resolve against the scene's grandparent directory unless an asset root was
+    // set (the reference CLI's default, src/main.cpp:140-141)
+    ctx->scene_root = std::filesystem::path(path).parent_path().parent_path().string();
     scene_changed(ctx);
+    return LGI_OK;
+}
+
+lgi_status lgi_set_asset_root(lgi_ctx* ctx, const char* dir) {
+    if (!ctx) return LGI_ERR_INVALID_ARG;
+    ctx->asset_root = dir ? dir : "";
+    ctx->asset_root_set = dir != nullptr;
+    ctx->tex_dirty = ctx->uploaded;  // textures reload before the next render
     return LGI_OK;
 }
 
@@ -278,6 +354,7 @@ lgi_status lgi_scene_load_bin(lgi_ctx* ctx, const char* path, int verbose) {
     lgi_status st = scene_load_bin(path, s, ctx->err, verbose != 0);
     if (st != LGI_OK) return st;
     ctx->scene = std::move(s);
+    ctx->scene_root = std::filesystem::path(path).parent_path().parent_path().string();
     scene_changed(ctx);
     return LGI_OK;
 }
@@ -515,12 +592,8 @@ lgi_status lgi_render(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t spp, int32
     lgi_status st = check_render_ready(ctx, cam);
     if (st != LGI_OK) return st;
     if (spp <= 0) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "spp must be positive");
-    if (has_textures(ctx->scene)) {
-        // textured materials render with their constant albedo (texture sampling not built)
-        int n = 0;
-        for (auto& m : ctx->scene.materials) n += !m.diffuse_tex.empty();
-        std::printf("textures: 0 loaded, %d missing (of %zu materials)\n", n, ctx->scene.materials.size());
-    }
+    std::printf("textures: %d loaded, %d missing (of %zu materials)\n", ctx->dev.tex_loaded, ctx->dev.tex_missing,
+                ctx->scene.materials.size());
     const size_t npx = (size_t)cam.width * cam.height;
     float* sum = (float*)ctx->scratch_get(40, npx * 12);
     float* comp = (float*)ctx->scratch_get(41, npx * 12);
@@ -633,6 +706,19 @@ lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, i
 
 lgi_status lgi_write_png(const char* path, const uint8_t* rgb, int32_t w, int32_t h) {
     return write_png_rgb(path, rgb, w, h);
+}
+
+lgi_status lgi_read_png(const char* path, int32_t* w, int32_t* h, uint8_t* rgba) {
+    if (!path || !w || !h) return LGI_ERR_INVALID_ARG;
+    int iw = 0, ih = 0;
+    std::vector<uint8_t> px;
+    std::string err;
+    lgi_status st = read_png_rgba(path, iw, ih, px, err);
+    if (st != LGI_OK) return st;
+    *w = iw;
+    *h = ih;
+    if (rgba) std::memcpy(rgba, px.data(), px.size());
+    return LGI_OK;
 }
 
 }  // extern "C"

@@ -453,6 +453,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_layout(const lgi_triangle* tris,
     for (int c = 0; c < 3; ++c)
         for (int k = 0; k < 3; ++k) s.n[c * 3 + k] = src.n[c][k];
     s.mat = src.material;
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 2; ++k) s.uv[c * 2 + k] = src.uv[c][k];
     tris_s[i] = s;
 }
 
@@ -525,7 +527,8 @@ void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, 
 }
 
 void lgi_free_dev_scene(DevScene& d) {
-    void* ptrs[] = {d.tri_in, d.nodes, d.tnodes, d.triv, d.tris, d.mat_albedo, d.points, d.perm, d.keys};
+    void* ptrs[] = {d.tri_in, d.nodes, d.tnodes, d.triv, d.tris, d.mat_albedo, d.points, d.perm, d.keys,
+                    d.tex_texels, d.tex_desc, d.srgb_lut};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     d = DevScene();

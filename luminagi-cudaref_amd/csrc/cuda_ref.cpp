@@ -73,7 +73,6 @@ int main(int argc, char** argv) {
                      argv[0]);
         return 1;
     }
-    (void)assetRoot;  // textures are not sampled by this build; kept for argv compatibility
     lgi_ctx* ctx = nullptr;
     if (lgi_create(&ctx, 0, nullptr) != LGI_OK) {
         std::fprintf(stderr, "no HIP device\n");
@@ -86,6 +85,7 @@ int main(int argc, char** argv) {
         lgi_destroy(ctx);
         return 2;
     }
+    if (!assetRoot.empty()) lgi_set_asset_root(ctx, assetRoot.c_str());
     lgi_camera cam;
     lgi_scene_get(ctx, &cam, nullptr, nullptr, nullptr, nullptr);
     if (haveCamPos && haveCamTgt) {

@@ -32,12 +32,14 @@ struct alignas(16) TriV {
 };
 static_assert(sizeof(TriV) == 48, "TriV");
 
-// Shading record in leaf order: vertex normals and material index.
+// Shading record in leaf order: vertex normals, material index and vertex texture
+// coordinates (one 64-byte line, read once per closest hit).
 struct TriS {
     float n[9];
     int32_t mat;
+    float uv[6];
 };
-static_assert(sizeof(TriS) == 40, "TriS");
+static_assert(sizeof(TriS) == 64, "TriS");
 
 struct DevLights {
     float sun_dir[3];
@@ -64,6 +66,10 @@ struct DevScene {
     TriV* triv = nullptr;
     TriS* tris = nullptr;
     float* mat_albedo = nullptr;      // 3 per material
+    uint32_t* tex_texels = nullptr;   // RGBA8 texels of every loaded texture, one after another
+    int4* tex_desc = nullptr;         // per material: texel offset, width, height, 1 if textured
+    float* srgb_lut = nullptr;        // 8-bit sRGB -> linear
+    int32_t tex_loaded = 0, tex_missing = 0;
     lgi_point_light* points = nullptr;
     int32_t n_points = 0;
     int32_t* perm = nullptr;          // sorted position -> original index
@@ -92,6 +98,10 @@ struct lgi_ctx {
     void* cap_stream = nullptr;  // hipStream_t used for graph capture
     void* graph_exec = nullptr;  // hipGraphExec_t of the last captured batch launch
     std::string graph_key;       // inputs the captured graph was built for
+    std::string asset_root;      // texture paths are relative to this directory
+    bool asset_root_set = false; // set explicitly (lgi_set_asset_root)
+    std::string scene_root;      // default root: the loaded scene file's grandparent directory
+    bool tex_dirty = false;      // asset root changed after the upload
     // scratch buffers (grow-only)
     std::vector<std::pair<void*, size_t>> scratch;
     void* scratch_get(int slot, size_t bytes);

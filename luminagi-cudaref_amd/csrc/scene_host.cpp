@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 HostScene::HostScene() {
@@ -832,6 +833,113 @@ lgi_status write_png_rgb(const char* path, const uint8_t* rgb, int w, int h) {
     bool ok = std::fwrite(o.data(), 1, o.size(), f) == o.size();
     ok = (std::fclose(f) == 0) && ok;
     return ok ? LGI_OK : LGI_ERR_IO;
+}
+
+// ===================================================================== PNG reader
+// Texture images (the reference loads them with stbi_load(..., 4), src/pathtracer.cu:553-561):
+// non-interlaced PNG, bit depth 8 or 16 (16-bit samples keep their high byte), grey, grey +
+// alpha, RGB, RGBA or palette, decoded to RGBA8 rows top to bottom.
+namespace {
+
+uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+int paeth(int a, int b, int c) {
+    const int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    if (pa <= pb && pa <= pc) return a;
+    return pb <= pc ? b : c;
+}
+
+}  // namespace
+
+lgi_status read_png_rgba(const char* path, int& w, int& h, std::vector<uint8_t>& rgba, std::string& err) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) { err = std::string("cannot open ") + path; return LGI_ERR_IO; }
+    std::vector<uint8_t> f((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    auto bad = [&](const char* what) { err = std::string(path) + ": " + what; return LGI_ERR_PARSE; };
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    if (f.size() < 8 || std::memcmp(f.data(), sig, 8) != 0) return bad("not a PNG file");
+    int depth = 0, ctype = -1;
+    bool have_hdr = false;
+    std::vector<uint8_t> idat, pal;
+    size_t pos = 8;
+    while (pos + 12 <= f.size()) {
+        const uint32_t len = be32(&f[pos]);
+        if (len > f.size() - pos - 12) return bad("truncated chunk");
+        const char* type = (const char*)&f[pos + 4];
+        const uint8_t* d = &f[pos + 8];
+        if (!std::memcmp(type, "IHDR", 4)) {
+            if (len < 13) return bad("short IHDR");
+            w = (int)be32(d);
+            h = (int)be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            if (d[12] != 0) return bad("interlaced PNG not supported");
+            have_hdr = true;
+        } else if (!std::memcmp(type, "PLTE", 4)) {
+            pal.assign(d, d + len);
+        } else if (!std::memcmp(type, "IDAT", 4)) {
+            idat.insert(idat.end(), d, d + len);
+        } else if (!std::memcmp(type, "IEND", 4)) {
+            break;
+        }
+        pos += 12 + len;
+    }
+    if (!have_hdr || w <= 0 || h <= 0 || w > (1 << 15) || h > (1 << 15)) return bad("bad or missing IHDR");
+    int ch = 0;
+    switch (ctype) {
+        case 0: ch = 1; break;
+        case 2: ch = 3; break;
+        case 3: ch = 1; break;
+        case 4: ch = 2; break;
+        case 6: ch = 4; break;
+        default: return bad("unsupported colour type");
+    }
+    if (!(depth == 8 || (depth == 16 && ctype != 3))) return bad("unsupported bit depth");
+    if (ctype == 3 && pal.size() < 3) return bad("palette image without PLTE");
+    const size_t bpp = (size_t)ch * depth / 8, stride = bpp * (size_t)w;
+    std::vector<uint8_t> raw(((size_t)stride + 1) * h);
+    uLongf rlen = raw.size();
+    if (uncompress(raw.data(), &rlen, idat.data(), idat.size()) != Z_OK || rlen != raw.size())
+        return bad("corrupt image data");
+    std::vector<uint8_t> img(stride * h), zero(stride, 0);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t ft = raw[(size_t)y * (stride + 1)];
+        const uint8_t* src = &raw[(size_t)y * (stride + 1) + 1];
+        uint8_t* dst = &img[(size_t)y * stride];
+        const uint8_t* up = y ? &img[(size_t)(y - 1) * stride] : zero.data();
+        for (size_t x = 0; x < stride; ++x) {
+            const int a = x >= bpp ? dst[x - bpp] : 0, b = up[x], c = x >= bpp ? up[x - bpp] : 0;
+            int v = src[x];
+            switch (ft) {
+                case 0: break;
+                case 1: v += a; break;
+                case 2: v += b; break;
+                case 3: v += (a + b) >> 1; break;
+                case 4: v += paeth(a, b, c); break;
+                default: return bad("bad filter type");
+            }
+            dst[x] = (uint8_t)v;
+        }
+    }
+    rgba.assign((size_t)w * h * 4, 255);
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        uint8_t s8[4];
+        for (int k = 0; k < ch; ++k) s8[k] = img[i * bpp + (size_t)k * depth / 8];  // high byte of 16-bit
+        uint8_t* o = &rgba[i * 4];
+        switch (ctype) {
+            case 0: o[0] = o[1] = o[2] = s8[0]; break;
+            case 4: o[0] = o[1] = o[2] = s8[0]; o[3] = s8[1]; break;
+            case 2: o[0] = s8[0]; o[1] = s8[1]; o[2] = s8[2]; break;
+            case 6: o[0] = s8[0]; o[1] = s8[1]; o[2] = s8[2]; o[3] = s8[3]; break;
+            case 3: {
+                const size_t e = (size_t)s8[0] * 3;
+                if (e + 2 >= pal.size()) return bad("palette index out of range");
+                o[0] = pal[e]; o[1] = pal[e + 1]; o[2] = pal[e + 2];
+                break;
+            }
+        }
+    }
+    return LGI_OK;
 }
 
 // ===================================================================== batch

@@ -44,4 +44,6 @@ lgi_status scene_generate(int kind, uint64_t seed, int width, int height, HostSc
 void camera_look_at(lgi_camera& cam, const float pos[3], const float tgt[3]);
 
 lgi_status write_png_rgb(const char* path, const uint8_t* rgb, int w, int h);
+// PNG -> RGBA8 (non-interlaced; depth 8 or 16; grey, grey+alpha, RGB, RGBA, palette).
+lgi_status read_png_rgba(const char* path, int& w, int& h, std::vector<uint8_t>& rgba, std::string& err);
 lgi_status parse_batch_file(const char* path, std::vector<lgi_batch_job>& jobs, std::string& err);

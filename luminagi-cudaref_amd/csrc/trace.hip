@@ -110,10 +110,11 @@ __device__ V3 trace_path(const TraceParams& P, Ray ray, Rng& rng, const TravStac
         Hit h = traverse<false>(P, ray, 1e30f, stk);
         if (h.tri < 0) break;
         int mat;
-        V3 N = hit_normal(P.tris, h, mat);
+        float tu, tv;
+        V3 N = hit_normal(P.tris, h, mat, tu, tv);
         if (dot(N, ray.d) > 0.0f) N = mul(N, -1.0f);
         V3 Pt = add(ray.o, mul(ray.d, h.t));
-        V3 albedo = v3(P.mat_albedo[3 * mat], P.mat_albedo[3 * mat + 1], P.mat_albedo[3 * mat + 2]);
+        V3 albedo = sample_albedo(P, mat, tu, tv);
         L = add(L, had(T, shade_direct(P, Pt, N, albedo, stk)));
         if (bounce == P.bounces) break;
         if (bounce >= 1 && rr) {
@@ -239,12 +240,14 @@ __global__ __launch_bounds__(kBlock) void k_gbuffer(TraceParams P, float3 bbMin,
         return;
     }
     int mat;
-    V3 N = hit_normal(P.tris, h, mat);
+    float tu, tv;
+    V3 N = hit_normal(P.tris, h, mat, tu, tv);
     if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
     auto enc = [](float v) { return (uint8_t)(fminf(1.0f, fmaxf(0.0f, v)) * 255.0f); };
-    alb[i] = enc(P.mat_albedo[3 * mat]);
-    alb[i + 1] = enc(P.mat_albedo[3 * mat + 1]);
-    alb[i + 2] = enc(P.mat_albedo[3 * mat + 2]);
+    const V3 a = sample_albedo(P, mat, tu, tv);
+    alb[i] = enc(a.x);
+    alb[i + 1] = enc(a.y);
+    alb[i + 2] = enc(a.z);
     nrm[i] = enc(N.x * 0.5f + 0.5f);
     nrm[i + 1] = enc(N.y * 0.5f + 0.5f);
     nrm[i + 2] = enc(N.z * 0.5f + 0.5f);

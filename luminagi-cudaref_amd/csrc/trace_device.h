@@ -76,11 +76,13 @@ struct TravStackT {
 };
 using TravStack = TravStackT<kStackLds>;
 
-// Shading normal at the closest hit (src/pathtracer.cu:149-155).
-LGI_D V3 hit_normal(const TriS* tris, const Hit& h, int& mat) {
+// Shading normal and texture coordinates at the closest hit (src/pathtracer.cu:149-157).
+LGI_D V3 hit_normal(const TriS* tris, const Hit& h, int& mat, float& tu, float& tv) {
     const TriS s = tris[h.tri];
     float w = 1.0f - h.u - h.v;
     mat = s.mat;
+    tu = w * s.uv[0] + h.u * s.uv[2] + h.v * s.uv[4];
+    tv = w * s.uv[1] + h.u * s.uv[3] + h.v * s.uv[5];
     return normalize(v3(w * s.n[0] + h.u * s.n[3] + h.v * s.n[6], w * s.n[1] + h.u * s.n[4] + h.v * s.n[7],
                         w * s.n[2] + h.u * s.n[5] + h.v * s.n[8]));
 }
@@ -103,6 +105,46 @@ LGI_D Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
 // the shadow ray, its tmax and the contribution albedo * (color * I) to add if unoccluded.
 // A point light at distance >= radius has attenuation exactly 0, so it is reported as not
 // contributing (its shadow ray would only ever add +0).
+// Diffuse texture lookup with the semantics of the reference's CUDA texture object
+// (src/pathtracer.cu:274-282, 563-576): normalised coordinates (u, 1 - v), wrap addressing,
+// RGBA8 texels converted from sRGB to linear before filtering, bilinear filtering with the
+// texture unit's 8-bit fractional weights (1/256 steps, rounded to nearest).
+LGI_D float tex_weight(float f) { return floorf(f * 256.0f + 0.5f) * (1.0f / 256.0f); }
+
+LGI_D V3 tex_sample(const uint32_t* tex, int W, int H, const float* lut, float tu, float tv) {
+    float u = tu - floorf(tu);
+    float v = (1.0f - tv) - floorf(1.0f - tv);
+    if (!(u >= 0.0f && u < 1.0f)) u = 0.0f;  // NaN or infinite coordinates
+    if (!(v >= 0.0f && v < 1.0f)) v = 0.0f;
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float xf = floorf(x), yf = floorf(y);
+    const float a = tex_weight(x - xf), b = tex_weight(y - yf);
+    int x0 = (int)xf, y0 = (int)yf;
+    x0 = x0 < 0 ? x0 + W : x0;
+    y0 = y0 < 0 ? y0 + H : y0;
+    x0 = min(max(x0, 0), W - 1);
+    y0 = min(max(y0, 0), H - 1);
+    const int x1 = x0 + 1 == W ? 0 : x0 + 1, y1 = y0 + 1 == H ? 0 : y0 + 1;
+    auto texel = [&](int xx, int yy) {
+        const uint32_t p = tex[(size_t)yy * W + xx];
+        return v3(lut[p & 255u], lut[(p >> 8) & 255u], lut[(p >> 16) & 255u]);
+    };
+    const V3 t00 = texel(x0, y0), t10 = texel(x1, y0), t01 = texel(x0, y1), t11 = texel(x1, y1);
+    const V3 top = add(mul(t00, 1.0f - a), mul(t10, a));
+    const V3 bot = add(mul(t01, 1.0f - a), mul(t11, a));
+    return add(mul(top, 1.0f - b), mul(bot, b));
+}
+
+// sample_albedo (src/pathtracer.cu:274-282): the texture when the material has one, else the
+// constant albedo.
+LGI_D V3 sample_albedo(const TraceParams& P, int mat, float tu, float tv) {
+    if (P.tex_desc) {
+        const int4 d = P.tex_desc[mat];
+        if (d.w) return tex_sample(P.tex_texels + d.x, d.y, d.z, P.srgb_lut, tu, tv);
+    }
+    return v3(P.mat_albedo[3 * mat], P.mat_albedo[3 * mat + 1], P.mat_albedo[3 * mat + 2]);
+}
+
 // Direction to light li from the shading point (li 0 = sun, then the point lights) and, for a
 // point light, its distance (src/pathtracer.cu:236-262).
 LGI_D void light_dir(const TraceParams& P, int li, V3 Pt, V3& toLight, float& dist) {

@@ -7,6 +7,9 @@ struct TraceParams {
     const TriV* triv;
     const TriS* tris;
     const float* mat_albedo;
+    const uint32_t* tex_texels;  // RGBA8 texels (null when no material is textured)
+    const int4* tex_desc;        // per material: texel offset, width, height, textured
+    const float* srgb_lut;       // 256 entries
     const lgi_point_light* points;
     int32_t n_points;
     float sun_dir[3];

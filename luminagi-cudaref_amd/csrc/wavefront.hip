@@ -337,10 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     if (live) {
         const V3 d = v3(e.d.x, e.d.y, e.d.z);
         int mat;
-        N = hit_normal(W.P.tris, h, mat);
+        float tu, tv;
+        N = hit_normal(W.P.tris, h, mat, tu, tv);
         if (dot(N, d) > 0.0f) N = mul(N, -1.0f);
         Pt = add(v3(e.o.x, e.o.y, e.o.z), mul(d, h.t));
-        albedo = v3(W.P.mat_albedo[3 * mat], W.P.mat_albedo[3 * mat + 1], W.P.mat_albedo[3 * mat + 2]);
+        albedo = sample_albedo(W.P, mat, tu, tv);
         const float* T = W.T + 3 * (int64_t)p;
         float* Tb = W.Tb + 3 * (int64_t)p;
         Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];

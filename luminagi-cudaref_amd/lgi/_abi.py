@@ -99,6 +99,8 @@ SIGNATURES = {
                                     _P, _P, _P]),
     "lgi_batch_parse": (C.c_int, [C.c_char_p, C.POINTER(BatchJob), C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_write_png": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
+    "lgi_read_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P]),
+    "lgi_set_asset_root": (C.c_int, [_P, C.c_char_p]),
     "lgi_image_metrics": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, _P]),
     # denoiser layers (device pointers + hipStream_t)
     "lgi_unet_conv3x3_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32,

@@ -36,6 +36,15 @@ def parse_batch(path):
     return [(j.out_png.decode(), j.spp, j.bounces, tuple(j.cam_pos), tuple(j.cam_tgt)) for j in jobs[:n.value]]
 
 
+def read_png(path):
+    """PNG -> HxWx4 uint8 RGBA (the texture loader's decoder)."""
+    w, h = C.c_int32(), C.c_int32()
+    _check(None, lib().lgi_read_png(str(path).encode(), C.byref(w), C.byref(h), None))
+    out = np.zeros((h.value, w.value, 4), np.uint8)
+    _check(None, lib().lgi_read_png(str(path).encode(), C.byref(w), C.byref(h), ptr(out)))
+    return out
+
+
 def write_png(path, rgb):
     rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
     h, w = rgb.shape[:2]
@@ -125,6 +134,10 @@ class Context:
         cam = Camera()
         self._c(lib().lgi_scene_get(self._h, C.byref(cam), None, None, None, None))
         return cam
+
+    def set_asset_root(self, path):
+        """Directory the materials' texture paths are relative to (None: the scene's grandparent)."""
+        self._c(lib().lgi_set_asset_root(self._h, None if path is None else str(path).encode()))
 
     def set_camera(self, cam):
         self._c(lib().lgi_scene_set_camera(self._h, C.byref(cam)))

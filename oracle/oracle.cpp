@@ -187,6 +187,14 @@ struct Counters {
     }
 };
 
+// Diffuse textures: RGBA8 texels of all textures one after another, per material
+// {texel offset, width, height, textured}, and the 8-bit sRGB -> linear table.
+struct TexView {
+    const uint32_t* texels = nullptr;
+    const int32_t* desc = nullptr;
+    const float* srgb = nullptr;
+};
+
 struct SceneView {
     const ONode* nodes;
     const OTri* tris;            // permuted (leaf) order
@@ -194,6 +202,7 @@ struct SceneView {
     const OPoint* points;
     int numPoints;
     OSun sun;
+    TexView tex;
 };
 
 struct CamView {
@@ -243,7 +252,7 @@ inline bool ray_tri(const Ray& r, const OTri& t, float tmax, float& outT, float&
 
 // src/pathtracer.cu:109-160. Stack of 64, push left then right (right popped first).
 inline int traverse_closest(const SceneView& s, const Ray& r, float tmax, float& outT, V3& outN,
-                            Counters& c) {
+                            float& outTU, float& outTV, Counters& c) {
     int stack[64];
     int sp = 0;
     stack[sp++] = 0;
@@ -278,6 +287,8 @@ inline int traverse_closest(const SceneView& s, const Ray& r, float tmax, float&
                         w * t.n[0][1] + bestU * t.n[1][1] + bestV * t.n[2][1],
                         w * t.n[0][2] + bestU * t.n[1][2] + bestV * t.n[2][2]));
     outT = bestT;
+    outTU = w * t.uv[0][0] + bestU * t.uv[1][0] + bestV * t.uv[2][0];   // src/pathtracer.cu:156-157
+    outTV = w * t.uv[0][1] + bestU * t.uv[1][1] + bestV * t.uv[2][1];
     return bestTri;
 }
 
@@ -375,6 +386,38 @@ inline Ray primary_ray(const CamView& cam, int px, int py, float jx, float jy) {
     return make_ray(cam.pos, dirWorld);
 }
 
+// sample_albedo, src/pathtracer.cu:274-282, with the texture object of :563-576: normalised
+// coordinates (u, 1 - v) wrapped into [0, 1), texel centres at (i + 0.5) / size, RGBA8 texels
+// converted from sRGB to linear before bilinear filtering, and filter weights in 1/256 steps
+// (rounded to nearest), as a texture unit with 8-bit fractional weights computes them.
+inline float tex_weight(float f) { return std::floor(f * 256.0f + 0.5f) * (1.0f / 256.0f); }
+
+inline V3 sample_albedo(const SceneView& s, int mat, float tu, float tv) {
+    if (!s.tex.desc || !s.tex.desc[4 * mat + 3]) return vfrom(&s.matAlbedo[3 * mat]);
+    const uint32_t* tex = s.tex.texels + s.tex.desc[4 * mat];
+    const int W = s.tex.desc[4 * mat + 1], H = s.tex.desc[4 * mat + 2];
+    float u = tu - std::floor(tu);
+    float v = (1.0f - tv) - std::floor(1.0f - tv);
+    if (!(u >= 0.0f && u < 1.0f)) u = 0.0f;
+    if (!(v >= 0.0f && v < 1.0f)) v = 0.0f;
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float xf = std::floor(x), yf = std::floor(y);
+    const float a = tex_weight(x - xf), b = tex_weight(y - yf);
+    int x0 = (int)xf, y0 = (int)yf;
+    if (x0 < 0) x0 += W;
+    if (y0 < 0) y0 += H;
+    x0 = std::min(std::max(x0, 0), W - 1);
+    y0 = std::min(std::max(y0, 0), H - 1);
+    const int x1 = x0 + 1 == W ? 0 : x0 + 1, y1 = y0 + 1 == H ? 0 : y0 + 1;
+    auto texel = [&](int xx, int yy) {
+        const uint32_t p = tex[(size_t)yy * W + xx];
+        return v3(s.tex.srgb[p & 255u], s.tex.srgb[(p >> 8) & 255u], s.tex.srgb[(p >> 16) & 255u]);
+    };
+    const V3 top = add(mul(texel(x0, y0), 1.0f - a), mul(texel(x1, y0), a));
+    const V3 bot = add(mul(texel(x0, y1), 1.0f - a), mul(texel(x1, y1), a));
+    return add(mul(top, 1.0f - b), mul(bot, b));
+}
+
 // src/pathtracer.cu:295-331. no_rr reproduces the sorted path's semantics (:486-547).
 inline V3 trace_path(const SceneView& s, Ray ray, int maxBounces, Rng& rng, bool no_rr, Counters& c) {
     V3 L = v3(0, 0, 0);
@@ -382,11 +425,12 @@ inline V3 trace_path(const SceneView& s, Ray ray, int maxBounces, Rng& rng, bool
     for (int bounce = 0; bounce <= maxBounces; ++bounce) {
         float hitT;
         V3 N;
-        int tri = traverse_closest(s, ray, 1e30f, hitT, N, c);
+        float tu, tv;
+        int tri = traverse_closest(s, ray, 1e30f, hitT, N, tu, tv, c);
         if (tri < 0) break;
         if (dot(N, ray.d) > 0.0f) N = mul(N, -1.0f);
         V3 P = add(ray.o, mul(ray.d, hitT));
-        V3 albedo = vfrom(&s.matAlbedo[3 * s.tris[tri].material]);
+        V3 albedo = sample_albedo(s, s.tris[tri].material, tu, tv);
         L = add(L, had(throughput, shade_direct(s, P, N, albedo, c)));
         if (bounce == maxBounces) break;
         if (bounce >= 1 && !no_rr) {
@@ -645,6 +689,9 @@ struct OrcRenderArgs {
     int32_t sample0;
     int32_t row_step;          // render rows y with y % row_step == 0 (1 = full frame)
     int32_t threads;
+    const uint32_t* tex_texels;  // nullable: diffuse textures (see TexView)
+    const int32_t* tex_desc;     // 4 per material
+    const float* srgb_lut;       // 256
 };
 
 // Full render. out_sum/out_comp: W*H*3 Kahan FP32 (only sampled rows written);
@@ -653,7 +700,8 @@ struct OrcRenderArgs {
 // rays, shadow_rays (nullable, uint64[5]).
 void orc_render(const OrcRenderArgs* a, float* out_sum, float* out_comp, double* out_sum64,
                 float* out_sumsq, uint64_t* counters) {
-    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun};
+    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun,
+                TexView{a->tex_texels, a->tex_desc, a->srgb_lut}};
     CamView cam = make_cam(a->cam, orc_tan_half_fov(a->cam.fovYDeg));
     const int W = a->cam.width, H = a->cam.height;
     const bool xs_ref = (a->flags & 1u) != 0;
@@ -719,7 +767,8 @@ void orc_render(const OrcRenderArgs* a, float* out_sum, float* out_comp, double*
 // Primary hit at the pixel centre (src/pathtracer.cu:381-384): triangle index in the
 // permuted order (-1 = miss) and hit distance.
 void orc_primary_hits(const OrcRenderArgs* a, int32_t* tri_out, float* t_out) {
-    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun};
+    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun,
+                TexView{a->tex_texels, a->tex_desc, a->srgb_lut}};
     CamView cam = make_cam(a->cam, orc_tan_half_fov(a->cam.fovYDeg));
     const int W = a->cam.width, H = a->cam.height;
     int nthreads = a->threads > 0 ? a->threads : (int)std::max(1u, std::thread::hardware_concurrency());
@@ -733,7 +782,8 @@ void orc_primary_hits(const OrcRenderArgs* a, int32_t* tri_out, float* t_out) {
                 Ray r = primary_ray(cam, px, py, 0.5f, 0.5f);
                 float t = 0.0f;
                 V3 N;
-                int tri = traverse_closest(s, r, 1e30f, t, N, c);
+                float tu, tv;
+                int tri = traverse_closest(s, r, 1e30f, t, N, tu, tv, c);
                 tri_out[(size_t)py * W + px] = tri;
                 t_out[(size_t)py * W + px] = tri < 0 ? -1.0f : t;
             }
@@ -764,14 +814,24 @@ int32_t orc_bvh_closest(const ONode* nodes, const OTri* tris, const float o[3], 
     Counters c;
     V3 N;
     float t = 1e30f;
-    int32_t tri = traverse_closest(s, r, 1e30f, t, N, c);
+    float tu, tv;
+    int32_t tri = traverse_closest(s, r, 1e30f, t, N, tu, tv, c);
     *t_out = t;
     return tri;
 }
 
+// sample_albedo of material `mat` at texture coordinates (u, v), for direct tests.
+void orc_tex_sample(const uint32_t* texels, const int32_t* desc, const float* srgb, const float* mat_albedo,
+                    int32_t mat, float u, float v, float* out3) {
+    SceneView s{nullptr, nullptr, mat_albedo, nullptr, 0, {}, TexView{texels, desc, srgb}};
+    const V3 a = sample_albedo(s, mat, u, v);
+    out3[0] = a.x; out3[1] = a.y; out3[2] = a.z;
+}
+
 // gbuffer_kernel, src/pathtracer.cu:366-411 and host extents :728-734.
 void orc_gbuffer(const OrcRenderArgs* a, uint8_t* alb, uint8_t* nrm, uint8_t* wp) {
-    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun};
+    SceneView s{a->nodes, a->tris, a->mat_albedo, a->points, a->n_points, a->sun,
+                TexView{a->tex_texels, a->tex_desc, a->srgb_lut}};
     CamView cam = make_cam(a->cam, orc_tan_half_fov(a->cam.fovYDeg));
     const ONode& root = a->nodes[0];
     V3 bbMin = vfrom(root.bbMin), bbMax = vfrom(root.bbMax);
@@ -784,7 +844,8 @@ void orc_gbuffer(const OrcRenderArgs* a, uint8_t* alb, uint8_t* nrm, uint8_t* wp
             Ray r = primary_ray(cam, px, py, 0.5f, 0.5f);
             float t;
             V3 N;
-            int tri = traverse_closest(s, r, 1e30f, t, N, c);
+            float tu, tv;
+            int tri = traverse_closest(s, r, 1e30f, t, N, tu, tv, c);
             size_t i = ((size_t)py * cam.width + px) * 3;
             if (tri < 0) {
                 alb[i] = alb[i + 1] = alb[i + 2] = 0;
@@ -793,8 +854,8 @@ void orc_gbuffer(const OrcRenderArgs* a, uint8_t* alb, uint8_t* nrm, uint8_t* wp
                 continue;
             }
             if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
-            const float* A = &s.matAlbedo[3 * s.tris[tri].material];
-            alb[i] = enc(A[0]); alb[i + 1] = enc(A[1]); alb[i + 2] = enc(A[2]);
+            const V3 A = sample_albedo(s, s.tris[tri].material, tu, tv);
+            alb[i] = enc(A.x); alb[i + 1] = enc(A.y); alb[i + 2] = enc(A.z);
             nrm[i] = enc(N.x * 0.5f + 0.5f); nrm[i + 1] = enc(N.y * 0.5f + 0.5f); nrm[i + 2] = enc(N.z * 0.5f + 0.5f);
             V3 P = add(r.o, mul(r.d, t));
             V3 Pn = had(sub(P, bbMin), invExt);

@@ -34,7 +34,27 @@ class RenderArgs(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("tris", C.c_void_p), ("mat_albedo", C.c_void_p), ("points", C.c_void_p),
                 ("n_points", C.c_int32), ("sun", OSun), ("cam", OCam), ("spp", C.c_int32), ("bounces", C.c_int32),
                 ("flags", C.c_uint32), ("seed", C.c_uint32), ("sample0", C.c_int32), ("row_step", C.c_int32),
-                ("threads", C.c_int32)]
+                ("threads", C.c_int32), ("tex_texels", C.c_void_p), ("tex_desc", C.c_void_p), ("srgb_lut", C.c_void_p)]
+
+
+def srgb_lut():
+    """8-bit sRGB -> linear, evaluated in FP64 and rounded to FP32 (the IEC 61966-2-1 curve)."""
+    c = np.arange(256) / 255.0
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4).astype(np.float32)
+
+
+def pack_textures(textures, n_mats):
+    """textures: {material index: HxWx4 uint8 RGBA} -> (texels u32, desc int32 [n_mats, 4])."""
+    desc = np.zeros((max(1, n_mats), 4), np.int32)
+    chunks, off = [], 0
+    for m, img in sorted(textures.items()):
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape[:2]
+        desc[m] = (off, w, h, 1)
+        chunks.append(img.reshape(-1, 4).view("<u4").reshape(-1))
+        off += w * h
+    texels = np.concatenate(chunks) if chunks else np.zeros(1, np.uint32)
+    return np.ascontiguousarray(texels, np.uint32), desc
 
 
 _lib = None
@@ -49,6 +69,7 @@ def lib():
         V = C.c_void_p
         sig = {
             "orc_morton3": (C.c_uint32, [C.c_float] * 3),
+            "orc_tex_sample": (None, [V, V, V, V, C.c_int32, C.c_float, C.c_float, V]),
             "orc_spread3": (C.c_uint32, [C.c_uint32]),
             "orc_sincos": (None, [V, V, V, C.c_int64]),
             "orc_pcg32_stream": (None, [C.c_uint64, C.c_uint64, V, C.c_int]),
@@ -152,11 +173,25 @@ def camera_look_at(cam, pos, tgt):
     return cam
 
 
+def tex_sample(textures, mat_albedo, mat, u, v):
+    """sample_albedo of one material at (u, v); textures as for Scene."""
+    mat_albedo = np.ascontiguousarray(mat_albedo, np.float32).reshape(-1, 3)
+    texels, desc = pack_textures(textures, len(mat_albedo))
+    lut = srgb_lut()
+    out = np.zeros(3, np.float32)
+    lib().orc_tex_sample(_p(texels), _p(desc), _p(lut), _p(mat_albedo), mat, u, v, _p(out))
+    return out
+
+
 class Scene:
     """Oracle-side scene in BVH order: nodes + permuted triangles + lights + camera."""
 
-    def __init__(self, tris, mat_albedo, points, sun, camera):
+    def __init__(self, tris, mat_albedo, points, sun, camera, textures=None):
+        """textures (optional): {material index: HxWx4 uint8 RGBA} diffuse textures."""
         self.nodes, self.perm, self.keys = bvh_build(tris)
+        self.tex = None
+        if textures:
+            self.tex = pack_textures(textures, len(np.asarray(mat_albedo).reshape(-1, 3))) + (srgb_lut(),)
         self.tris = np.ascontiguousarray(np.asarray(tris, TRI_DTYPE)[self.perm])
         self.mat_albedo = np.ascontiguousarray(mat_albedo, np.float32).reshape(-1, 3)
         self.points = np.ascontiguousarray(points, POINT_DTYPE)
@@ -172,6 +207,8 @@ class Scene:
         a.nodes, a.tris, a.mat_albedo, a.points = _p(self.nodes), _p(self.tris), _p(self.mat_albedo), _p(self.points)
         a.n_points = len(self.points)
         a.sun = self.sun
+        if self.tex is not None:
+            a.tex_texels, a.tex_desc, a.srgb_lut = _p(self.tex[0]), _p(self.tex[1]), _p(self.tex[2])
         if camera is not None:
             C.memmove(C.addressof(a.cam), C.addressof(camera), C.sizeof(OCam))
         else:
