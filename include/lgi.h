@@ -214,6 +214,13 @@ LGI_API lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam, const i
  * {MSE, SSIM (7x7 uniform window, sample covariance, K1 .01, K2 .03, range 255), mean |a-b|}. */
 LGI_API lgi_status lgi_image_metrics(const uint8_t* d_a, const uint8_t* d_b, int32_t h, int32_t w,
                                      double* d_scratch, double* d_out, void* stream);
+/* Difference heatmap (heatmap, tools/diff.py:41-45): per pixel, the mean absolute channel
+ * difference (FP64) times gain, clipped to [0, 255] and truncated, indexes the colour table of
+ * lgi_heat_lut. d_a, d_b, d_out: h*w*3 u8 RGB on the device. Asynchronous on stream. */
+LGI_API lgi_status lgi_diff_heatmap(const uint8_t* d_a, const uint8_t* d_b, int32_t h, int32_t w, double gain,
+                                    uint8_t* d_out, void* stream);
+/* The heatmap's 256-entry RGB colour table (viridis_lut, tools/diff.py:32-38), host only. */
+LGI_API lgi_status lgi_heat_lut(uint8_t* rgb768);
 
 /* ---- batch (read_batch_file, src/main.cpp:61-91) ---- */
 LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs);

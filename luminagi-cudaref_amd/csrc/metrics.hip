@@ -88,7 +88,63 @@ __global__ void k_final(const double* perr, const double* pssim, int nb, int64_t
     out[2] = ab / (double)n;
 }
 
+// Heat colour table held in LDS; one pixel per thread.
+struct HeatLut {
+    uint8_t rgb[768];
+};
+
+__global__ __launch_bounds__(kBlock) void k_heatmap(const uint8_t* a, const uint8_t* b, int64_t npx, double gain,
+                                                    HeatLut lut, uint8_t* out) {
+    __shared__ uint8_t t[768];
+    for (int i = threadIdx.x; i < 768; i += kBlock) t[i] = lut.rgb[i];
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < npx; p += (int64_t)gridDim.x * kBlock) {
+        const uint8_t* pa = a + 3 * p;
+        const uint8_t* pb = b + 3 * p;
+        const int s = abs((int)pa[0] - (int)pb[0]) + abs((int)pa[1] - (int)pb[1]) + abs((int)pa[2] - (int)pb[2]);
+        // numpy: mean over channels in float64 (sum / 3), times gain, clip to [0, 255], truncate
+        double v = (double)s / 3.0 * gain;
+        v = v > 0.0 ? (v < 255.0 ? v : 255.0) : 0.0;
+        const int k = 3 * (int)v;
+        out[3 * p] = t[k];
+        out[3 * p + 1] = t[k + 1];
+        out[3 * p + 2] = t[k + 2];
+    }
+}
+
 }  // namespace
+
+/* The heatmap's colour table (tools/diff.py:32-38): for x = i * (1/255) (x = 1 at i = 255, as
+ * numpy.linspace(0, 1, 256) spaces it), three quadratics clipped to [0, 1], times 255,
+ * truncated. Evaluated left to right in double precision as numpy does, no contraction. */
+extern "C" lgi_status lgi_heat_lut(uint8_t* rgb768) {
+    if (!rgb768) return LGI_ERR_INVALID_ARG;
+    const double step = 1.0 / 255.0;
+    auto chan = [](double v) {
+        v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+        return (uint8_t)(v * 255.0);
+    };
+    for (int i = 0; i < 256; ++i) {
+        const double x = i == 255 ? 1.0 : (double)i * step;
+        const double x2 = x * x;
+        rgb768[3 * i] = chan(-0.2 + 2.4 * x - 1.3 * x2);
+        rgb768[3 * i + 1] = chan(0.1 + 0.8 * x + 0.1 * x2);
+        rgb768[3 * i + 2] = chan(0.4 + 0.8 * x - 1.1 * x2);
+    }
+    return LGI_OK;
+}
+
+extern "C" lgi_status lgi_diff_heatmap(const uint8_t* d_a, const uint8_t* d_b, int32_t h, int32_t w, double gain,
+                                       uint8_t* d_out, void* stream) {
+    if (!d_a || !d_b || !d_out || h <= 0 || w <= 0 || !(gain == gain)) return LGI_ERR_INVALID_ARG;
+    HeatLut lut;
+    lgi_heat_lut(lut.rgb);
+    const int64_t npx = (int64_t)h * w;
+    const int64_t blocks = (npx + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_heatmap, dim3((unsigned)(blocks < 4096 ? blocks : 4096)), dim3(kBlock), 0,
+                       (hipStream_t)stream, d_a, d_b, npx, gain, lut, d_out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
 
 extern "C" lgi_status lgi_image_metrics(const uint8_t* d_a, const uint8_t* d_b, int32_t h, int32_t w, double* d_scratch,
                                         double* d_out, void* stream) {

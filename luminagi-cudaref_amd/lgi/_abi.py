@@ -102,6 +102,8 @@ SIGNATURES = {
     "lgi_read_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P]),
     "lgi_set_asset_root": (C.c_int, [_P, C.c_char_p]),
     "lgi_image_metrics": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, _P]),
+    "lgi_diff_heatmap": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_double, _P, _P]),
+    "lgi_heat_lut": (C.c_int, [_P]),
     # denoiser layers (device pointers + hipStream_t)
     "lgi_unet_conv3x3_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, _P, _P]),

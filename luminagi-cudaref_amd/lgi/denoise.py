@@ -12,31 +12,13 @@ import torch
 
 from . import _abi
 from ._abi import LgiError
+from .metrics import image_metrics, psnr_from_mse  # noqa: F401  (re-exported)
 
 LUM = (0.2126, 0.7152, 0.0722)
 
 
 def _stream():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
-
-
-def image_metrics(a_u8, b_u8):
-    """(MSE, SSIM, MAD) of two HxWx3 uint8 CUDA tensors, reduced in FP64 on the GPU."""
-    assert a_u8.dtype == torch.uint8 and a_u8.shape == b_u8.shape and a_u8.is_cuda
-    h, w, _ = a_u8.shape
-    scratch = torch.empty(3 * 1024 + 8, dtype=torch.float64, device=a_u8.device)
-    out = torch.empty(3, dtype=torch.float64, device=a_u8.device)
-    st = _abi.lib().lgi_image_metrics(C.c_void_p(a_u8.contiguous().data_ptr()), C.c_void_p(b_u8.contiguous().data_ptr()),
-                                      h, w, C.c_void_p(scratch.data_ptr()), C.c_void_p(out.data_ptr()), _stream())
-    if st != 0:
-        raise LgiError(st, "lgi_image_metrics")
-    mse, ssim, mad = out.cpu().tolist()
-    return mse, ssim, mad
-
-
-def psnr_from_mse(mse):
-    import math
-    return float("inf") if mse == 0 else 20.0 * math.log10(255.0 / math.sqrt(mse))
 
 
 def tonemap(ctx, hdr):
@@ -81,6 +63,30 @@ def denoise(model, hdr):
     with torch.no_grad():
         y = model(torch.log1p(hdr.clamp_min(0)).unsqueeze(0).contiguous())
     return torch.expm1(y[0]).clamp_min(0)
+
+
+def radiance_input(png_u8, hdr_path=None):
+    """HxWx3 float32 CUDA radiance for the denoiser from an 8-bit tonemapped image: the raw FP32
+    HxWx3 sidecar that `cuda_ref --hdr` writes when one is given, otherwise the inverse of the
+    renderer's sqrt tonemap (values the PNG clipped at 1.0 stay clipped)."""
+    import numpy as np
+    H, W = png_u8.shape[:2]
+    if hdr_path:
+        hdr = np.fromfile(hdr_path, dtype=np.float32)
+        if hdr.size != H * W * 3:
+            raise ValueError(f"{hdr_path}: {hdr.size} floats, expected {H}x{W}x3 = {H * W * 3}")
+        return torch.from_numpy(hdr.reshape(H, W, 3)).cuda()
+    return (torch.tensor(np.asarray(png_u8, dtype=np.uint8)).cuda().float() / 255.0) ** 2
+
+
+def denoise_any_size(model, rad):
+    """denoise() of an HxWx3 radiance image of any size: edge-replicated to a multiple of 16
+    (four 2x pools), denoised and cropped back."""
+    import torch.nn.functional as F
+    H, W = rad.shape[:2]
+    ph, pw = (16 - H % 16) % 16, (16 - W % 16) % 16
+    x = F.pad(rad.permute(2, 0, 1).unsqueeze(0), (0, pw, 0, ph), mode="replicate")[0].permute(1, 2, 0)
+    return denoise(model, x.contiguous())[:H, :W].contiguous()
 
 
 class DenoiseGraph:

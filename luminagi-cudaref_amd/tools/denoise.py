@@ -27,7 +27,6 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     import torch
-    import torch.nn.functional as F
     from PIL import Image
 
     import lgi
@@ -38,24 +37,18 @@ def main(argv=None):
         sys.exit("denoise.py needs a GPU: the denoiser runs only on the HIP kernels")
     png = np.asarray(Image.open(args.noisy_png).convert("RGB"), dtype=np.uint8)
     H, W = png.shape[:2]
-    if args.hdr:
-        hdr = np.fromfile(args.hdr, dtype=np.float32)
-        if hdr.size != H * W * 3:
-            sys.exit(f"{args.hdr}: {hdr.size} floats, expected {H}x{W}x3 = {H * W * 3}")
-        rad = torch.from_numpy(hdr.reshape(H, W, 3)).cuda()
-    else:
-        rad = (torch.from_numpy(png).cuda().float() / 255.0) ** 2  # inverse of sqrt(clamp(x, 0, 1)) * 255
+    try:
+        rad = D.radiance_input(png, args.hdr)
+    except ValueError as e:
+        sys.exit(str(e))
     model = load_model(args.checkpoint)
-    ph, pw = (16 - H % 16) % 16, (16 - W % 16) % 16
-    x = F.pad(rad.permute(2, 0, 1).unsqueeze(0), (0, pw, 0, ph), mode="replicate")[0].permute(1, 2, 0)
-    D.denoise(model, x.contiguous())  # first call allocates; time the second
+    D.denoise_any_size(model, rad)  # first call allocates; time the second
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    y = D.denoise(model, x.contiguous())
+    y = D.denoise_any_size(model, rad)
     torch.cuda.synchronize()
     dt_ms = (time.perf_counter() - t0) * 1000.0
-    ctx = lgi.Context()
-    out = D.tonemap(ctx, y[:H, :W].contiguous()).cpu().numpy()
+    out = D.tonemap(lgi.Context(), y).cpu().numpy()
     Image.fromarray(out).save(args.out_png)
     print(f"inference: {dt_ms:.1f} ms ({W}x{H}, base=32)  wrote {args.out_png}")
 

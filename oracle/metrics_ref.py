@@ -7,6 +7,9 @@ ssim: the scikit-image default that tools/diff.py:100-101 calls
 fits (3-pixel border cropped), averaged over channels. scikit-image is not installed here; the
 restatement is pinned by the reference's published numbers for its eval PNGs.
 mad: mean absolute difference (tools/diff.py:102).
+heat_lut / heatmap: the difference heatmap of tools/diff.py:32-45 (a 256-entry colour table
+from three clipped quadratics; per pixel the channel-mean absolute difference times a gain,
+clipped to [0, 255] and truncated, indexes it).
 """
 import numpy as np
 
@@ -42,3 +45,20 @@ def ssim(a, b, data_range=255.0):
 
 def mad(a, b):
     return float(np.mean(np.abs(a.astype(np.int16) - b.astype(np.int16))))
+
+
+def heat_lut():
+    """tools/diff.py:32-38: x = linspace(0, 1, 256); r, g, b quadratics clipped to [0, 1]; x 255;
+    truncated to uint8."""
+    x = np.linspace(0.0, 1.0, 256)
+    r = np.clip(-0.2 + 2.4 * x - 1.3 * x ** 2, 0, 1)
+    g = np.clip(0.1 + 0.8 * x + 0.1 * x ** 2, 0, 1)
+    b = np.clip(0.4 + 0.8 * x - 1.1 * x ** 2, 0, 1)
+    return (np.stack([r, g, b], axis=1) * 255).astype(np.uint8)
+
+
+def heatmap(a, b, gain=4.0):
+    """tools/diff.py:41-45: channel mean of |a - b| (float64), times gain, clip, truncate, colour."""
+    d = np.mean(np.abs(a.astype(np.int16) - b.astype(np.int16)), axis=2)
+    d = np.clip(d * gain, 0, 255).astype(np.uint8)
+    return heat_lut()[d]

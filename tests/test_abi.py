@@ -12,6 +12,7 @@ import lgi
 from lgi import _abi
 
 GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+PKG = pathlib.Path(lgi.__file__).resolve().parent.parent
 
 
 def test_library_exports_every_header_symbol():
@@ -145,3 +146,14 @@ def test_cli_usage_and_load_failure(tmp_path):
     assert r.returncode == 1 and "usage" in r.stderr
     r = subprocess.run([str(cli), str(tmp_path / "missing.json")], capture_output=True, text=True)
     assert r.returncode in (2, 4)
+
+
+def test_validate_tool_argument_errors(tmp_path):
+    """validate.py stops with the reference's messages before any GPU work."""
+    import sys
+    tool = PKG / "tools" / "validate.py"
+    r = subprocess.run([sys.executable, str(tool), str(tmp_path / "none.json")], capture_output=True, text=True)
+    assert r.returncode == 1 and "no such file:" in r.stderr
+    (tmp_path / "s.json").write_text("{}")
+    r = subprocess.run([sys.executable, str(tool), str(tmp_path / "s.json")], capture_output=True, text=True)
+    assert r.returncode == 1 and "engine screenshot missing:" in r.stderr

@@ -110,3 +110,96 @@ def test_cli_reads_binary_scene(gpu, tmp_path):
                            timeout=120)
         assert r.returncode == 0, r.stderr
     assert np.array_equal(read_png(a), read_png(b))
+
+
+def load_tool(name):
+    spec = importlib.util.spec_from_file_location(f"lgi_tool_{name}", PKG / "tools" / f"{name}.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def fmt_line(a, b):
+    from oracle import metrics_ref as M
+    return f"PSNR {M.psnr(a, b):.2f} dB | SSIM {M.ssim(a, b):.4f} | MAD {M.mad(a, b):.2f}"
+
+
+def test_tools_diff_report(gpu, tmp_path, capsys):
+    """diff.py <engine> <ref> --out: the report line, the heatmap sidecar and the HTML page, each
+    against the oracle's restatement of the reference's metrics and heatmap."""
+    import lgi
+    from oracle import metrics_ref as M
+    ctx, _ = small_scene(tmp_path, w=120, h=72)
+    ctx.build_bvh()
+    eng, ref = ctx.render(2, 2)["rgb"], ctx.render(64, 2)["rgb"]
+    lgi.write_png(tmp_path / "eng.png", eng)
+    lgi.write_png(tmp_path / "ref.png", ref)
+    out = tmp_path / "rep" / "report.html"
+    load_tool("diff").main([str(tmp_path / "eng.png"), str(tmp_path / "ref.png"), "--out", str(out)])
+    lines = capsys.readouterr().out.splitlines()
+    assert lines[0] == fmt_line(eng, ref) and lines[1] == f"wrote {out}"
+    assert np.array_equal(read_png(tmp_path / "rep" / "report.heatmap.png"), M.heatmap(eng, ref))
+    page = out.read_text()
+    assert f"{M.psnr(eng, ref):.2f} dB" in page and f"{M.ssim(eng, ref):.4f}" in page
+    assert page.count("data:image/png;base64,") == 3 and "120x72" in page
+
+
+def test_tools_diff_resizes_engine(gpu, tmp_path, capsys):
+    from PIL import Image
+    from oracle import metrics_ref as M
+    rng = np.random.default_rng(9)
+    ref = rng.integers(0, 256, (48, 64, 3), dtype=np.uint8)
+    eng = rng.integers(0, 256, (30, 40, 3), dtype=np.uint8)
+    Image.fromarray(ref).save(tmp_path / "ref.png")
+    Image.fromarray(eng).save(tmp_path / "eng.png")
+    load_tool("diff").main([str(tmp_path / "eng.png"), str(tmp_path / "ref.png"), "--out", str(tmp_path / "r.html")])
+    cap = capsys.readouterr()
+    assert "resizing engine (40, 30) -> (64, 48)" in cap.err
+    resized = np.asarray(Image.fromarray(eng).resize((64, 48), Image.BILINEAR), dtype=np.uint8)
+    assert cap.out.splitlines()[0] == fmt_line(resized, ref)
+
+
+def test_tools_validate_flow(gpu, tmp_path, capsys):
+    """validate.py <scene.json>: cuda_ref renders the reference next to the engine screenshot
+    (<scene>.png), diff.py compares them; the report's numbers are the oracle's on those files."""
+    import lgi
+    ctx, scene = small_scene(tmp_path)
+    ctx.build_bvh()
+    lgi.write_png(scene.with_suffix(".png"), ctx.render(1, 1)["rgb"])
+    report = load_tool("validate").main([str(scene), "--out-dir", str(tmp_path / "out")])
+    out = capsys.readouterr().out
+    ref_png = tmp_path / "out" / "ref_scene.png"
+    assert report == tmp_path / "out" / "report_scene.html" and report.is_file()
+    assert "--- path-traced reference render ---" in out and "--- image diff ---" in out
+    assert fmt_line(read_png(scene.with_suffix(".png")), read_png(ref_png)) in out
+    ref = lgi.Context()
+    ref.load_json(scene)
+    ref.build_bvh()
+    assert np.array_equal(read_png(ref_png), ref.render(64, 2)["rgb"])  # cuda_ref defaults: 64 spp, 2 bounces
+
+
+def test_tools_eval_checkpoints_table(gpu, tmp_path, capsys):
+    import torch
+    import lgi
+    from oracle import metrics_ref as M
+    from lgi.train import ARCH
+    from lgi.unet import UNet
+    ckpt = tmp_path / "a.pt"
+    torch.save({"model": UNet(seed=2).state_dict(), "base": 32, "in_ch": 3, "arch": ARCH}, ckpt)
+    ctx, _ = small_scene(tmp_path, w=80, h=48)
+    ctx.build_bvh()
+    noisy = ctx.render(4, 2, hdr=True)
+    gt = ctx.render(256, 2)["rgb"]
+    lgi.write_png(tmp_path / "noisy.png", noisy["rgb"])
+    noisy["hdr"].astype(np.float32).tofile(tmp_path / "noisy.f32")
+    lgi.write_png(tmp_path / "gt.png", gt)
+    rows = load_tool("eval_checkpoints").main([str(tmp_path / "noisy.png"), str(tmp_path / "gt.png"), str(ckpt),
+                                              "--hdr", str(tmp_path / "noisy.f32"), "--save-dir", str(tmp_path / "s")])
+    out = capsys.readouterr().out.splitlines()
+    assert out[0] == "| checkpoint | infer ms | PSNR (dB) | SSIM |"
+    raw = noisy["rgb"]
+    assert out[2] == f"| (raw input) | — | {M.psnr(raw, gt):.2f} | {M.ssim(raw, gt):.4f} |"
+    den = read_png(tmp_path / "s" / "a.denoised.png")
+    assert rows[1][0] == "a.pt" and rows[1][1] > 0
+    assert rows[1][2] == pytest.approx(M.psnr(den, gt), abs=1e-9)
+    assert rows[1][3] == pytest.approx(M.ssim(den, gt), abs=1e-12)

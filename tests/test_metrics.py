@@ -50,3 +50,54 @@ def test_gpu_metrics_match_host(gpu):
     assert 20 * np.log10(255 / np.sqrt(mse)) == pytest.approx(M.psnr(a, b), abs=1e-9)
     assert s == pytest.approx(M.ssim(a, b), abs=1e-12)
     assert m == pytest.approx(M.mad(a, b), abs=1e-12)
+
+
+def test_heat_lut_closed_forms():
+    """The heatmap colour table against exact rational evaluation of its quadratics: equal
+    wherever the exact value is not within 1e-9 of an integer (where float64 may land on either
+    side), and hand-evaluated endpoints and midpoint."""
+    from fractions import Fraction as F
+    lut = M.heat_lut()
+    coef = [(F(-2, 10), F(24, 10), F(-13, 10)), (F(1, 10), F(8, 10), F(1, 10)), (F(4, 10), F(8, 10), F(-11, 10))]
+    for i in range(256):
+        x = F(i, 255)
+        for c, (a0, a1, a2) in enumerate(coef):
+            v = min(max(a0 + a1 * x + a2 * x * x, F(0)), F(1)) * 255
+            if abs(v - round(v)) > F(1, 10 ** 9):
+                assert lut[i, c] == v.numerator // v.denominator, (i, c)
+            else:
+                assert lut[i, c] in (round(v) - 1, round(v)), (i, c)
+    assert lut[0].tolist() == [0, 25, 102]      # (clip(-0.2), 0.1, 0.4) * 255
+    assert lut[255].tolist() == [229, 255, 25]  # (0.9, 1.0, 0.1) * 255
+    assert lut[128].tolist() == [172, 134, 133]
+
+
+def test_library_heat_lut_equals_oracle():
+    from lgi import metrics
+    assert np.array_equal(metrics.heat_lut(), M.heat_lut())
+
+
+def test_heatmap_closed_forms():
+    """Index = min(255, floor(mean |a - b| * gain)): pixel pairs with known channel sums."""
+    lut = M.heat_lut()
+    a = np.zeros((1, 7, 3), np.uint8)
+    b = np.array([[[0, 0, 0], [3, 0, 0], [1, 0, 0], [2, 0, 0], [64, 64, 64], [63, 64, 64], [0, 255, 255]]], np.uint8)
+    want = [0, 4, 1, 2, 255, 254, 255]   # 0, 3/3*4, 1/3*4, 2/3*4, clip(256), 191/3*4 = 254.67, clip
+    assert np.array_equal(M.heatmap(a, b), lut[want][None])
+    assert np.array_equal(M.heatmap(b, a), lut[want][None])  # symmetric
+    assert np.array_equal(M.heatmap(a, b, gain=0.0), lut[[0] * 7][None])
+    assert np.array_equal(M.heatmap(a, b, gain=1.0), lut[[0, 1, 0, 0, 64, 63, 170]][None])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("h,w", [(1, 1), (37, 53), (864, 1728)])
+def test_gpu_heatmap_matches_host(gpu, h, w):
+    import torch
+    from lgi import metrics
+    rng = np.random.default_rng(h * 7 + w)
+    a = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    b = np.clip(a.astype(np.int16) + rng.integers(-80, 81, a.shape), 0, 255).astype(np.uint8)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    for gain in (4.0, 1.5, 0.0, -2.0, 1e3):
+        got = metrics.heatmap(ta, tb, gain).cpu().numpy()
+        assert np.array_equal(got, M.heatmap(a, b, gain)), gain
