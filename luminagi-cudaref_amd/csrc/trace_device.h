@@ -66,12 +66,27 @@ struct TravStackT {
     int* spill;        // spill + global thread id, stride spill_stride
     int spill_stride;
     static constexpr int kBlockThreads = 256;
+    // The empty asm ends the LDS branch with an instruction the global branch lacks, so the
+    // compiler cannot sink the two accesses into one generic-pointer (FLAT) access: a FLAT
+    // access waits on both the vector-memory and the LDS counters, an LDS access only on the
+    // latter.
     LGI_D void push(int sp, int v) const {
-        if (sp < kLds) lds[sp * kBlockThreads] = v;
-        else spill[(sp - kLds) * spill_stride] = v;
+        if (sp < kLds) {
+            lds[sp * kBlockThreads] = v;
+            __asm__ volatile("" ::: "memory");
+        } else {
+            spill[(sp - kLds) * spill_stride] = v;
+        }
     }
     LGI_D int pop(int sp) const {
-        return sp < kLds ? lds[sp * kBlockThreads] : spill[(sp - kLds) * spill_stride];
+        int v;
+        if (sp < kLds) {
+            v = lds[sp * kBlockThreads];
+            __asm__ volatile("" ::: "memory");
+        } else {
+            v = spill[(sp - kLds) * spill_stride];
+        }
+        return v;
     }
 };
 using TravStack = TravStackT<kStackLds>;

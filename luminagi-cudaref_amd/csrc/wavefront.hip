@@ -231,7 +231,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
         // wave then tests triangles with most lanes busy instead of alternating one node step
         // with one whole leaf per iteration.
         while (node >= 0) {
-            const TNode nd = (kTop > 0 && node < kTop) ? s_top[node] : tnodes[node];
+            TNode nd;
+            if (kTop > 0 && node < kTop) {
+                nd = s_top[node];
+                __asm__ volatile("" ::: "memory");  // keeps this an LDS read (see TravStackT)
+            } else {
+                nd = tnodes[node];
+            }
             float ta, tb;
             const bool ha = slab(r, nd.b0xy.x, nd.b0xy.y, nd.b0xy.z, nd.b0xy.w, nd.bz.x, nd.bz.y, best_t, ta);
             const bool hb = slab(r, nd.b1xy.x, nd.b1xy.y, nd.b1xy.z, nd.b1xy.w, nd.bz.z, nd.bz.w, best_t, tb);
