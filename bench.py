@@ -59,6 +59,37 @@ def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1):
     return samples / dt / 1e6, bytes_per_sample, trace_bytes, c, dt, samples, bvh_s
 
 
+def read_bw_probe(stream):
+    """Read bandwidth (GB/s) the chip delivers from L2-, MALL- and HBM-resident buffers
+    (lgi_probe_read_bw: 4096 workgroups, each XCD re-reading its eighth of the buffer): 16 MB
+    stays in the eight 4 MB L2s, 128 MB (the size of the traversal's working set) is backed by
+    the 256 MB MALL, 4 GB comes from HBM. Best of three launches after a warm-up launch, each
+    timed by the library with HIP events on its stream."""
+    import ctypes as C
+    import torch
+    import lgi
+    lib = lgi._abi.lib()
+    sink = torch.zeros(4, dtype=torch.int32, device="cuda")
+    out = {}
+    for level, mib, reps in (("l2", 16, 2000), ("mall", 128, 240), ("hbm", 4096, 6)):
+        buf = torch.ones(mib << 18, dtype=torch.int32, device="cuda")
+        nbytes = buf.numel() * 4
+
+        def launch(r):
+            ms = C.c_float(0.0)
+            st = lib.lgi_probe_read_bw(C.c_void_p(buf.data_ptr()), nbytes, r, 4096, C.c_void_p(sink.data_ptr()),
+                                       C.c_void_p(stream), C.byref(ms))
+            if st != 0:
+                raise lgi.LgiError(st, "lgi_probe_read_bw")
+            return ms.value
+        torch.cuda.synchronize()
+        launch(1)
+        best = min(launch(reps) for _ in range(3))
+        out[level] = round(nbytes * reps / (best * 1e-3) / 1e9, 1)
+        del buf
+    return out
+
+
 def denoise_section(ckpt, bounces):
     """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864 (sorted renderer)."""
     from lgi import train as T
@@ -92,6 +123,7 @@ def main():
                     help="sorted wavefront renderer (default) or the inline megakernel; same image bits")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
+    ap.add_argument("--no-probe", action="store_true", help="skip the L2/MALL/HBM read-bandwidth probe")
     ap.add_argument("--ckpt", default=str(DENOISER_CKPT), help="denoiser checkpoint for the 40 dB measurement")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
@@ -196,6 +228,10 @@ def main():
                               else "k_render", "algorithmic_bytes": int(algo_bytes),
                               "trace_bytes_per_sample": round(trace_bps, 1),
                               "frame_bytes_per_sample": round(bps, 1), "counters": counters_src}
+        if not args.no_probe:
+            peaks = read_bw_probe(torch.cuda.current_stream(dev).cuda_stream)
+            result["roofline"]["measured_read_peaks"] = dict(peaks, unit="GB/s")
+            result["roofline"]["frac_of_measured"] = {k: round(achieved / v, 4) for k, v in peaks.items()}
         result["cpu_baseline"] = {"value": round(cpu_msps, 4), "unit": "Msamples/s", "cores": threads,
                                   "kind": "port",
                                   "sample": "CPU oracle, same scene and PCG32 streams, %d spp on every %dth row "

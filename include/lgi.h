@@ -38,7 +38,7 @@ typedef enum lgi_status {
     LGI_ERR_HIP         = 5,  /* HIP runtime error (message in lgi_last_error) */
     LGI_ERR_NO_SCENE    = 6,  /* render/build before a scene was set */
     LGI_ERR_NO_BVH      = 7,  /* render before lgi_bvh_build */
-    LGI_ERR_UNSUPPORTED = 8   /* feature not built (e.g. textured materials) */
+    LGI_ERR_UNSUPPORTED = 8   /* request outside what the library implements */
 } lgi_status;
 
 typedef struct lgi_ctx lgi_ctx;
@@ -221,6 +221,16 @@ LGI_API lgi_status lgi_diff_heatmap(const uint8_t* d_a, const uint8_t* d_b, int3
                                     uint8_t* d_out, void* stream);
 /* The heatmap's 256-entry RGB colour table (viridis_lut, tools/diff.py:32-38), host only. */
 LGI_API lgi_status lgi_heat_lut(uint8_t* rgb768);
+
+/* ---- memory-hierarchy probe (no counterpart in the reference: it measures the L2, MALL and
+ * HBM read bandwidth the tracer's roofline is compared with) ----
+ * Reads the first bytes (rounded down to 16) of d_buf reps times with `blocks` workgroups of
+ * 256 threads; each XCD re-reads the same 1/8 of the buffer on every pass. d_buf 16-byte
+ * aligned; d_sink: one device u32 (written only on a data-dependent condition). With ms
+ * non-null the launch is bracketed by HIP events on `stream` and waited for, and ms receives
+ * its duration; otherwise the call is asynchronous. */
+LGI_API lgi_status lgi_probe_read_bw(const void* d_buf, uint64_t bytes, int32_t reps, int32_t blocks,
+                                     uint32_t* d_sink, void* stream, float* ms);
 
 /* ---- batch (read_batch_file, src/main.cpp:61-91) ---- */
 LGI_API lgi_status lgi_batch_parse(const char* path, lgi_batch_job* jobs, int32_t cap, int32_t* n_jobs);

@@ -104,6 +104,7 @@ SIGNATURES = {
     "lgi_image_metrics": (C.c_int, [_P, _P, C.c_int32, C.c_int32, _P, _P, _P]),
     "lgi_diff_heatmap": (C.c_int, [_P, _P, C.c_int32, C.c_int32, C.c_double, _P, _P]),
     "lgi_heat_lut": (C.c_int, [_P]),
+    "lgi_probe_read_bw": (C.c_int, [_P, C.c_uint64, C.c_int32, C.c_int32, _P, _P, _P]),
     # denoiser layers (device pointers + hipStream_t)
     "lgi_unet_conv3x3_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_int32, _P, _P]),

@@ -157,3 +157,13 @@ def test_validate_tool_argument_errors(tmp_path):
     (tmp_path / "s.json").write_text("{}")
     r = subprocess.run([sys.executable, str(tool), str(tmp_path / "s.json")], capture_output=True, text=True)
     assert r.returncode == 1 and "engine screenshot missing:" in r.stderr
+
+
+def test_probe_rejects_bad_arguments():
+    """lgi_probe_read_bw validates on the host before touching the device."""
+    L = _abi.lib()
+    sink = C.c_void_p(64)
+    assert L.lgi_probe_read_bw(None, 1 << 20, 1, 8, sink, None, None) == 1
+    assert L.lgi_probe_read_bw(C.c_void_p(4096 + 8), 1 << 20, 1, 8, sink, None, None) == 1  # not 16-byte aligned
+    assert L.lgi_probe_read_bw(C.c_void_p(4096), 8, 1, 8, sink, None, None) == 1           # under one word
+    assert L.lgi_probe_read_bw(C.c_void_p(4096), 1 << 20, 0, 8, sink, None, None) == 1     # no passes
