@@ -9,8 +9,8 @@
 //   * closest-hit and shadow rays are traced by one persistent kernel that pulls 64-ray
 //     chunks from a queue with one atomic per chunk and refills each lane the moment its ray
 //     finishes, so no lane idles while its wave still has work;
-//   * secondary rays are sorted between bounces by a 30-bit key (18-bit origin Morton code in
-//     the scene box, 12-bit direction Morton code) with the project's LSD radix sort;
+//   * secondary rays are sorted between bounces by a 30-bit key (24-bit origin Morton code in
+//     the scene box, 6-bit direction Morton code) with the project's LSD radix sort;
 //   * results are written per path, never with atomics: each path keeps its own radiance, and
 //     samples are added to the pixel in sample order with Kahan compensation at the end.
 // Unlike the reference sorted path, Russian roulette and the per-path RNG stream are kept,
@@ -73,13 +73,20 @@ __device__ __forceinline__ uint32_t q01(float f, float scale, uint32_t maxv) {
     f = fminf(fmaxf(f, 0.0f), (float)maxv);
     return (uint32_t)f;
 }
-// 30-bit sort key: origin Morton code (6 bits per axis) above direction Morton code (4 bits).
+// Sort key: origin Morton code (kKeyOriginBits per axis) above direction Morton code
+// (kKeyDirBits per axis); 30 bits. On the 1.05M-triangle scene at 64 spp, 8 + 2 bits render
+// 1.2% faster than 6 + 4 (7 + 3: +0.8%, 5 + 5: -0.8%, 9 + 1: same as 8 + 2, origin only:
+// +0.9%, direction bits above origin bits: +0.3%). The key orders work only: each path keeps
+// its own RNG stream and radiance, so the image does not depend on it.
+constexpr int kKeyOriginBits = 8, kKeyDirBits = 2;
+static_assert(3 * (kKeyOriginBits + kKeyDirBits) <= 32, "sort key fits 32 bits");
 __device__ __forceinline__ uint32_t ray_key(const WfParams& W, V3 o, V3 d) {
-    uint32_t ox = q01((o.x - W.bb_min.x) * W.bb_inv.x, 64.0f, 63), oy = q01((o.y - W.bb_min.y) * W.bb_inv.y, 64.0f, 63),
-             oz = q01((o.z - W.bb_min.z) * W.bb_inv.z, 64.0f, 63);
-    uint32_t dx = q01(d.x * 0.5f + 0.5f, 16.0f, 15), dy = q01(d.y * 0.5f + 0.5f, 16.0f, 15),
-             dz = q01(d.z * 0.5f + 0.5f, 16.0f, 15);
-    return (((spread3(ox) << 2) | (spread3(oy) << 1) | spread3(oz)) << 12) |
+    constexpr float so = (float)(1 << kKeyOriginBits), sd = (float)(1 << kKeyDirBits);
+    constexpr uint32_t mo = (1u << kKeyOriginBits) - 1u, md = (1u << kKeyDirBits) - 1u;
+    uint32_t ox = q01((o.x - W.bb_min.x) * W.bb_inv.x, so, mo), oy = q01((o.y - W.bb_min.y) * W.bb_inv.y, so, mo),
+             oz = q01((o.z - W.bb_min.z) * W.bb_inv.z, so, mo);
+    uint32_t dx = q01(d.x * 0.5f + 0.5f, sd, md), dy = q01(d.y * 0.5f + 0.5f, sd, md), dz = q01(d.z * 0.5f + 0.5f, sd, md);
+    return (((spread3(ox) << 2) | (spread3(oy) << 1) | spread3(oz)) << (3 * kKeyDirBits)) |
            ((spread3(dx) << 2) | (spread3(dy) << 1) | spread3(dz));
 }
 
