@@ -465,11 +465,11 @@ struct SortBufs {
     uint32_t* sums;
 };
 
-// Stable LSD radix sort of (keys, vals) over `bits` low bits (a multiple of 16, so the
-// even number of ping-pong passes leaves the result in buffer 0).
+// Stable LSD radix sort of (keys, vals) in buffer 0 over `bits` low bits (a multiple of 8);
+// returns the buffer holding the result (0 when bits is a multiple of 16).
 // n_dev (nullable): the key count on the device, at most n; the grids are sized for n.
-void radix_sort(hipStream_t st, SortBufs& b, int n, int bits, const uint32_t* n_dev = nullptr) {
-    if (n <= 1) return;
+int radix_sort(hipStream_t st, SortBufs& b, int n, int bits, const uint32_t* n_dev = nullptr) {
+    if (n <= 1) return 0;
     const int nb = ceil_div(n, kTile);
     const int ncount = 256 * nb;
     const int nsb = ceil_div(ncount, kTile);
@@ -483,6 +483,7 @@ void radix_sort(hipStream_t st, SortBufs& b, int n, int bits, const uint32_t* n_
                            b.v[cur ^ 1], n, n_dev, shift, b.counts, nb);
         cur ^= 1;
     }
+    return cur;
 }
 
 template <class T>
@@ -501,8 +502,8 @@ size_t lgi_sort_scratch_bytes(int64_t n) {
     return 4 * (4 * (size_t)n + ncount + ceil_div(ncount, kTile) + 64);
 }
 
-void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
-                    const uint32_t* n_dev) {
+const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
+                              const uint32_t* n_dev) {
     SortBufs b;
     char* p = (char*)scratch;
     const int nb = ceil_div(n, kTile);
@@ -513,7 +514,7 @@ void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bi
     b.v[1] = (int32_t*)p; p += 4 * (size_t)n;
     b.counts = (uint32_t*)p; p += 4 * ncount;
     b.sums = (uint32_t*)p;
-    radix_sort(st, b, n, bits, n_dev);
+    return b.v[radix_sort(st, b, n, bits, n_dev)];
 }
 
 size_t lgi_scan_scratch_bytes(int64_t n) { return 4 * ((size_t)ceil_div(n, kTile) + 64); }

@@ -122,12 +122,14 @@ lgi_status lgi_set_err(lgi_ctx* ctx, lgi_status st, const std::string& msg);
 lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx);
 // Records the start (begin=true) or end event of the next traversal launch on ctx->stream.
 lgi_status lgi_mark_trace(lgi_ctx* ctx, bool begin);
-// Stable LSD radix sort of (keys, vals) on `bits` (multiple of 16) low key bits, in place;
-// scratch must hold lgi_sort_scratch_bytes(n). With n_dev set, the key count is read on the
-// device (at most n) and n only sizes the launches.
+// Stable LSD radix sort of (keys, vals) on `bits` (multiple of 8) low key bits; in place when
+// bits is a multiple of 16, else the result is left in the scratch buffers (the returned
+// pointer tells). scratch must hold lgi_sort_scratch_bytes(n). With n_dev set, the key count
+// is read on the device (at most n) and n only sizes the launches.
 size_t lgi_sort_scratch_bytes(int64_t n);
-void lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
-                    const uint32_t* n_dev = nullptr);
+// Returns the sorted values: vals itself when bits / 8 is even, else the scratch copy.
+const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
+                              const uint32_t* n_dev = nullptr);
 // In-place exclusive scan of n u32 values; *total (device) receives the sum.
 size_t lgi_scan_scratch_bytes(int64_t n);
 void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, void* scratch);

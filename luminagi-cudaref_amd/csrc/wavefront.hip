@@ -79,6 +79,7 @@ __device__ __forceinline__ uint32_t q01(float f, float scale, uint32_t maxv) {
 // +0.9%, direction bits above origin bits: +0.3%). The key orders work only: each path keeps
 // its own RNG stream and radiance, so the image does not depend on it.
 constexpr int kKeyOriginBits = 8, kKeyDirBits = 2;
+constexpr int kKeySortBits = (3 * (kKeyOriginBits + kKeyDirBits) + 7) / 8 * 8;  // radix passes of 8 bits
 static_assert(3 * (kKeyOriginBits + kKeyDirBits) <= 32, "sort key fits 32 bits");
 __device__ __forceinline__ uint32_t ray_key(const WfParams& W, V3 o, V3 d) {
     constexpr float so = (float)(1 << kKeyOriginBits), sd = (float)(1 << kKeyDirBits);
@@ -655,8 +656,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             // compact the continuations, sort them by origin/direction key, gather in key order
             hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont,
                                n_max, n_dev, (const uint32_t*)blk_n, cur, keys, idx);
-            lgi_sort_pairs(st, keys, idx, n_max, 32, sort_scratch, cnt + 2);
-            hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, idx, other, n_max, cnt + 2);
+            const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
+            hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, sorted, other, n_max, cnt + 2);
             std::swap(cur, other);
             n_dev = cnt + 2;
         }
