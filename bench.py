@@ -90,6 +90,59 @@ def read_bw_probe(stream):
     return out
 
 
+def poses_workload(n_poses, seed, bounces):
+    """The multi-pose workload: 50 poses on the 15 m orbit of the 16-mesh scene at 1728x864, spp
+    log-uniform in [8, 256], 2 bounces. Timed three ways: the persistent queue (one launch),
+    the same replayed from its HIP graph, and one sorted-renderer frame per pose. Every pose's
+    image is the same in all three (checked here on two poses)."""
+    import torch
+    import lgi
+    from lgi import poses
+    ctx = lgi.Context(0, torch.cuda.current_stream().cuda_stream)
+    ctx.generate(1, 1, 1728, 864)
+    ctx.build_bvh()
+    base = ctx.camera()
+    n = base.width * base.height
+    jobs = [(poses.camera_for(base, pos, tgt), spp, 0) for pos, tgt, spp in poses.orbit_poses(n_poses, seed=seed)]
+    samples = sum(j[1] for j in jobs) * n
+    bs = torch.zeros(n_poses, n * 3, device="cuda")
+    bc = torch.zeros_like(bs)
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+    out = {}
+    for key, graph in (("queue_eager", False), ("queue_graph_capture", True), ("queue_graph_replay", True)):
+        bs.zero_()
+        bc.zero_()
+        dt = timed(lambda: ctx.render_batch(jobs, bs, bc, bounces, use_graph=graph))
+        out[key] = {"ms": round(dt * 1e3, 1), "msamples_per_s": round(samples / dt / 1e6, 1)}
+    # one sorted-renderer frame per pose; poses 0 and n/2 keep their sums for the comparison
+    check = (0, n_poses // 2)
+    keep_s = torch.zeros(len(check), n * 3, device="cuda")
+    keep_c = torch.zeros_like(keep_s)
+    tmp_s = torch.zeros(n * 3, device="cuda")
+    tmp_c = torch.zeros_like(tmp_s)
+    dt = 0.0
+    for i, (cam, spp, s0) in enumerate(jobs):
+        if i in check:
+            d_s, d_c = keep_s[check.index(i)], keep_c[check.index(i)]
+        else:
+            tmp_s.zero_()
+            tmp_c.zero_()
+            d_s, d_c = tmp_s, tmp_c
+        dt += timed(lambda: ctx.accumulate(d_s, d_c, spp, bounces, flags=lgi.RENDER_SORTED, sample0=s0, camera=cam))
+    out["sorted_per_pose"] = {"ms": round(dt * 1e3, 1), "msamples_per_s": round(samples / dt / 1e6, 1)}
+    same = all(torch.equal(keep_s[k], bs[i]) for k, i in enumerate(check))
+    return {"workload": "multi-pose: 16-mesh procedural scene (%d triangles), %d poses on a 15 m orbit, "
+                        "spp log-uniform in [8, 256] (seed %d), 1728x864, %d bounces"
+                        % (ctx.counts()[0], n_poses, seed, bounces),
+            "samples": samples, "images_equal": same, **out}
+
+
 def denoise_section(ckpt, bounces):
     """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864 (sorted renderer)."""
     from lgi import train as T
@@ -125,6 +178,9 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-probe", action="store_true", help="skip the L2/MALL/HBM read-bandwidth probe")
     ap.add_argument("--ckpt", default=str(DENOISER_CKPT), help="denoiser checkpoint for the 40 dB measurement")
+    ap.add_argument("--workload", choices=["frame", "poses"], default="frame",
+                    help="frame: the 64-spp frame of the 1.05M-triangle scene (default); poses: only time the "
+                         "50-pose multi-pose workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -142,6 +198,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the tracer would shard by pose or image tile with one accumulator reduce)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "poses":
+        print(json.dumps(poses_workload(50, 3, args.bounces)), flush=True)
+        return
 
     import numpy as np
     import torch
