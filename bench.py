@@ -144,7 +144,8 @@ def poses_workload(n_poses, seed, bounces):
 
 
 def denoise_section(ckpt, bounces):
-    """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864 (sorted renderer)."""
+    """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864, with the faster of
+    the two renderers on that scene (same image bits)."""
     from lgi import train as T
     ctx = T.make_scene(1, 1, 1728, 864)
     model = T.load_model(ckpt)
@@ -154,6 +155,9 @@ def denoise_section(ckpt, bounces):
     out = {k: (round(r[k], 3) if isinstance(r[k], float) else r[k]) for k in keep if k in r}
     out["unet"] = {k: round(v, 3) for k, v in r["unet"].items()}
     out["noisy8_psnr"] = round(r["noisy8"]["psnr"], 3)
+    out["renderer"] = r["renderer"]
+    if "renderer_ms_8spp" in r:
+        out["renderer_ms_8spp"] = {k: round(v, 3) for k, v in r["renderer_ms_8spp"].items()}
     if "unet_fallback" in r:
         out["unet_fallback"] = {k: round(v, 3) for k, v in r["unet_fallback"].items()}
     out["plain_curve"] = [(p["spp"], round(p["ms"], 2), round(p["psnr"], 3)) for p in r["plain"]]

@@ -504,10 +504,14 @@ lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t
     }
     if ((st = attach_spill(ctx, P, lgi_grid_threads(cam.width, cam.height))) != LGI_OK) return st;
     ctx->kev_used = 0;
-    LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    // timing events only outside stream capture, so that the call can be captured in a graph
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    LGI_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+    const bool timed = cap == hipStreamCaptureStatusNone;
+    if (timed) LGI_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     lgi_launch_render(P, ctx->stream);
     LGI_HIP(ctx, hipGetLastError());
-    LGI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (timed) LGI_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     return LGI_OK;
 }
 

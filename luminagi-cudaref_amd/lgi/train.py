@@ -176,13 +176,30 @@ def plain_spp_for(pts, psnr):
     return None
 
 
-def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=2, log=print):
+def pick_renderer(ctx, cam, bounces=2, spp=8):
+    """The faster renderer for this scene and camera: (flags, {flags: ms}). Both renderers give
+    the same bits; the inline megakernel wins while the scene stays in L2, the sorted wavefront
+    renderer on large scenes. Times spp samples with each after a warm-up."""
+    times = {}
+    for fl in (0, 2):
+        D.Frame(cam).render(ctx, spp, bounces, sample0=1 << 25, flags=fl)
+        f = D.Frame(cam)
+        _, times[fl] = timed(lambda: f.render(ctx, spp, bounces, sample0=1 << 25, flags=fl))
+    return min(times, key=times.get), times
+
+
+def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None, log=print):
     """PSNR/SSIM (8-bit sqrt-tonemapped domain, as tools/diff.py) of plain renders at growing spp
     and of 8 spp + UNet against a ref_spp plain render; times to 40 dB. flags selects the
-    renderer used for the timed renders (default: sorted wavefront; both give the same bits)."""
+    renderer used for the timed renders (None: the faster one on this scene, pick_renderer;
+    both give the same bits)."""
     gt = D.tonemap(ctx, render_mean(ctx, cam, ref_spp, 0, bounces, batch=256))
     test0 = 1 << 24  # sample streams disjoint from the reference's
     res = {"ref_spp": ref_spp, "plain": []}
+    if flags is None:
+        flags, rt = pick_renderer(ctx, cam, bounces)
+        res["renderer_ms_8spp"] = {("sorted" if k else "inline"): v for k, v in rt.items()}
+    res["renderer"] = "sorted" if flags & 2 else "inline"
     # plain curve: progressive accumulation, timed per added chunk
     f = D.Frame(cam)
     total_ms, spp = 0.0, 0

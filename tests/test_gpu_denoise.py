@@ -85,8 +85,10 @@ def test_denoise_graph_replays_match_eager(gpu):
         _ = (x * 2).sum().item()  # eager work between replays
 
 
-def test_frame_graph_replays_match_eager(gpu):
-    """Render + UNet + tonemap captured as one graph: each replay equals the eager frame."""
+@pytest.mark.parametrize("flags", [2, 0], ids=["sorted", "inline"])
+def test_frame_graph_replays_match_eager(gpu, flags):
+    """Render + UNet + tonemap captured as one graph, with either renderer: each replay equals
+    the eager frame."""
     import torch
     from lgi import denoise as D
     from lgi import train
@@ -94,9 +96,9 @@ def test_frame_graph_replays_match_eager(gpu):
     ctx = train.make_scene(1, 2, 96, 64)
     cam = ctx.camera()
     model = UNet(seed=5).cuda().eval()
-    fg = D.FrameGraph(ctx, model, cam, spp=4, sample0=9)
+    fg = D.FrameGraph(ctx, model, cam, spp=4, sample0=9, flags=flags)
     f = D.Frame(cam)
-    f.render(ctx, 4, 2, sample0=9, flags=2)
+    f.render(ctx, 4, 2, sample0=9, flags=flags)
     want = D.denoise(model, f.mean())
     for k in range(3):
         hdr, rgb = fg.replay()

@@ -207,24 +207,24 @@ def test_gbuffer_matches_oracle(gpu):
         assert frac < 1e-3
 
 
-def test_sorted_frame_graph_replays_match_eager(gpu):
-    """A sorted-renderer frame (zero accumulators, trace, sort, accumulate) captured in a HIP
-    graph: every replay reproduces the eager frame bit for bit, with eager work and
+@pytest.mark.parametrize("flags", [2, 0], ids=["sorted", "inline"])
+def test_frame_graph_replays_match_eager_renderers(gpu, flags):
+    """A frame (zero accumulators, trace (+ sort), accumulate) of either renderer captured in a
+    HIP graph: every replay reproduces the eager frame bit for bit, with eager work and
     synchronisations between the replays."""
     import torch
-    import lgi
     ctx, _, s = make_pair(seed=4)
     cam = small_camera(s["camera"], 160, 96)
-    eager = gpu_accumulate(ctx, cam, 8, 2, flags=lgi.RENDER_SORTED, sample0=5)
+    eager = gpu_accumulate(ctx, cam, 8, 2, flags=flags, sample0=5)
     bufs = [torch.zeros_like(t) for t in eager]
-    ctx.accumulate(bufs[0], bufs[1], 8, 2, flags=lgi.RENDER_SORTED, sample0=5, d_sumsq=bufs[2], camera=cam)  # warm-up
+    ctx.accumulate(bufs[0], bufs[1], 8, 2, flags=flags, sample0=5, d_sumsq=bufs[2], camera=cam)  # warm-up
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
         ctx.set_stream(torch.cuda.current_stream().cuda_stream)  # the renderer joins the capture stream
         for t in bufs:
             t.zero_()
-        ctx.accumulate(bufs[0], bufs[1], 8, 2, flags=lgi.RENDER_SORTED, sample0=5, d_sumsq=bufs[2], camera=cam)
+        ctx.accumulate(bufs[0], bufs[1], 8, 2, flags=flags, sample0=5, d_sumsq=bufs[2], camera=cam)
     ctx.set_stream(None)
     for k in range(3):
         g.replay()
