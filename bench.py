@@ -143,6 +143,40 @@ def poses_workload(n_poses, seed, bounces):
             "samples": samples, "images_equal": same, **out}
 
 
+UNET_GFLOP = 635.8  # 2*H*W*Cin*Cout*9 summed over the 18 3x3 convolutions at 1728x864
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_workload(ckpt, reps=20):
+    """The denoiser alone on one 1728x864 frame (log1p radiance in, radiance out): eager forward
+    and the HIP-graph replay, best of `reps` after warm-up, against the dense BF16 MFMA peak."""
+    import torch
+    from lgi import denoise as D
+    from lgi import train as T
+    model = T.load_model(ckpt)
+    x = torch.rand(864, 1728, 3, device="cuda", generator=torch.Generator("cuda").manual_seed(0)) * 4.0
+    for _ in range(3):
+        D.denoise(model, x)
+    g = D.DenoiseGraph(model, 864, 1728)
+
+    def best(f):
+        t_best = float("inf")
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            t_best = min(t_best, (time.perf_counter() - t) * 1e3)
+        return t_best
+    eager = best(lambda: D.denoise(model, x))
+    graph = best(lambda: g(x))
+    assert torch.equal(g(x), D.denoise(model, x)), "graph replay differs from the eager forward"
+    return {"workload": "UNet 32/64/128/256 + 512 bottleneck, one 1728x864 frame, BF16 MFMA convolutions",
+            "checkpoint": os.path.relpath(ckpt, ROOT), "gflop": UNET_GFLOP,
+            "eager_ms": round(eager, 3), "graph_ms": round(graph, 3),
+            "tflops_graph": round(UNET_GFLOP / graph, 1), "frac_of_bf16_peak": round(UNET_GFLOP / graph / BF16_PEAK_TFLOPS, 4)}
+
+
 def denoise_section(ckpt, bounces):
     """Time to 40 dB on the held-out pose of the 16-mesh scene at 1728x864, with the faster of
     the two renderers on that scene (same image bits)."""
@@ -182,9 +216,9 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-probe", action="store_true", help="skip the L2/MALL/HBM read-bandwidth probe")
     ap.add_argument("--ckpt", default=str(DENOISER_CKPT), help="denoiser checkpoint for the 40 dB measurement")
-    ap.add_argument("--workload", choices=["frame", "poses"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "poses", "unet"], default="frame",
                     help="frame: the 64-spp frame of the 1.05M-triangle scene (default); poses: only time the "
-                         "50-pose multi-pose workload")
+                         "50-pose multi-pose workload; unet: only time the denoiser on one 1728x864 frame")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -205,6 +239,9 @@ def main():
 
     if args.workload == "poses":
         print(json.dumps(poses_workload(50, 3, args.bounces)), flush=True)
+        return
+    if args.workload == "unet":
+        print(json.dumps(unet_workload(args.ckpt)), flush=True)
         return
 
     import numpy as np
