@@ -103,6 +103,9 @@ typedef struct lgi_batch_job {       /* BatchJob, src/main.cpp:63-69 */
 #define LGI_RNG_XORSHIFT_REF   1u   /* reference xorshift32 chunked seeding (src/pathtracer.cu:349) */
 #define LGI_RENDER_SORTED      2u   /* wavefront path: radix-sort rays between bounces */
 #define LGI_RENDER_NO_RR       4u   /* disable Russian roulette (reference sorted path semantics) */
+#define LGI_RENDER_NO_RAY_SORT 8u   /* with LGI_RENDER_SORTED: no key sort between bounces, the
+                                       continuations keep their compaction order (same image bits;
+                                       no reference counterpart) */
 
 /* ---- context ---- */
 LGI_API lgi_status  lgi_create(lgi_ctx** out, int device, void* hip_stream);

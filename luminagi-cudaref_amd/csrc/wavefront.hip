@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             nr.o = make_float4(next.o.x, next.o.y, next.o.z, __int_as_float(p));
             nr.d = make_float4(next.d.x, next.d.y, next.d.z, 1e30f);
             nq[i] = nr;
-            nkeys[i] = ray_key(W, next.o, next.d);
+            if (nkeys) nkeys[i] = ray_key(W, next.o, next.d);
         }
         ncont[i] = cont ? 1 : 0;
     }
@@ -442,8 +442,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const ui
     if (!f) return;
     const uint32_t o = blk_off[blockIdx.x] + r;
     dst[o] = src[i];
-    dst_key[o] = src_key[i];
-    dst_idx[o] = (int32_t)o;
+    if (dst_key) {
+        dst_key[o] = src_key[i];
+        dst_idx[o] = (int32_t)o;
+    }
 }
 
 // NEE candidate ids j = slot * nl + light of block b to blk_off[b] + their rank in the block
@@ -612,6 +614,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                     (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
+    const bool sort = !(P.flags & LGI_RENDER_NO_RAY_SORT);
     // No host round trip inside a frame: queue lengths stay on the device (each kernel reads its
     // count and the launches are sized for the capacity), so a frame can be captured in a graph.
     // Timing events are skipped while the stream is being captured.
@@ -641,7 +644,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                                 (uint8_t*)nullptr, cnt + 0, spill, shadow, n_dev);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
-                               keys_tmp, ncont, blk_s, blk_n);
+                               sort ? keys_tmp : nullptr, ncont, blk_s, blk_n);
             // block offsets of the shadow-ray candidates and of the continuations
             lgi_exclusive_scan(st, blk_s, nb, cnt + 1, scan_scratch);
             lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
@@ -654,11 +657,14 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev);
             if (b == P.bounces) break;
             // compact the continuations, sort them by origin/direction key, gather in key order
+            // (without the sort, the compacted queue in cur is the next bounce's queue)
             hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont,
-                               n_max, n_dev, (const uint32_t*)blk_n, cur, keys, idx);
-            const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
-            hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, sorted, other, n_max, cnt + 2);
-            std::swap(cur, other);
+                               n_max, n_dev, (const uint32_t*)blk_n, cur, sort ? keys : nullptr, idx);
+            if (sort) {
+                const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
+                hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, sorted, other, n_max, cnt + 2);
+                std::swap(cur, other);
+            }
             n_dev = cnt + 2;
         }
         const int n_acc = d_tiles ? n_tiles * 64 : npx;

@@ -22,6 +22,7 @@ RNG_PCG32 = 0
 RNG_XORSHIFT_REF = 1
 RENDER_SORTED = 2
 RENDER_NO_RR = 4
+RENDER_NO_RAY_SORT = 8
 
 
 class Camera(C.Structure):

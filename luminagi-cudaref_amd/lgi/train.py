@@ -176,12 +176,16 @@ def plain_spp_for(pts, psnr):
     return None
 
 
+RENDERERS = {0: "inline", 2: "sorted", 2 | 8: "wavefront without ray sort"}
+
+
 def pick_renderer(ctx, cam, bounces=2, spp=8):
-    """The faster renderer for this scene and camera: (flags, {flags: ms}). Both renderers give
-    the same bits; the inline megakernel wins while the scene stays in L2, the sorted wavefront
-    renderer on large scenes. Times spp samples with each after a warm-up."""
+    """The fastest renderer for this scene and camera: (flags, {flags: ms}). All three give the
+    same bits: the inline megakernel, the wavefront renderer with its ray sort between bounces,
+    and the wavefront renderer without it (LGI_RENDER_NO_RAY_SORT). Which one wins depends on
+    the scene and the view. Times spp samples with each after a warm-up."""
     times = {}
-    for fl in (0, 2):
+    for fl in RENDERERS:
         D.Frame(cam).render(ctx, spp, bounces, sample0=1 << 25, flags=fl)
         f = D.Frame(cam)
         _, times[fl] = timed(lambda: f.render(ctx, spp, bounces, sample0=1 << 25, flags=fl))
@@ -198,8 +202,8 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None
     res = {"ref_spp": ref_spp, "plain": []}
     if flags is None:
         flags, rt = pick_renderer(ctx, cam, bounces)
-        res["renderer_ms_8spp"] = {("sorted" if k else "inline"): v for k, v in rt.items()}
-    res["renderer"] = "sorted" if flags & 2 else "inline"
+        res["renderer_ms_8spp"] = {RENDERERS[k]: v for k, v in rt.items()}
+    res["renderer"] = RENDERERS.get(flags, str(flags))
     # plain curve: progressive accumulation, timed per added chunk
     f = D.Frame(cam)
     total_ms, spp = 0.0, 0

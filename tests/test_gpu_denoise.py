@@ -85,7 +85,7 @@ def test_denoise_graph_replays_match_eager(gpu):
         _ = (x * 2).sum().item()  # eager work between replays
 
 
-@pytest.mark.parametrize("flags", [2, 0], ids=["sorted", "inline"])
+@pytest.mark.parametrize("flags", [2, 0, 10], ids=["sorted", "inline", "wavefront-no-sort"])
 def test_frame_graph_replays_match_eager(gpu, flags):
     """Render + UNet + tonemap captured as one graph, with either renderer: each replay equals
     the eager frame."""

@@ -103,9 +103,10 @@ def test_two_bounce_64spp_small(gpu, flags):
     assert np.all(np.abs(g - o["sum64"]) <= 1e-6 * np.abs(o["sum64"]) + 1e-6)
 
 
-@pytest.mark.parametrize("flags", [2, 2 | 4], ids=["sorted", "sorted_no_rr"])
+@pytest.mark.parametrize("flags", [2, 2 | 4, 2 | 8], ids=["sorted", "sorted_no_rr", "no_ray_sort"])
 def test_sorted_wavefront_matches_inline_and_oracle(gpu, flags):
-    """The sorted wavefront renderer produces the inline kernel's sums bit for bit."""
+    """The wavefront renderer, with or without its ray sort, produces the inline kernel's sums
+    bit for bit."""
     ctx, osc, s = make_pair(seed=6)
     cam = small_camera(s["camera"], 200, 100)
     inline = gpu_accumulate(ctx, cam, 16, 2, flags=flags & 4)
@@ -207,7 +208,7 @@ def test_gbuffer_matches_oracle(gpu):
         assert frac < 1e-3
 
 
-@pytest.mark.parametrize("flags", [2, 0], ids=["sorted", "inline"])
+@pytest.mark.parametrize("flags", [2, 0, 10], ids=["sorted", "inline", "wavefront-no-sort"])
 def test_frame_graph_replays_match_eager_renderers(gpu, flags):
     """A frame (zero accumulators, trace (+ sort), accumulate) of either renderer captured in a
     HIP graph: every replay reproduces the eager frame bit for bit, with eager work and
