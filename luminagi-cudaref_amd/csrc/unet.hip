@@ -496,14 +496,27 @@ __device__ __forceinline__ uint4 pack8(const float f[8]) {
     return *reinterpret_cast<const uint4*>(&e);
 }
 
+// Thread mapping of the per-pixel NHWC kernels below: blockIdx.y = n * rows + row, and the
+// threads along x cover the (column, 8-channel group) pairs of that row. Index math stays in
+// 32 bits, with one division per thread (64-bit division is a long instruction sequence).
+__device__ __forceinline__ bool row_thread(int rows, int cols, int C8, int& n, int& y, int& x, int& c) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= cols * C8) return false;
+    n = (int)blockIdx.y / rows;
+    y = (int)blockIdx.y - n * rows;
+    x = j / C8;
+    c = (j - x * C8) * 8;
+    return true;
+}
+
+inline dim3 row_grid(int n, int rows, int cols, int C8) { return dim3(cdiv((int64_t)cols * C8, kBlock), n * rows); }
+
 // MaxPool2d(2): out[n,y,x,c] = max over the 2x2 window (first maximum in row-major order).
 __global__ __launch_bounds__(kBlock) void k_pool_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
-    if (i >= (int64_t)N * Ho * Wo * C8) return;
-    const int c = (int)(i % C8) * 8;
-    const int64_t p = i / C8;
-    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    int n, y, x, c;
+    if (!row_thread(Ho, Wo, C8, n, y, x, c)) return;
+    const int64_t p = ((int64_t)n * Ho + y) * Wo + x;
     const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
     const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
     float m[8], v[8];
@@ -519,12 +532,10 @@ __global__ __launch_bounds__(kBlock) void k_pool_fwd(const bf16* in, int N, int 
 // Gradient to the first maximum of each window (the pooled input is re-read, not stored).
 __global__ __launch_bounds__(kBlock) void k_pool_bwd(const bf16* in, const bf16* dout, int N, int H, int W, int C,
                                                      bf16* din) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
-    if (i >= (int64_t)N * Ho * Wo * C8) return;
-    const int c = (int)(i % C8) * 8;
-    const int64_t p = i / C8;
-    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    int n, y, x, c;
+    if (!row_thread(Ho, Wo, C8, n, y, x, c)) return;
+    const int64_t p = ((int64_t)n * Ho + y) * Wo + x;
     const int64_t o[4] = {0, C, (int64_t)W * C, (int64_t)W * C + C};
     const int64_t base = (((int64_t)n * H + 2 * y) * W + 2 * x) * C + c;
     float m[8], v[8];
@@ -553,12 +564,10 @@ __device__ __forceinline__ void lin_src(int d, int in, int& i0, int& i1, float& 
 }
 
 __global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int Ho = 2 * H, Wo = 2 * W, C8 = C / 8;
-    if (i >= (int64_t)N * Ho * Wo * C8) return;
-    const int c = (int)(i % C8) * 8;
-    const int64_t p = i / C8;
-    const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), n = (int)(p / ((int64_t)Wo * Ho));
+    int n, y, x, c;
+    if (!row_thread(Ho, Wo, C8, n, y, x, c)) return;
+    const int64_t p = ((int64_t)n * Ho + y) * Wo + x;
     int y0, y1, x0, x1;
     float ly, lx;
     lin_src(y, H, y0, y1, ly);
@@ -576,12 +585,10 @@ __global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H,
 // Adjoint of k_up_fwd as a gather: each input pixel sums the weighted gradients of the
 // (at most 4 x 4) output pixels that read it.
 __global__ __launch_bounds__(kBlock) void k_up_bwd(const bf16* dout, int N, int H, int W, int C, bf16* din) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int C8 = C / 8;
-    if (i >= (int64_t)N * H * W * C8) return;
-    const int c = (int)(i % C8) * 8;
-    const int64_t p = i / C8;
-    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
+    int n, y, x, c;
+    if (!row_thread(H, W, C8, n, y, x, c)) return;
+    const int64_t p = ((int64_t)n * H + y) * W + x;
     const int Ho = 2 * H, Wo = 2 * W;
     float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, g[8];
     for (int oy = max(0, 2 * y - 2); oy <= min(Ho - 1, 2 * y + 2); ++oy) {
@@ -902,35 +909,37 @@ lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32
     return LGI_OK;
 }
 
+// The row-mapped kernels need n * rows <= 65535 (grid y) and cols * C/8 < 2^31.
+static bool row_grid_fits(int64_t n, int64_t rows, int64_t cols, int64_t c8) {
+    return n > 0 && rows > 0 && cols > 0 && n * rows <= 65535 && cols * c8 < (int64_t(1) << 31);
+}
+
 lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
-    if (!in || !out || h % 2 || w % 2 || c % 8) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * (c / 8);
-    hipLaunchKernelGGL(k_pool_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n,
-                       h, w, c, (bf16*)out);
+    if (!in || !out || h % 2 || w % 2 || c % 8 || !row_grid_fits(n, h / 2, w / 2, c / 8)) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_pool_fwd, row_grid(n, h / 2, w / 2, c / 8), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const bf16*)in, n, h, w, c, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int32_t h, int32_t w, int32_t c,
                                 void* din, void* stream) {
-    if (!in || !dout || !din || h % 2 || w % 2 || c % 8) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * (h / 2) * (w / 2) * (c / 8);
-    hipLaunchKernelGGL(k_pool_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in,
-                       (const bf16*)dout, n, h, w, c, (bf16*)din);
+    if (!in || !dout || !din || h % 2 || w % 2 || c % 8 || !row_grid_fits(n, h / 2, w / 2, c / 8))
+        return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_pool_bwd, row_grid(n, h / 2, w / 2, c / 8), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const bf16*)in, (const bf16*)dout, n, h, w, c, (bf16*)din);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_upsample_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
-    if (!in || !out || c % 8) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * 2 * h * 2 * w * (c / 8);
-    hipLaunchKernelGGL(k_up_fwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)in, n, h,
-                       w, c, (bf16*)out);
+    if (!in || !out || c % 8 || !row_grid_fits(n, 2 * (int64_t)h, 2 * (int64_t)w, c / 8)) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_up_fwd, row_grid(n, 2 * h, 2 * w, c / 8), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const bf16*)in, n, h, w, c, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 lgi_status lgi_unet_upsample_bwd(const void* dout, int32_t n, int32_t h, int32_t w, int32_t c, void* din, void* stream) {
-    if (!dout || !din || c % 8) return LGI_ERR_INVALID_ARG;
-    const int64_t total = (int64_t)n * h * w * (c / 8);
-    hipLaunchKernelGGL(k_up_bwd, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)dout, n,
+    if (!dout || !din || c % 8 || !row_grid_fits(n, h, w, c / 8)) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_up_bwd, row_grid(n, h, w, c / 8), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)dout, n,
                        h, w, c, (bf16*)din);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
