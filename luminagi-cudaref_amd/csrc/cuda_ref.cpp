@@ -2,8 +2,10 @@
 // log lines (src/main.cpp:93-229), built on the liblgi C ABI.
 //
 //   cuda_ref <scene.json | scene.lgis> [out.png] [-o out.png] [--spp N] [--bounces N] [--asset-root DIR]
-//            [--sort] [--gbuffer] [--cam-pos x y z --cam-target x y z] [--batch file]
+//            [--sort] [--no-ray-sort] [--gbuffer] [--cam-pos x y z --cam-target x y z] [--batch file]
 //            [--hdr out.f32] [--rng xorshift]
+// --no-ray-sort (with --sort): the wavefront renderer without its key sort between bounces
+// (same image bits).
 // Exit codes: 1 usage, 2 scene load failure, 3 PNG write failure, 4 GPU failure.
 #include <chrono>
 #include <cstdio>
@@ -42,7 +44,7 @@ int main(int argc, char** argv) {
     const char* pngPath = "output/reference.png";
     std::string assetRoot, batchFile, hdrPath;
     int spp = 64, bounces = 2;
-    bool useSort = false, gbuffer = false, haveCamPos = false, haveCamTgt = false;
+    bool useSort = false, noRaySort = false, gbuffer = false, haveCamPos = false, haveCamTgt = false;
     uint32_t rngFlag = LGI_RNG_PCG32;
     float camPos[3] = {0, 0, 0}, camTgt[3] = {0, 0, 0};
     auto read_vec3 = [&](int& i, float out[3]) -> bool {
@@ -59,6 +61,7 @@ int main(int argc, char** argv) {
         else if (a == "--asset-root" && i + 1 < argc) assetRoot = argv[++i];
         else if (a == "-o" && i + 1 < argc) pngPath = argv[++i];
         else if (a == "--sort") useSort = true;
+        else if (a == "--no-ray-sort") noRaySort = true;
         else if (a == "--gbuffer") gbuffer = true;
         else if (a == "--cam-pos") haveCamPos = read_vec3(i, camPos);
         else if (a == "--cam-target") haveCamTgt = read_vec3(i, camTgt);
@@ -94,7 +97,8 @@ int main(int argc, char** argv) {
         std::printf("camera override: pos (%.2f,%.2f,%.2f) -> target (%.2f,%.2f,%.2f)\n", camPos[0], camPos[1],
                     camPos[2], camTgt[0], camTgt[1], camTgt[2]);
     }
-    const uint32_t flags = rngFlag | (useSort ? LGI_RENDER_SORTED : 0u);
+    const uint32_t flags =
+        rngFlag | (useSort ? LGI_RENDER_SORTED : 0u) | (useSort && noRaySort ? LGI_RENDER_NO_RAY_SORT : 0u);
     auto t0 = std::chrono::steady_clock::now();
     if (lgi_bvh_build(ctx) != LGI_OK) return fail_gpu(ctx, "bvh build");
     int64_t nn = 0;

@@ -203,3 +203,16 @@ def test_tools_eval_checkpoints_table(gpu, tmp_path, capsys):
     assert rows[1][0] == "a.pt" and rows[1][1] > 0
     assert rows[1][2] == pytest.approx(M.psnr(den, gt), abs=1e-9)
     assert rows[1][3] == pytest.approx(M.ssim(den, gt), abs=1e-12)
+
+
+def test_cli_renderers_write_the_same_png(gpu, tmp_path):
+    """cuda_ref with and without --sort, and with --sort --no-ray-sort: one image."""
+    _, scene = small_scene(tmp_path)
+    outs = []
+    for k, extra in enumerate(([], ["--sort"], ["--sort", "--no-ray-sort"])):
+        out = tmp_path / f"r{k}.png"
+        r = subprocess.run([str(CLI), str(scene), "-o", str(out), "--spp", "4"] + extra, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(read_png(out))
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
