@@ -233,3 +233,25 @@ def test_frame_graph_replays_match_eager_renderers(gpu, flags):
         for a, b in zip(eager, bufs):
             assert torch.equal(a, b), f"replay {k}"
         _ = gpu_accumulate(ctx, cam, 2, 1)  # eager work between replays
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (9, 17), (65, 1)])
+def test_tiny_and_ragged_images(gpu, w, h):
+    """Image sizes that leave partial 8x8 tiles (and a single pixel): the inline renderer, the
+    wavefront renderer with and without its ray sort, and the tile renderer over every tile
+    give the oracle's sums."""
+    import torch
+    import lgi
+    ctx, osc, s = make_pair(seed=5)
+    cam = small_camera(s["camera"], w, h)
+    o = osc.render(8, 2, camera=cam)
+    sums = [gpu_accumulate(ctx, cam, 8, 2, flags=f)[0].cpu().numpy()
+            for f in (0, lgi.RENDER_SORTED, lgi.RENDER_SORTED | lgi.RENDER_NO_RAY_SORT)]
+    tiles = list(range(((w + 7) // 8) * ((h + 7) // 8)))
+    t_sum, t_comp = torch.zeros(w * h * 3, device="cuda"), torch.zeros(w * h * 3, device="cuda")
+    ctx.render_tiles(tiles, 8, t_sum, t_comp, 2, flags=lgi.RENDER_SORTED, camera=cam)
+    torch.cuda.synchronize()
+    sums.append(t_sum.cpu().numpy())
+    for k, g in enumerate(sums):
+        assert np.array_equal(g, sums[0]), f"renderer {k} differs from the inline one"
+    assert_close_images(sums[0], o["sum"], f"{w}x{h}")
