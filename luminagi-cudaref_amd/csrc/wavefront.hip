@@ -116,8 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
     const float jy = rng.next();
     Ray ray = primary_ray(c, px, py, jx, jy);
     W.rng[p] = rng.g.state;
-    W.T[3 * p] = 1.f; W.T[3 * p + 1] = 1.f; W.T[3 * p + 2] = 1.f;
-    W.L[3 * p] = 0.f; W.L[3 * p + 1] = 0.f; W.L[3 * p + 2] = 0.f;
+    W.L[3 * p] = 0.f; W.L[3 * p + 1] = 0.f; W.L[3 * p + 2] = 0.f;  // throughput starts at 1: see k_wf_shade
     r.o = make_float4(ray.o.x, ray.o.y, ray.o.z, __int_as_float((int)p));
     r.d = make_float4(ray.d.x, ray.d.y, ray.d.z, 1e30f);
     q[i] = r;
@@ -358,9 +357,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         if (dot(N, d) > 0.0f) N = mul(N, -1.0f);
         Pt = add(v3(e.o.x, e.o.y, e.o.z), mul(d, h.t));
         albedo = sample_albedo(W.P, mat, tu, tv);
-        const float* T = W.T + 3 * (int64_t)p;
+        // a path's throughput is exactly 1 until its first bounce, so bounce 0 does not read it
         float* Tb = W.Tb + 3 * (int64_t)p;
-        Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
+        if (bounce == 0) {
+            Tb[0] = 1.f; Tb[1] = 1.f; Tb[2] = 1.f;
+        } else {
+            const float* T = W.T + 3 * (int64_t)p;
+            Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
+        }
     }
     // NEE candidates: flags and contributions in fixed slots i*nl + li (compacted afterwards);
     // the shading point and normal for the shadow rays
@@ -377,8 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
             ncand += cand ? 1u : 0u;
         }
-        if (ncand) {
-            W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, 0.f);
+        if (ncand) {  // w: the path id, for k_wf_resolve
+            W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, __int_as_float(p));
             W.vN[i] = make_float4(N.x, N.y, N.z, 0.f);
         }
         // continuation
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         Ray next;
         if (cont) {
             float* T = W.T + 3 * (int64_t)p;
-            V3 Tv = v3(T[0], T[1], T[2]);
+            V3 Tv = bounce == 0 ? v3(1.f, 1.f, 1.f) : v3(T[0], T[1], T[2]);
             Rng rng;
             rng.pcg = true;
             const uint32_t pix = (uint32_t)(p % W.npx);
@@ -466,19 +470,25 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, i
     }
 }
 
-// L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313).
-__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, const QRay* q, const float4* hits, int n,
-                                                       const uint32_t* n_dev) {
+// L += Tb * sum of unoccluded contributions, lights in order (src/pathtracer.cu:313). A slot
+// without an unoccluded candidate would add Tb * 0 = +0 to a non-negative L, which changes no
+// bit, so it is skipped; the others find their path id in vP.w, written by the shading kernel
+// for exactly the slots that have candidates.
+__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const uint32_t* n_dev) {
     if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int p = __float_as_int(q[i].o.w);
-    if (p < 0 || __float_as_int(hits[i].w) < 0) return;
     V3 color = v3(0.f, 0.f, 0.f);
+    bool any = false;
     for (int li = 0; li < W.nl; ++li) {
         const int64_t k = (int64_t)i * W.nl + li;
-        if (W.nee[k] == 1) color = add(color, v3(W.contrib[3 * k], W.contrib[3 * k + 1], W.contrib[3 * k + 2]));
+        if (W.nee[k] == 1) {
+            color = add(color, v3(W.contrib[3 * k], W.contrib[3 * k + 1], W.contrib[3 * k + 2]));
+            any = true;
+        }
     }
+    if (!any) return;
+    const int p = __float_as_int(W.vP[i].w);
     float* L = W.L + 3 * (int64_t)p;
     const float* Tb = W.Tb + 3 * (int64_t)p;
     V3 Lv = add(v3(L[0], L[1], L[2]), had(v3(Tb[0], Tb[1], Tb[2]), color));
@@ -654,7 +664,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             launch_trace<true>(trace_grid(n_max * nl), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr,
                                n_max * nl, (float4*)nullptr, W.nee, cnt + 3, spill, shadow, cnt + 1);
             if ((ms = mark(false)) != LGI_OK) return ms;
-            hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev);
+            hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
             if (b == P.bounces) break;
             // compact the continuations, sort them by origin/direction key, gather in key order
             // (without the sort, the compacted queue in cur is the next bounce's queue)
