@@ -433,11 +433,24 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     }
 }
 
-// Continuations of block b (flag ncont) to blk_off[b] + their rank in the block, with their
-// sort keys and identity indices (order-preserving).
-__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint32_t* src_key, const uint8_t* flag,
-                                                       int n, const uint32_t* n_dev, const uint32_t* blk_off, QRay* dst,
-                                                       uint32_t* dst_key, int32_t* dst_idx) {
+// Continuation rays of block b (flag ncont) to blk_off[b] + their rank in the block
+// (order-preserving): the next bounce's queue when the rays are not sorted.
+__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint8_t* flag, int n,
+                                                       const uint32_t* n_dev, const uint32_t* blk_off, QRay* dst) {
+    __shared__ uint32_t s_w[kBlock / 64];
+    if (n_dev) n = min(n, (int)*n_dev);
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
+    const uint32_t r = block_excl_scan(f, s_w);
+    if (f) dst[blk_off[blockIdx.x] + r] = src[i];
+}
+
+// Sort input of the continuations of block b: their keys and source slots, to blk_off[b] + their
+// rank in the block. The rays themselves stay where shading wrote them; the gather after the
+// sort reads them from there, so each continuation ray is copied once.
+__global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_key, const uint8_t* flag, int n,
+                                                            const uint32_t* n_dev, const uint32_t* blk_off,
+                                                            uint32_t* dst_key, int32_t* dst_slot) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -445,11 +458,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const ui
     const uint32_t r = block_excl_scan(f, s_w);
     if (!f) return;
     const uint32_t o = blk_off[blockIdx.x] + r;
-    dst[o] = src[i];
-    if (dst_key) {
-        dst_key[o] = src_key[i];
-        dst_idx[o] = (int32_t)o;
-    }
+    dst_key[o] = src_key[i];
+    dst_slot[o] = i;
 }
 
 // NEE candidate ids j = slot * nl + light of block b to blk_off[b] + their rank in the block
@@ -666,14 +676,17 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
             if (b == P.bounces) break;
-            // compact the continuations, sort them by origin/direction key, gather in key order
-            // (without the sort, the compacted queue in cur is the next bounce's queue)
-            hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, keys_tmp, (const uint8_t*)ncont,
-                               n_max, n_dev, (const uint32_t*)blk_n, cur, sort ? keys : nullptr, idx);
+            // The next bounce's queue goes to cur (this bounce's queue is no longer read): with the
+            // sort, (key, slot) pairs of the continuations are compacted, sorted by key, and the
+            // rays gathered from shading's output in key order; without it, the rays are compacted.
             if (sort) {
+                hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, keys_tmp, (const uint8_t*)ncont,
+                                   n_max, n_dev, (const uint32_t*)blk_n, keys, idx);
                 const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
-                hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, cur, sorted, other, n_max, cnt + 2);
-                std::swap(cur, other);
+                hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, other, sorted, cur, n_max, cnt + 2);
+            } else {
+                hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, (const uint8_t*)ncont, n_max,
+                                   n_dev, (const uint32_t*)blk_n, cur);
             }
             n_dev = cnt + 2;
         }
