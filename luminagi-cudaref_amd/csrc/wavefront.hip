@@ -334,6 +334,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
                                                      const uint32_t* n_dev, int bounce, QRay* nq, uint32_t* nkeys,
                                                      uint8_t* ncont, uint32_t* blk_s, uint32_t* blk_n) {
     if (n_dev) n = min(n, (int)*n_dev);
+    if ((int)blockIdx.x * kBlock >= n) {  // whole block past the count: zero counts for the scans
+        if (threadIdx.x == 0) {
+            blk_s[blockIdx.x] = 0;
+            blk_n[blockIdx.x] = 0;
+        }
+        return;
+    }
     const int i = blockIdx.x * kBlock + threadIdx.x;
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
@@ -439,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const ui
                                                        const uint32_t* n_dev, const uint32_t* blk_off, QRay* dst) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
+    if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
     const uint32_t r = block_excl_scan(f, s_w);
@@ -453,6 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_
                                                             uint32_t* dst_key, int32_t* dst_slot) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
+    if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
     const uint32_t r = block_excl_scan(f, s_w);
@@ -468,6 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, i
                                                            const uint32_t* blk_off, uint32_t* ids) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
+    if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
     const int i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t c = 0;
     if (i < n)
