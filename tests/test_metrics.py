@@ -101,3 +101,30 @@ def test_gpu_heatmap_matches_host(gpu, h, w):
     for gain in (4.0, 1.5, 0.0, -2.0, 1e3):
         got = metrics.heatmap(ta, tb, gain).cpu().numpy()
         assert np.array_equal(got, M.heatmap(a, b, gain)), gain
+
+
+GOLDEN_HEAT = __import__("pathlib").Path(__file__).resolve().parent / "golden" / "heatmap_golden.npz"
+
+
+def test_heatmap_oracle_reproduces_reference_output():
+    """Crops of the reference's own diff-tool output (docs/heatmap.png made by tools/diff.py from
+    docs/engine.png and docs/reference.png; tests/golden/make_heatmap_golden.py)."""
+    d = np.load(GOLDEN_HEAT)
+    for k in range(3):
+        e, r, h = d[f"c{k}_engine"], d[f"c{k}_reference"], d[f"c{k}_heatmap"]
+        assert np.array_equal(M.heatmap(e, r), h), k
+
+
+def test_heatmap_oracle_reproduces_reference_output_full(reference_dir):
+    e, r, h = (load(reference_dir / "docs" / f"{n}.png") for n in ("engine", "reference", "heatmap"))
+    assert np.array_equal(M.heatmap(e, r), h)
+
+
+@pytest.mark.gpu
+def test_gpu_heatmap_reproduces_reference_output(gpu):
+    import torch
+    from lgi import metrics
+    d = np.load(GOLDEN_HEAT)
+    for k in range(3):
+        e, r = (torch.from_numpy(d[f"c{k}_{n}"]).cuda() for n in ("engine", "reference"))
+        assert np.array_equal(metrics.heatmap(e, r).cpu().numpy(), d[f"c{k}_heatmap"]), k
