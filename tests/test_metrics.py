@@ -137,3 +137,23 @@ def test_metrics_reproduce_reference_diff_numbers(reference_dir):
     assert round(M.psnr(e, r), 1) == 21.8
     assert round(M.ssim(e, r), 3) == 0.631
     assert round(M.mad(e, r), 1) == 13.4
+
+
+@pytest.mark.gpu
+def test_gpu_read_probe_times_a_launch(gpu):
+    """lgi_probe_read_bw with ms: the launch is bracketed by events and waited for; a pass over
+    a larger buffer takes longer than one over a smaller buffer."""
+    import ctypes as C
+    import torch
+    from lgi import _abi
+    L = _abi.lib()
+    sink = torch.zeros(4, dtype=torch.int32, device="cuda")
+    times = []
+    for mib in (16, 1024):
+        buf = torch.ones(mib << 18, dtype=torch.int32, device="cuda")
+        ms = C.c_float(-1.0)
+        st = L.lgi_probe_read_bw(C.c_void_p(buf.data_ptr()), buf.numel() * 4, 2, 2048, C.c_void_p(sink.data_ptr()),
+                                 None, C.byref(ms))
+        assert st == 0 and ms.value > 0.0
+        times.append(ms.value)
+    assert times[1] > times[0]
