@@ -154,8 +154,19 @@ LGI_API lgi_status lgi_bvh_node_count(const lgi_ctx* ctx, int64_t* n_nodes);
 /* nodes: n_nodes entries in BFS order (root 0); tri_perm[i] = original index of the
  * triangle now at position i; keys: sorted 30-bit Morton codes. Any output may be null. */
 LGI_API lgi_status lgi_bvh_get(lgi_ctx* ctx, lgi_bvh_node* nodes, int32_t* tri_perm, uint32_t* keys);
-/* Milliseconds of the last GPU build (device time, HIP events). */
+/* Traversal tree (no reference counterpart). The tracers traverse a copy of the reference tree
+ * whose topology above the leaves is restructured to lower its surface-area cost (7-leaf
+ * treelets, Karras & Aila 2013): same leaves and triangle order, every box the exact union of
+ * its children's, so the closest hit equals the reference tree's except where visit order
+ * decides it (exact distance ties, boxes grazed within rounding). passes 0..8 (default 2; 0 =
+ * traverse the reference tree itself) takes effect at the next lgi_bvh_build. */
+LGI_API lgi_status lgi_bvh_set_treelet_passes(lgi_ctx* ctx, int32_t passes);
+/* nodes: n_nodes entries, the traversal tree in the slots of lgi_bvh_get's array (root 0). */
+LGI_API lgi_status lgi_bvh_get_traversal(lgi_ctx* ctx, lgi_bvh_node* nodes);
+/* Milliseconds of the last GPU build (device time, HIP events), restructuring included. */
 LGI_API lgi_status lgi_bvh_last_build_ms(const lgi_ctx* ctx, float* ms);
+/* Milliseconds of the last build's traversal-tree restructuring (0 when it did not run). */
+LGI_API lgi_status lgi_bvh_last_treelet_ms(const lgi_ctx* ctx, float* ms);
 
 /* ---- rendering (RenderSceneCUDA / RenderSceneCUDASorted) ---- */
 /* Host-buffer render of the scene camera (or *cam when non-null): rgb = W*H*3 u8

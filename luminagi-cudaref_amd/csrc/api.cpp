@@ -54,6 +54,8 @@ lgi_status ensure_gpu(lgi_ctx* ctx) {
     LGI_HIP(ctx, hipSetDevice(ctx->device));
     LGI_HIP(ctx, hipEventCreate(&ctx->ev0));
     LGI_HIP(ctx, hipEventCreate(&ctx->ev1));
+    LGI_HIP(ctx, hipEventCreate(&ctx->ev_tree0));
+    LGI_HIP(ctx, hipEventCreate(&ctx->ev_tree1));
     ctx->gpu_ready = true;
     return LGI_OK;
 }
@@ -306,6 +308,8 @@ void lgi_destroy(lgi_ctx* ctx) {
             if (e.first) (void)hipFree(e.first);
         if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
         if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+        if (ctx->ev_tree0) (void)hipEventDestroy(ctx->ev_tree0);
+        if (ctx->ev_tree1) (void)hipEventDestroy(ctx->ev_tree1);
         for (auto e : ctx->kev)
             if (e) (void)hipEventDestroy(e);
         if (ctx->graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)ctx->graph_exec);
@@ -471,6 +475,28 @@ lgi_status lgi_bvh_get(lgi_ctx* ctx, lgi_bvh_node* nodes, int32_t* tri_perm, uin
     if (nodes) LGI_HIP(ctx, hipMemcpy(nodes, d.nodes, d.n_nodes * sizeof(lgi_bvh_node), hipMemcpyDeviceToHost));
     if (tri_perm) LGI_HIP(ctx, hipMemcpy(tri_perm, d.perm, d.n_tris * 4, hipMemcpyDeviceToHost));
     if (keys) LGI_HIP(ctx, hipMemcpy(keys, d.keys, d.n_tris * 4, hipMemcpyDeviceToHost));
+    return LGI_OK;
+}
+
+lgi_status lgi_bvh_set_treelet_passes(lgi_ctx* ctx, int32_t passes) {
+    if (!ctx) return LGI_ERR_INVALID_ARG;
+    if (passes < 0 || passes > 8) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "treelet passes must be 0..8");
+    ctx->bvh_treelet_passes = passes;
+    return LGI_OK;
+}
+
+lgi_status lgi_bvh_get_traversal(lgi_ctx* ctx, lgi_bvh_node* nodes) {
+    if (!ctx || !nodes) return LGI_ERR_INVALID_ARG;
+    if (!ctx->bvh_ready) return lgi_set_err(ctx, LGI_ERR_NO_BVH, "no BVH");
+    const DevScene& d = ctx->dev;
+    LGI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LGI_HIP(ctx, hipMemcpy(nodes, d.trav_nodes, d.n_nodes * sizeof(lgi_bvh_node), hipMemcpyDeviceToHost));
+    return LGI_OK;
+}
+
+lgi_status lgi_bvh_last_treelet_ms(const lgi_ctx* ctx, float* ms) {
+    if (!ctx || !ms) return LGI_ERR_INVALID_ARG;
+    *ms = ctx->last_treelet_ms;
     return LGI_OK;
 }
 

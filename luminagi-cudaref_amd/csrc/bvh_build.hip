@@ -15,12 +15,15 @@
 //   7. bottom-up AABB refit, one launch per level from the deepest up (no atomics); leaf
 //      boxes from the triangle vertices in order, parent box = left box then strict-union
 //      right box (src/bvh.cpp:96-103, 115-117)
-//   8. traversal layout: TNode (child boxes in the parent), TriV (v0, e1, e2), TriS.
+//   8. treelet restructuring of a copy of the tree for traversal (below); the reference node
+//      array itself stays as built
+//   9. traversal layout: TNode (child boxes in the parent), TriV (v0, e1, e2), TriS.
 #include <algorithm>
 #include <cstring>
 
 #include "lgi_internal.h"
 #include "lgi_math.h"
+#include "trace_kernels.h"
 
 namespace {
 
@@ -29,6 +32,10 @@ constexpr int kItems = 8;
 constexpr int kTile = kBlock * kItems;   // keys per radix-sort block
 constexpr int kDepthShift = 26;          // BFS key = depth << 26 | first
 constexpr int kMaxLevels = 32;           // reference depth <= 30 (one split per Morton bit) + leaf
+// Deepest tree the traversal stacks (kStackMax entries, one push per internal level) accept;
+// the reference tree never exceeds it (a deeper one is rejected at step 6).
+constexpr int kMaxTraversalDepth = kMaxLevels - 1;
+static_assert(kMaxTraversalDepth <= kStackMax, "one stack entry per internal level");
 
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
@@ -396,6 +403,219 @@ __global__ __launch_bounds__(kBlock) void k_level_starts(const uint32_t* bfs_key
     if (p == m - 1) level_start[d + 1] = m;
 }
 
+// ------------------------------------------------------------------ treelet restructuring
+// The traversal tree is a copy of the reference tree whose topology above the leaves is
+// rebuilt treelet by treelet to lower its surface-area cost, after Karras and Aila, "Fast
+// Parallel Construction of High-Quality Bounding Volume Hierarchies" (HPG 2013): a treelet
+// is a node plus descendants, grown by repeatedly opening the treelet leaf of largest surface
+// area until it has 7 leaves; dynamic programming over the 127 leaf subsets finds the binary
+// tree over those leaves of least cost, C(S) = area(S) * kSahNode + min over splits of
+// C(P) + C(S \ P), and the treelet's internal node slots are rewired to it when it is cheaper.
+// Leaves (triangle ranges) never change, and every box stays the exact FP32 union of its
+// children's, so a ray enters a leaf's ancestors whenever it enters the leaf, and the closest
+// hit is the same as on the reference tree except where visit order decides it: exact
+// distance ties, and a hit inside a box the slab test rounds to missed.
+// Treelet roots are processed one BFS level of the reference tree per launch, deepest first:
+// a root's treelet lies inside its own subtree, so the treelets of one launch are disjoint
+// and each launch sees its subtrees already restructured. One wave per treelet.
+constexpr float kSahNode = 1.2f, kSahTri = 1.0f;
+constexpr int kTreeletLeaves = 7;
+constexpr int kSubsets = 1 << kTreeletLeaves;
+
+__device__ __forceinline__ float half_area(const float* mn, const float* mx) {
+    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Subtree cost of every node of BFS level [p0, p1), children first (the level below ran before).
+__global__ __launch_bounds__(kBlock) void k_sah_level(const lgi_bvh_node* nodes, int p0, int p1, float* cost) {
+    const int p = p0 + blockIdx.x * kBlock + threadIdx.x;
+    if (p >= p1) return;
+    const lgi_bvh_node nd = nodes[p];
+    const float a = half_area(nd.bb_min, nd.bb_max);
+    cost[p] = nd.tri_count > 0 ? a * kSahTri * (float)nd.tri_count : a * kSahNode + cost[nd.left] + cost[nd.right];
+}
+
+// Restructures the treelet rooted at order[p0 + blockIdx.x] (p0 + blockIdx.x when order is
+// null); the subtree cost is kept current for every node it rewires.
+__global__ __launch_bounds__(64) void k_treelet(lgi_bvh_node* nodes, const int32_t* order, int p0, float* cost) {
+    const int p = order ? order[p0 + blockIdx.x] : p0 + blockIdx.x;  // one treelet root per 64-thread block
+    if (nodes[p].tri_count > 0) return;
+    __shared__ int s_leaf[kTreeletLeaves], s_int[kTreeletLeaves - 1], s_n;
+    __shared__ float s_mn[kSubsets][3], s_mx[kSubsets][3], s_c[kSubsets];
+    __shared__ uint8_t s_part[kSubsets];
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        // the subtrees below were restructured already: refresh this node's cost first
+        const lgi_bvh_node nd = nodes[p];
+        cost[p] = half_area(nd.bb_min, nd.bb_max) * kSahNode + cost[nd.left] + cost[nd.right];
+        int nl = 2, ni = 1;
+        s_leaf[0] = nd.left;
+        s_leaf[1] = nd.right;
+        s_int[0] = p;
+        while (nl < kTreeletLeaves) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int i = 0; i < nl; ++i) {
+                const lgi_bvh_node c = nodes[s_leaf[i]];
+                if (c.tri_count > 0) continue;
+                const float a = half_area(c.bb_min, c.bb_max);
+                if (a > ba) { ba = a; best = i; }
+            }
+            if (best < 0) break;
+            const int x = s_leaf[best];
+            s_int[ni++] = x;
+            s_leaf[best] = nodes[x].left;
+            s_leaf[nl++] = nodes[x].right;
+        }
+        s_n = nl;
+    }
+    __syncthreads();
+    const int nl = s_n;
+    if (nl < 3) return;  // two leaves have one tree (uniform: s_n is shared)
+    const int ns = 1 << nl;
+    if (lane < nl) {
+        const lgi_bvh_node c = nodes[s_leaf[lane]];
+        const int s1 = 1 << lane;
+        for (int k = 0; k < 3; ++k) {
+            s_mn[s1][k] = c.bb_min[k];
+            s_mx[s1][k] = c.bb_max[k];
+        }
+        s_c[s1] = cost[s_leaf[lane]];
+    }
+    __syncthreads();
+    for (int sub = lane; sub < ns; sub += 64) {  // boxes of every subset of two or more leaves
+        if (__popc(sub) < 2) continue;
+        float mn[3], mx[3];
+        const int first = __ffs(sub) - 1;
+        for (int k = 0; k < 3; ++k) { mn[k] = s_mn[1 << first][k]; mx[k] = s_mx[1 << first][k]; }
+        for (int b = sub & (sub - 1); b; b &= b - 1) {
+            const int i = __ffs(b) - 1;
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = fminf(mn[k], s_mn[1 << i][k]);
+                mx[k] = fmaxf(mx[k], s_mx[1 << i][k]);
+            }
+        }
+        for (int k = 0; k < 3; ++k) { s_mn[sub][k] = mn[k]; s_mx[sub][k] = mx[k]; }
+    }
+    __syncthreads();
+    for (int size = 2; size <= nl; ++size) {
+        for (int sub = lane; sub < ns; sub += 64) {
+            if (__popc(sub) != size) continue;
+            const int low = sub & -sub;
+            float best = 3.4e38f;
+            int bp = low;
+            for (int q = (sub - 1) & sub; q > 0; q = (q - 1) & sub) {  // splits with `low` on the left
+                if (!(q & low)) continue;
+                const float v = s_c[q] + s_c[sub ^ q];
+                if (v < best) { best = v; bp = q; }
+            }
+            s_c[sub] = half_area(s_mn[sub], s_mx[sub]) * kSahNode + best;
+            s_part[sub] = (uint8_t)bp;
+        }
+        __syncthreads();
+    }
+    if (lane != 0 || !(s_c[ns - 1] < cost[p] * (1.0f - 1e-6f))) return;
+    // rewire: internal slot s_int[0] = p takes the full set, the others take the inner splits
+    int st_sub[kTreeletLeaves], st_node[kTreeletLeaves], sp = 0, next = 1;
+    st_sub[sp] = ns - 1;
+    st_node[sp++] = p;
+    while (sp > 0) {
+        --sp;
+        const int sub = st_sub[sp], idx = st_node[sp];
+        const int l = s_part[sub], r = sub ^ l;
+        int ch[2];
+        const int parts[2] = {l, r};
+        for (int k = 0; k < 2; ++k) {
+            if (__popc(parts[k]) == 1) {
+                ch[k] = s_leaf[__ffs(parts[k]) - 1];
+            } else {
+                ch[k] = s_int[next++];
+                st_sub[sp] = parts[k];
+                st_node[sp++] = ch[k];
+            }
+        }
+        lgi_bvh_node& nd = nodes[idx];
+        nd.left = ch[0];
+        nd.right = ch[1];
+        for (int k = 0; k < 3; ++k) {
+            nd.bb_min[k] = s_mn[sub][k];
+            nd.bb_max[k] = s_mx[sub][k];
+        }
+        cost[idx] = s_c[sub];
+    }
+}
+
+// Breadth-first levels of the current tree in one workgroup: order[] lists the slots level by
+// level (children of a level in the order of their parents), level_start[l] is where level l
+// begins, level_start[levels] = m. Deterministic (block scans, no atomics). A structure that
+// is not a tree of m slots with at most max_levels levels stops early with *levels = -1.
+constexpr int kBfsThreads = 1024;
+__global__ __launch_bounds__(kBfsThreads) void k_bfs_levels(const lgi_bvh_node* nodes, int m, int max_levels,
+                                                            int32_t* order, int32_t* level_start, int32_t* levels) {
+    __shared__ uint32_t s_wave[kBfsThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        order[0] = 0;
+        level_start[0] = 0;
+    }
+    __syncthreads();
+    int a = 0, b = 1, nlev = 0;
+    bool ok = true;
+    while (a < b) {
+        if (nlev == max_levels) { ok = false; break; }
+        level_start[nlev++] = a;  // same value from every thread
+        int out = b;
+        for (int base = a; base < b; base += kBfsThreads) {
+            const int i = base + tid;
+            const int x = i < b ? order[i] : -1;
+            const uint32_t inner = (x >= 0 && nodes[x].tri_count == 0) ? 1u : 0u;
+            uint32_t v = inner;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(v, o, 64);
+                if (lane >= o) v += y;
+            }
+            if (lane == 63) s_wave[wave] = v;
+            __syncthreads();
+            uint32_t pre = 0, total = 0;
+            for (int w = 0; w < kBfsThreads / 64; ++w) {
+                pre += w < wave ? s_wave[w] : 0u;
+                total += s_wave[w];
+            }
+            const uint32_t rank = pre + v - inner;
+            if ((int64_t)out + 2 * (int64_t)total > m) { ok = false; break; }  // uniform
+            if (inner) {
+                order[out + 2 * rank] = nodes[x].left;
+                order[out + 2 * rank + 1] = nodes[x].right;
+            }
+            out += 2 * (int)total;
+            __syncthreads();
+        }
+        if (!ok) break;
+        a = b;
+        b = out;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        level_start[nlev] = b;
+        *levels = (ok && b == m) ? nlev : -1;
+    }
+}
+
+// Traversal numbering of the restructured tree: its internal nodes in breadth-first order
+// (order[] from k_bfs_levels), as the reference tree's are in its own array.
+__global__ __launch_bounds__(kBlock) void k_bfs_flags(const lgi_bvh_node* nodes, const int32_t* order, int m,
+                                                      uint32_t* flags) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) flags[i] = nodes[order[i]].tri_count == 0 ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bfs_index(const lgi_bvh_node* nodes, const int32_t* order, int m,
+                                                      const uint32_t* rank, uint32_t* idx) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m && nodes[order[i]].tri_count == 0) idx[order[i]] = rank[i];
+}
+
 // ------------------------------------------------------------------ traversal layout
 __global__ __launch_bounds__(kBlock) void k_internal_flags(const lgi_bvh_node* nodes, int m, uint32_t* flags,
                                                            uint8_t* leaf_end) {
@@ -592,6 +812,8 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
     hipLaunchKernelGGL(k_gather_verts, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, d.perm, n, sv);
 
     int64_t m = 1;
+    int32_t hls[kMaxLevels + 2];
+    int max_depth = 0;
     if (n == 1) {
         // single triangle: one leaf
         hipLaunchKernelGGL(k_init_single, dim3(1), dim3(1), 0, st, parent, sb.v[0], new_index);
@@ -613,10 +835,8 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
                            level_start);
         hipLaunchKernelGGL(k_leaf_boxes, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, n, (int)m, sb.v[0], rfirst,
                            rlast, splittable, sv, d.nodes);
-        int32_t hls[kMaxLevels + 2];
         LGI_HIP(ctx, hipMemcpyAsync(hls, level_start, sizeof(hls), hipMemcpyDeviceToHost, st));
         LGI_HIP(ctx, hipStreamSynchronize(st));
-        int max_depth = 0;
         while (max_depth + 1 <= kMaxLevels && hls[max_depth + 1] != (int)m) ++max_depth;
         if (max_depth >= kMaxLevels) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "BVH deeper than expected");
         for (int lv = max_depth - 1; lv >= 0; --lv) {
@@ -639,11 +859,69 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
     d.root_is_leaf = (n_int == 0);
     const int64_t n_t = std::max<int64_t>(1, n_int);
     if ((s = dev_alloc(ctx, &d.tnodes, n_t)) != LGI_OK) return s;
+    // the tree the traversal layout is made from: the reference tree, or its restructured copy
+    const lgi_bvh_node* tsrc = d.nodes;
+    const int passes = ctx->bvh_treelet_passes;
+    ctx->last_treelet_ms = 0.f;
+    if (n_int > 1 && passes > 0) {
+        LGI_HIP(ctx, hipEventRecord(ctx->ev_tree0, st));
+        lgi_bvh_node* onodes = (lgi_bvh_node*)S(20, (size_t)m * sizeof(lgi_bvh_node));
+        float* cost = (float*)S(21, (size_t)m * 4);
+        uint32_t* rank = (uint32_t*)S(22, (size_t)m * 4);
+        int32_t* order = (int32_t*)S(24, (size_t)m * 4);
+        constexpr int kBfsLevels = 64;
+        int32_t* lstart = (int32_t*)S(25, (kBfsLevels + 2) * 4);
+        if (!onodes || !cost || !rank || !order || !lstart)
+            return lgi_set_err(ctx, LGI_ERR_HIP, "scratch allocation failed");
+        LGI_HIP(ctx, hipMemcpyAsync(onodes, d.nodes, (size_t)m * sizeof(lgi_bvh_node), hipMemcpyDeviceToDevice, st));
+        for (int lv = max_depth; lv >= 0; --lv)
+            hipLaunchKernelGGL(k_sah_level, dim3(ceil_div(hls[lv + 1] - hls[lv], kBlock)), dim3(kBlock), 0, st, onodes,
+                               hls[lv], hls[lv + 1], cost);
+        // Pass 1 takes its treelet roots by the reference tree's BFS levels (the array order). A
+        // restructured tree no longer follows them (a root of one level may now lie below
+        // another), so each later pass first takes the levels of the current tree.
+        bool tree_ok = true;
+        int32_t lv_host[kBfsLevels + 2];  // level starts, then the level count at [kBfsLevels + 1]
+        const int32_t* lv_order = nullptr;
+        int n_lev = max_depth + 1;
+        std::copy(hls, hls + n_lev + 1, lv_host);
+        // The levels are also taken after the last pass: the traversal uses the restructured tree
+        // only when its breadth-first walk covers the m slots within kMaxTraversalDepth levels
+        // below the root (the traversal stacks hold one entry per internal level).
+        for (int pass = 0; pass <= passes; ++pass) {
+            if (pass > 0) {
+                hipLaunchKernelGGL(k_bfs_levels, dim3(1), dim3(kBfsThreads), 0, st, onodes, (int)m, kBfsLevels, order,
+                                   lstart, lstart + kBfsLevels + 1);
+                LGI_HIP(ctx, hipMemcpyAsync(lv_host, lstart, sizeof(lv_host), hipMemcpyDeviceToHost, st));
+                LGI_HIP(ctx, hipStreamSynchronize(st));
+                n_lev = lv_host[kBfsLevels + 1];
+                lv_order = order;
+                if (n_lev < 1) { tree_ok = false; break; }
+            }
+            if (pass == passes) break;
+            for (int lv = n_lev - 2; lv >= 0; --lv)
+                hipLaunchKernelGGL(k_treelet, dim3(lv_host[lv + 1] - lv_host[lv]), dim3(64), 0, st, onodes, lv_order,
+                                   lv_host[lv], cost);
+        }
+        if (tree_ok && n_lev - 1 <= kMaxTraversalDepth) {
+            hipLaunchKernelGGL(k_bfs_flags, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, onodes, order, (int)m,
+                               rank);
+            hipLaunchKernelGGL(k_scan_tiles, dim3(nsb), dim3(kBlock), 0, st, rank, (int)m, tsums);
+            hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, tsums, nsb, counter);
+            hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(kBlock), 0, st, rank, (int)m, tsums);
+            hipLaunchKernelGGL(k_bfs_index, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, onodes, order, (int)m,
+                               rank, tflags);
+            tsrc = onodes;
+        }
+        LGI_HIP(ctx, hipEventRecord(ctx->ev_tree1, st));
+        LGI_HIP(ctx, hipEventSynchronize(ctx->ev_tree1));
+        LGI_HIP(ctx, hipEventElapsedTime(&ctx->last_treelet_ms, ctx->ev_tree0, ctx->ev_tree1));
+    }
+    d.trav_nodes = tsrc;
     if (n_int == 0)
         hipLaunchKernelGGL(k_tnode_root_leaf, dim3(1), dim3(1), 0, st, d.nodes, d.tnodes);
     else
-        hipLaunchKernelGGL(k_tnodes, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, d.nodes, (int)m, tflags,
-                           d.tnodes);
+        hipLaunchKernelGGL(k_tnodes, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, tsrc, (int)m, tflags, d.tnodes);
     hipLaunchKernelGGL(k_tri_layout, dim3(ceil_div(n, kBlock)), dim3(kBlock), 0, st, d.tri_in, d.perm, n, sv,
                        leaf_end, d.triv, d.tris);
     LGI_HIP(ctx, hipEventRecord(ctx->ev1, st));

@@ -63,6 +63,7 @@ struct DevScene {
     lgi_triangle* tri_in = nullptr;   // original order, AoS 100 B
     lgi_bvh_node* nodes = nullptr;    // BFS order, reference format
     TNode* tnodes = nullptr;
+    const lgi_bvh_node* trav_nodes = nullptr;  // tree the traversal layout was made from (not owned)
     TriV* triv = nullptr;
     TriS* tris = nullptr;
     float* mat_albedo = nullptr;      // 3 per material
@@ -89,8 +90,11 @@ struct lgi_ctx {
     lgi_bvh_node root{};         // host copy of the BVH root (scene box)
     bool bvh_ready = false;
     float last_build_ms = 0.f;
+    int32_t bvh_treelet_passes = 2;  // restructuring passes of the traversal tree (0: reference tree)
     float last_render_ms = 0.f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_tree0 = nullptr, ev_tree1 = nullptr;  // around the traversal-tree restructuring
+    float last_treelet_ms = 0.f;
     // event pairs around each traversal-kernel launch of the last wavefront render
     std::vector<hipEvent_t> kev;
     int kev_used = 0;           // pairs recorded by the last render (0: inline renderer)

@@ -86,6 +86,9 @@ SIGNATURES = {
     "lgi_bvh_node_count": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "lgi_bvh_get": (C.c_int, [_P, _P, _P, _P]),
     "lgi_bvh_last_build_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "lgi_bvh_set_treelet_passes": (C.c_int, [_P, C.c_int32]),
+    "lgi_bvh_last_treelet_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "lgi_bvh_get_traversal": (C.c_int, [_P, _P]),
     "lgi_render": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, _P, _P, _P, _P, _P]),
     "lgi_render_accumulate": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
                                          _P, _P, _P]),

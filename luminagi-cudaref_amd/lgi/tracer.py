@@ -157,9 +157,28 @@ class Context:
         self._c(lib().lgi_bvh_get(self._h, ptr(nodes), ptr(perm), ptr(keys)))
         return nodes, perm, keys
 
+    def set_treelet_passes(self, passes):
+        """Restructuring passes of the tree the tracers traverse (0 = the reference tree itself);
+        takes effect at the next build_bvh()."""
+        self._c(lib().lgi_bvh_set_treelet_passes(self._h, passes))
+
+    def bvh_traversal(self):
+        """The traversal tree: nodes[NODE_DTYPE] in the slots of bvh()'s array (same leaves)."""
+        n = C.c_int64()
+        self._c(lib().lgi_bvh_node_count(self._h, C.byref(n)))
+        nodes = np.zeros(n.value, NODE_DTYPE)
+        self._c(lib().lgi_bvh_get_traversal(self._h, ptr(nodes)))
+        return nodes
+
     def last_build_ms(self):
         ms = C.c_float()
         self._c(lib().lgi_bvh_last_build_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def last_treelet_ms(self):
+        """Device time of the last build's traversal-tree restructuring (0 when it did not run)."""
+        ms = C.c_float()
+        self._c(lib().lgi_bvh_last_treelet_ms(self._h, C.byref(ms)))
         return ms.value
 
     # ------------------------------------------------------------------ render

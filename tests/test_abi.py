@@ -167,3 +167,17 @@ def test_probe_rejects_bad_arguments():
     assert L.lgi_probe_read_bw(C.c_void_p(4096 + 8), 1 << 20, 1, 8, sink, None, None) == 1  # not 16-byte aligned
     assert L.lgi_probe_read_bw(C.c_void_p(4096), 8, 1, 8, sink, None, None) == 1           # under one word
     assert L.lgi_probe_read_bw(C.c_void_p(4096), 1 << 20, 0, 8, sink, None, None) == 1     # no passes
+
+
+def test_treelet_passes_argument_checks():
+    """lgi_bvh_set_treelet_passes accepts 0..8; the traversal tree exists only after a build."""
+    ctx = lgi.Context()
+    for bad in (-1, 9):
+        with pytest.raises(lgi.LgiError) as e:
+            ctx.set_treelet_passes(bad)
+        assert e.value.status == 1  # LGI_ERR_INVALID_ARG
+    for ok in (0, 8, 2):
+        ctx.set_treelet_passes(ok)
+    with pytest.raises(lgi.LgiError) as e:
+        ctx.bvh_traversal()
+    assert e.value.status == 7  # LGI_ERR_NO_BVH
