@@ -126,6 +126,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
 // Closest hit (hits[i] = t, u, v, tri) or any hit (nee[slot] = 2 when occluded) for queue
 // entries [0, n). Lanes whose ray finishes take the next ray from the wave's 64-entry chunk;
 // a wave grabs a new chunk with one atomicAdd when the current one runs out.
+// The queue is split into G = min(8, grid) contiguous segments, one per XCD (workgroups are
+// dispatched to the 8 XCDs round-robin, so block b runs on XCD b % 8): each segment has its
+// own cursor (cursors[16 * g], one 64-byte line each), so the rays one XCD traces at a time
+// are neighbours in queue order (ray-sorted, or tile order) and share its L2. A wave whose
+// segment is drained takes chunks from the following segments.
+constexpr int kCursorStride = 16;  // u32 entries between segment cursors
+constexpr int kSegments = 8;
 // kTop > 0: the first kTop traversal nodes (the top levels: nodes are in breadth-first order)
 // are copied into LDS once per block and read from there; kLds stack entries live in LDS.
 // kMinLanes: phase 1 hands over to the leaf tests once fewer than kMinLanes of the lanes still
@@ -133,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int ti
 template <bool kAny, int kLds, int kTop, int kMinLanes>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
-                                                     float4* hits, uint8_t* nee, uint32_t* counter, int* spill,
+                                                     float4* hits, uint8_t* nee, uint32_t* cursors, int* spill,
                                                      ShadowSrc shadow, const uint32_t* n_dev) {
     if (n_dev) n = min(n, (int)*n_dev);
     __shared__ int s_stack[kLds * kBlock];
@@ -148,9 +155,12 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                                (int)gridDim.x * kBlock};
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // wave-uniform chunk cursor
+    // wave-uniform chunk cursor and segment
     uint32_t cbase = 0, cend = 0;
     bool exhausted = false;
+    const int n_seg = min(kSegments, (int)gridDim.x);
+    int seg = (int)blockIdx.x % n_seg, tried = 0;
+    auto seg_lo = [&](int g) { return (uint32_t)((uint64_t)(uint32_t)n * (uint32_t)g / (uint32_t)n_seg); };
     bool active = false;
     // node: >= 0 internal node to visit, < 0 leaf (~first triangle), kNone when the stack is
     // exhausted; leaf: a parked leaf (0 = none)
@@ -166,13 +176,22 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
             uint32_t avail = cend - cbase;
             uint32_t nb = 0xFFFFFFFFu, nend = 0;
             if (avail < cnt) {
-                if (lane == 0) nb = atomicAdd(counter, 64u);
-                nb = __shfl(nb, 0, 64);
-                if (nb >= (uint32_t)n) {
-                    exhausted = true;
-                    nb = 0xFFFFFFFFu;
-                } else {
-                    nend = min(nb + 64u, (uint32_t)n);
+                for (;;) {  // ends: every failed grab moves to the next segment, G segments at most
+                    uint32_t old = 0;
+                    if (lane == 0) old = atomicAdd(cursors + kCursorStride * seg, 64u);
+                    old = __shfl(old, 0, 64);
+                    const uint32_t s0 = seg_lo(seg), s1 = seg_lo(seg + 1);
+                    if (old < s1 - s0) {
+                        nb = s0 + old;
+                        nend = min(nb + 64u, s1);
+                        break;
+                    }
+                    if (++tried == n_seg) {
+                        exhausted = true;
+                        nb = 0xFFFFFFFFu;
+                        break;
+                    }
+                    seg = seg + 1 == n_seg ? 0 : seg + 1;
                 }
             }
             if (!active) {
@@ -574,10 +593,10 @@ static_assert(kTraceLdsStack >= kStackLdsMin, "spill area sized for kStackMax - 
 
 template <bool kAny>
 void launch_trace(int grid, hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
-                  float4* hits, uint8_t* nee, uint32_t* counter, int* spill, const ShadowSrc& shadow,
+                  float4* hits, uint8_t* nee, uint32_t* cursors, int* spill, const ShadowSrc& shadow,
                   const uint32_t* n_dev) {
     hipLaunchKernelGGL((k_wf_trace<kAny, kTraceLdsStack, kTraceTopNodes, kAny ? kTraceMinLanesAny : kTraceMinLanes>),
-                       dim3(grid), dim3(kBlock), 0, st, tn, n_tn, tv, q, n, hits, nee, counter, spill, shadow, n_dev);
+                       dim3(grid), dim3(kBlock), 0, st, tn, n_tn, tv, q, n, hits, nee, cursors, spill, shadow, n_dev);
 }
 
 int trace_grid(int n) {
@@ -615,7 +634,9 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     int32_t* idx = (int32_t*)S_(55, cap * 4);
     void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
     void* scan_scratch = S_(68, lgi_scan_scratch_bytes(nblk));
-    constexpr int kCounters = 8;  // per bounce
+    // per bounce: [1] shadow rays, [2] continuations, then the closest-hit and the shadow-ray
+    // traversal's segment cursors, kCursorStride apart
+    constexpr int kCounters = kCursorStride * (1 + 2 * kSegments);
     if (P.bounces > 64) return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "too many bounces");
     uint32_t* counters = (uint32_t*)S_(57, (size_t)kCounters * 4 * 65);
     int* spill = (int*)S_(69, (size_t)kStackSpill * kTraceGridMax * kBlock * 4);
@@ -668,10 +689,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         QRay* other = qb;
         const uint32_t* n_dev = nullptr;  // bounce 0: all n_max entries
         for (int b = 0; b <= P.bounces; ++b) {
-            uint32_t* cnt = counters + kCounters * b;  // [0] chunk cursor, [1] shadow rays, [2] continuations, [3] cursor
+            uint32_t* cnt = counters + kCounters * b;
             if ((ms = mark(true)) != LGI_OK) return ms;
             launch_trace<false>(trace_grid(n_max), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, hits,
-                                (uint8_t*)nullptr, cnt + 0, spill, shadow, n_dev);
+                                (uint8_t*)nullptr, cnt + kCursorStride, spill, shadow, n_dev);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
                                sort ? keys_tmp : nullptr, ncont, blk_s, blk_n);
@@ -682,7 +703,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                                (const uint32_t*)blk_s, sids);
             if ((ms = mark(true)) != LGI_OK) return ms;
             launch_trace<true>(trace_grid(n_max * nl), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr,
-                               n_max * nl, (float4*)nullptr, W.nee, cnt + 3, spill, shadow, cnt + 1);
+                               n_max * nl, (float4*)nullptr, W.nee, cnt + kCursorStride * (1 + kSegments), spill,
+                               shadow, cnt + 1);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
             if (b == P.bounces) break;
