@@ -41,8 +41,20 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1):
-    """Oracle render of every row_step-th row at spp samples; returns (Msamples/s, bytes/sample, counters, s)."""
+def trace_bytes_per_sample(c, samples):
+    """Algorithmic traversal bytes per sample (32 B per node visit, 36 B per triangle test) and
+    the frame's (plus 40 B per closest hit for its shading record)."""
+    trace = (BYTES_NODE * c["node_visits"] + BYTES_TRI * c["tri_tests"]) / samples
+    return trace, trace + BYTES_HIT * c["closest_hits"] / samples
+
+
+def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1, trav_nodes=None):
+    """Oracle render (reference BVH and traversal) of every row_step-th row at spp samples, timed:
+    the CPU baseline. With trav_nodes (the GPU's traversal tree: the same leaves, restructured
+    above them) the same rows are rendered again through that tree, untimed, for the node-visit
+    and triangle-test counts of the tree the GPU traverses. Returns (Msamples/s, counters on the
+    reference tree, counters on the traversal tree, seconds, samples, BVH build seconds)."""
+    import numpy as np
     from oracle import oracle as orc
     osc = orc.Scene(scene["tris"], scene["materials"][:, 0, :], scene["points"], scene["sun"], cam)
     t0 = time.perf_counter()
@@ -50,13 +62,15 @@ def cpu_baseline(scene, cam, bounces, row_step, threads, spp=1):
     dt = time.perf_counter() - t0
     rows = (cam.height + row_step - 1) // row_step
     samples = rows * cam.width * spp
-    c = out["counters"]
-    trace_bytes = (BYTES_NODE * c["node_visits"] + BYTES_TRI * c["tri_tests"]) / samples
-    bytes_per_sample = trace_bytes + BYTES_HIT * c["closest_hits"] / samples
+    c_ref = out["counters"]
+    c_trav = c_ref
+    if trav_nodes is not None:
+        osc.nodes = np.ascontiguousarray(trav_nodes)
+        c_trav = osc.render(spp, bounces, row_step=row_step, threads=threads)["counters"]
     t0 = time.perf_counter()
     orc.bvh_build(scene["tris"])
     bvh_s = time.perf_counter() - t0
-    return samples / dt / 1e6, bytes_per_sample, trace_bytes, c, dt, samples, bvh_s
+    return samples / dt / 1e6, c_ref, c_trav, dt, samples, bvh_s
 
 
 def read_bw_probe(stream):
@@ -310,10 +324,13 @@ def main():
     }
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
-        cpu_msps, bps, trace_bps, counters, cpu_s, cpu_samples, bvh_s = cpu_baseline(
-            scene, cam, args.bounces, args.cpu_row_step, threads, args.cpu_spp)
-        # dominant kernel = BVH traversal: its algorithmic bytes per frame over its summed
-        # launch time per frame (HIP events, live)
+        cpu_msps, c_ref, counters, cpu_s, cpu_samples, bvh_s = cpu_baseline(
+            scene, cam, args.bounces, args.cpu_row_step, threads, args.cpu_spp, trav_nodes=ctx.bvh_traversal())
+        trace_bps, bps = trace_bytes_per_sample(counters, cpu_samples)
+        ref_trace_bps, _ = trace_bytes_per_sample(c_ref, cpu_samples)
+        # dominant kernel = BVH traversal: its algorithmic bytes per frame (node visits and
+        # triangle tests on the tree it traverses) over its summed launch time per frame (HIP
+        # events, live)
         algo_bytes = trace_bps * samples_per_step
         achieved = algo_bytes / (avg_trace_ms * 1e-3) / 1e9
         traffic, counters_src = None, None
@@ -331,6 +348,7 @@ def main():
                               "kernel": "k_wf_trace (all traversal launches of one frame)" if args.mode == "sorted"
                               else "k_render", "algorithmic_bytes": int(algo_bytes),
                               "trace_bytes_per_sample": round(trace_bps, 1),
+                              "reference_tree_trace_bytes_per_sample": round(ref_trace_bps, 1),
                               "frame_bytes_per_sample": round(bps, 1), "counters": counters_src}
         if not args.no_probe:
             peaks = read_bw_probe(torch.cuda.current_stream(dev).cuda_stream)
@@ -343,6 +361,7 @@ def main():
                                                                           cpu_s, cpu_model()),
                                   "bvh_build_ms_1thread": round(bvh_s * 1e3, 1)}
         result["oracle_counters_per_sample"] = {k: round(v / cpu_samples, 2) for k, v in counters.items()}
+        result["oracle_counters_per_sample_reference_tree"] = {k: round(v / cpu_samples, 2) for k, v in c_ref.items()}
         # rays per sample (closest-hit + shadow) from the oracle's deterministic counts
         rays = (counters["rays"] + counters["shadow_rays"]) / cpu_samples
         result["mrays_per_s"] = round(msps * rays, 1)

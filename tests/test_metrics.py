@@ -148,6 +148,10 @@ def test_gpu_read_probe_times_a_launch(gpu):
     from lgi import _abi
     L = _abi.lib()
     sink = torch.zeros(4, dtype=torch.int32, device="cuda")
+    warm = torch.ones(1 << 18, dtype=torch.int32, device="cuda")  # the first launch also loads the kernel
+    assert L.lgi_probe_read_bw(C.c_void_p(warm.data_ptr()), warm.numel() * 4, 1, 64, C.c_void_p(sink.data_ptr()),
+                               None, None) == 0
+    torch.cuda.synchronize()
     times = []
     for mib in (16, 1024):
         buf = torch.ones(mib << 18, dtype=torch.int32, device="cuda")
