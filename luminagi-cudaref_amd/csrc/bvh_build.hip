@@ -550,7 +550,7 @@ __global__ __launch_bounds__(64) void k_treelet(lgi_bvh_node* nodes, const int32
 // level (children of a level in the order of their parents), level_start[l] is where level l
 // begins, level_start[levels] = m. Deterministic (block scans, no atomics). A structure that
 // is not a tree of m slots with at most max_levels levels stops early with *levels = -1.
-constexpr int kBfsThreads = 1024;
+constexpr int kBfsThreads = 1024, kBfsItems = 4;
 __global__ __launch_bounds__(kBfsThreads) void k_bfs_levels(const lgi_bvh_node* nodes, int m, int max_levels,
                                                             int32_t* order, int32_t* level_start, int32_t* levels) {
     __shared__ uint32_t s_wave[kBfsThreads / 64];
@@ -566,11 +566,23 @@ __global__ __launch_bounds__(kBfsThreads) void k_bfs_levels(const lgi_bvh_node* 
         if (nlev == max_levels) { ok = false; break; }
         level_start[nlev++] = a;  // same value from every thread
         int out = b;
-        for (int base = a; base < b; base += kBfsThreads) {
-            const int i = base + tid;
-            const int x = i < b ? order[i] : -1;
-            const uint32_t inner = (x >= 0 && nodes[x].tri_count == 0) ? 1u : 0u;
-            uint32_t v = inner;
+        for (int base = a; base < b; base += kBfsThreads * kBfsItems) {
+            // kBfsItems consecutive entries per thread, their loads in flight together
+            int x[kBfsItems], l[kBfsItems], r[kBfsItems];
+            uint32_t cnt = 0;
+            for (int k = 0; k < kBfsItems; ++k) {
+                const int i = base + tid * kBfsItems + k;
+                x[k] = i < b ? order[i] : -1;
+            }
+            for (int k = 0; k < kBfsItems; ++k) {
+                l[k] = -1;
+                if (x[k] >= 0 && nodes[x[k]].tri_count == 0) {
+                    l[k] = nodes[x[k]].left;
+                    r[k] = nodes[x[k]].right;
+                    ++cnt;
+                }
+            }
+            uint32_t v = cnt;
             for (int o = 1; o < 64; o <<= 1) {
                 const uint32_t y = __shfl_up(v, o, 64);
                 if (lane >= o) v += y;
@@ -582,11 +594,13 @@ __global__ __launch_bounds__(kBfsThreads) void k_bfs_levels(const lgi_bvh_node* 
                 pre += w < wave ? s_wave[w] : 0u;
                 total += s_wave[w];
             }
-            const uint32_t rank = pre + v - inner;
+            uint32_t rank = pre + v - cnt;
             if ((int64_t)out + 2 * (int64_t)total > m) { ok = false; break; }  // uniform
-            if (inner) {
-                order[out + 2 * rank] = nodes[x].left;
-                order[out + 2 * rank + 1] = nodes[x].right;
+            for (int k = 0; k < kBfsItems; ++k) {
+                if (l[k] < 0) continue;
+                order[out + 2 * rank] = l[k];
+                order[out + 2 * rank + 1] = r[k];
+                ++rank;
             }
             out += 2 * (int)total;
             __syncthreads();
