@@ -289,7 +289,7 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
  * capture and mode 0 use a fixed choice. force_variant(v >= 0) makes every call use variant v
  * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
  * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, masked, relu, variant},
- * us[5*i..] = microseconds per launch of each variant (0 = does not fit); *n = entries. */
+ * us[8*i..] = microseconds per launch of each of the 8 variants (0 = does not fit); *n = entries. */
 LGI_API lgi_status lgi_unet_conv_tuning(int32_t mode);
 LGI_API lgi_status lgi_unet_conv_force_variant(int32_t variant);
 LGI_API lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32_t* n);

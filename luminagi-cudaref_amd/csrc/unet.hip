@@ -31,6 +31,7 @@ namespace {
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kBlock = 256;
 constexpr int kTH = 8, kTW = 16;               // output tile (rows x cols)
@@ -178,6 +179,133 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
                 const int x = x0 + 4 * g + j;
                 if (x >= a.W) continue;
                 float v = acc[mi][ni][j] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
+            }
+        }
+    }
+}
+
+// 3x3 convolution on v_mfma_f32_32x32x16_bf16. A block computes (4*RW rows x 32 columns) pixels
+// x TN channels; wave w owns rows [w*RW, (w+1)*RW), all TN channels. One 32x32x16 MFMA does
+// twice the multiply-adds of a 16x16x32 one for the same two 16-byte operand fragments, so per
+// multiply-add the waves read half as many bytes from LDS, whose bandwidth bounds k_conv.
+// Operand layouts (checked on the device): A[m][k] in lane m + 32*(k/8), element k%8, i.e. the
+// pixel m of the row and 8 channels of the 16-channel half; B[k][n] in lane n + 32*(k/8); D[m][n]
+// in lane n + 32*((m/4)%2), element 4*(m/8) + m%4.
+constexpr int kTW32 = 32, kHW32 = kTW32 + 2;
+
+template <int TN, int RW, bool MASK>
+__global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
+    constexpr int NG = TN / 32;
+    constexpr int TH = RW * (kBlock / 64), HH = TH + 2;
+    constexpr int kHalo = HH * kHW32 * 4;   // 16-byte loads of one chunk's halo
+    constexpr int kWLd = TN * 9 * 4;        // 16-byte loads of one chunk's weights
+    constexpr int kHIt = (kHalo + kBlock - 1) / kBlock, kWIt = (kWLd + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 s_in[HH * kHW32 * kCK];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[TN * kWRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.W + kTW32 - 1) / kTW32, tiles_y = (a.H + TH - 1) / TH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * TH, x0 = tx * kTW32;
+    const int co0 = blockIdx.y * TN;
+    const int Cin = a.C0 + a.C1;
+    f32x16 acc[RW][NG];
+    for (int mi = 0; mi < RW; ++mi)
+        for (int ng = 0; ng < NG; ++ng)
+            for (int j = 0; j < 16; ++j) acc[mi][ng][j] = 0.f;
+    const int l32 = lane & 31, kg = lane >> 5;
+    // same software pipeline as k_conv: chunk c + 1 is loaded into registers during chunk c
+    uint4 hv[kHIt], hm[MASK ? kHIt : 1], wv[kWIt];
+    bool masked = false;
+    auto load_chunk = [&](int ci0) {
+        const bool first = ci0 < a.C0;
+        const bf16* src = first ? a.src0 : a.src1;
+        const bf16* msk = first ? a.mask0 : a.mask1;
+        const int Cs = first ? a.C0 : a.C1;
+        const int cc0 = first ? ci0 : ci0 - a.C0;
+        masked = MASK && msk != nullptr;
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            hv[i] = zero16();
+            if (MASK) hm[i] = zero16();
+            if (t < kHalo) {
+                const int px = t >> 2, part = t & 3;
+                const int hy = px / kHW32, hx = px % kHW32;
+                const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                    const int64_t off = (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8;
+                    hv[i] = *reinterpret_cast<const uint4*>(src + off);
+                    if (MASK && msk) hm[i] = *reinterpret_cast<const uint4*>(msk + off);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            wv[i] = zero16();
+            if (t < kWLd) {
+                const int co = t / 36, r = t % 36, tap = r >> 2, part = r & 3;
+                wv[i] = *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * 9 + tap) * Cin + ci0 + part * 8);
+            }
+        }
+    };
+    load_chunk(0);
+    for (int ci0 = 0; ci0 < Cin; ci0 += kCK) {
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kHalo) {
+                uint4 v = hv[i];
+                if (MASK && masked) v = relu_mask(v, hm[MASK ? i : 0]);
+                *reinterpret_cast<uint4*>(&s_in[(t >> 2) * kCK + (t & 3) * 8]) = v;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kWLd) {
+                const int co = t / 36, r = t % 36, tap = r >> 2, part = r & 3;
+                *reinterpret_cast<uint4*>(&s_w[co * kWRow + tap * kCK + part * 8]) = wv[i];
+            }
+        }
+        __syncthreads();
+        if (ci0 + kCK < Cin) load_chunk(ci0 + kCK);
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+                bf16x8 A[RW], B[NG];
+                for (int mi = 0; mi < RW; ++mi) {
+                    const int row = wave * RW + mi + ky;
+                    A[mi] = *reinterpret_cast<const bf16x8*>(&s_in[(row * kHW32 + l32 + kx) * kCK + kh * 16 + kg * 8]);
+                }
+                for (int ng = 0; ng < NG; ++ng)
+                    B[ng] = *reinterpret_cast<const bf16x8*>(&s_w[(ng * 32 + l32) * kWRow + tap * kCK + kh * 16 + kg * 8]);
+                for (int mi = 0; mi < RW; ++mi)
+                    for (int ng = 0; ng < NG; ++ng)
+                        acc[mi][ng] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[mi], B[ng], acc[mi][ng], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // epilogue: element j of lane l is pixel column 8*(j/4) + 4*(l/32) + j%4, channel l%32
+    for (int mi = 0; mi < RW; ++mi) {
+        const int y = y0 + wave * RW + mi;
+        if (y >= a.H) continue;
+        for (int ng = 0; ng < NG; ++ng) {
+            const int co = co0 + ng * 32 + l32;
+            const float b = a.bias ? a.bias[co] : 0.0f;
+            bf16* out = co < a.Cs ? a.out0 : a.out1;
+            const int oc = co < a.Cs ? co : co - a.Cs;
+            const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+            for (int j = 0; j < 16; ++j) {
+                const int x = x0 + 8 * (j >> 2) + 4 * kg + (j & 3);
+                if (x >= a.W) continue;
+                float v = acc[mi][ng][j] + b;
                 if (a.relu) v = fmaxf(v, 0.0f);
                 out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
             }
@@ -690,14 +818,19 @@ __global__ void k_tail_reduce(const float* partial, int blocks, float* dw, float
 
 // ------------------------------------------------------------------ tile-shape selection
 // Convolution variants: 0 = 8x16-pixel tile x 32/64 channels (k_conv), 1..4 = register-blocked
-// k_conv_rb<RW, NB>. Which one is fastest depends on the layer (pixels, channels, how many
-// blocks fill 256 CUs), so the first call of each shape times every variant that fits it on
+// k_conv_rb<RW, NB>, 5..7 = k_conv32 on 32x32x16 MFMAs. Which one is fastest depends on the
+// layer (pixels, channels, how many blocks fill 256 CUs), so the first call of each shape
+// times every variant that fits it on
 // the caller's own buffers and stream, and later calls reuse the winner (the role cuDNN's
 // benchmark mode plays for the reference's PyTorch UNet). Under stream capture, or with
 // tuning off, the shape heuristic below is used.
-enum ConvVariant { kVarSimple = 0, kVarRb4x4, kVarRb8x2, kVarRb4x2, kVarRb8x4, kNumVariants };
-constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8};
-constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4};
+enum ConvVariant {
+    kVarSimple = 0, kVarRb4x4, kVarRb8x2, kVarRb4x2, kVarRb8x4,
+    kVarM32r2n64, kVarM32r2n32, kVarM32r4n32,  // k_conv32<TN, RW> (RW 4 with TN 64 spills)
+    kNumVariants
+};
+constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8, 2, 2, 4};
+constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4, 4, 2, 2};  // output channels / 16
 
 struct TuneEntry {
     int variant;
@@ -738,6 +871,26 @@ void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
         return;
     }
     const int th = kVarRW[v] * (kBlock / 64), tn = 16 * kVarNB[v];
+    if (v >= kVarM32r2n64) {
+        const dim3 grid(cdiv(a.W, kTW32) * cdiv(a.H, th) * a.N, a.Cout / tn);
+        const bool m = a.mask0 || a.mask1;
+        switch (v) {
+            case kVarM32r2n64:
+                if (m) hipLaunchKernelGGL((k_conv32<64, 2, true>), grid, dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((k_conv32<64, 2, false>), grid, dim3(kBlock), 0, st, a);
+                break;
+            case kVarM32r2n32:
+                if (m) hipLaunchKernelGGL((k_conv32<32, 2, true>), grid, dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((k_conv32<32, 2, false>), grid, dim3(kBlock), 0, st, a);
+                break;
+            case kVarM32r4n32:
+                if (m) hipLaunchKernelGGL((k_conv32<32, 4, true>), grid, dim3(kBlock), 0, st, a);
+                else hipLaunchKernelGGL((k_conv32<32, 4, false>), grid, dim3(kBlock), 0, st, a);
+                break;
+            default: break;
+        }
+        return;
+    }
     const dim3 grid(cdiv(a.W, kTW) * cdiv(a.H, th) * a.N, a.Cout / tn);
     switch (v) {
         case kVarRb4x4: hipLaunchKernelGGL((k_conv_rb<4, 4>), grid, dim3(kBlock), 0, st, a); break;

@@ -235,6 +235,22 @@ def test_frame_graph_replays_match_eager_renderers(gpu, flags):
         _ = gpu_accumulate(ctx, cam, 2, 1)  # eager work between replays
 
 
+WAVEFRONT_PATHS = 96 << 20  # paths the wavefront renderer keeps in flight per pass
+
+
+@pytest.mark.parametrize("flags", [2, 10], ids=["sorted", "no_ray_sort"])
+def test_wavefront_multi_pass_equals_inline(gpu, flags):
+    """A render with more samples than the wavefront renderer keeps in flight runs as several
+    passes (a full one, then the rest); the sums equal the inline kernel's bit for bit."""
+    ctx, _, s = make_pair(seed=4)
+    cam = small_camera(s["camera"], 64, 64)  # 64 tiles of 64 paths per sample
+    spp = WAVEFRONT_PATHS // (64 * 64) + 40
+    inline = gpu_accumulate(ctx, cam, spp, 2)
+    wave = gpu_accumulate(ctx, cam, spp, 2, flags=flags)
+    for a, b in zip(inline, wave):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (9, 17), (65, 1)])
 def test_tiny_and_ragged_images(gpu, w, h):
     """Image sizes that leave partial 8x8 tiles (and a single pixel): the inline renderer, the

@@ -37,16 +37,17 @@ CONV_CASES = [  # n, h, w, c0, c1, cout, relu
     (1, 12, 20, 32, 32, 32, True),      # ragged tiles + fused concat
     (1, 8, 16, 128, 64, 64, False),
     (2, 4, 6, 256, 0, 96, True),        # image smaller than one tile, Cout not a multiple of 64
+    (1, 10, 45, 64, 64, 128, True),     # ragged 32-column tiles, concat, two 64-channel blocks
 ]
 
 
-@pytest.fixture(params=[None, 0, 1, 2, 3, 4], ids=lambda v: "tuned" if v is None else f"variant{v}")
+@pytest.fixture(params=[None, 0, 1, 2, 3, 4, 5, 6, 7], ids=lambda v: "tuned" if v is None else f"variant{v}")
 def conv_variant(request, gpu):
     """Every tiling variant forced in turn (None = per-shape selection by timing)."""
     from lgi import unet
     v = request.param
     cout = request.node.callspec.params["case"][5]
-    if v in (1, 4) and cout % 64:
+    if v in (1, 4, 5) and cout % 64:
         pytest.skip("64-channel tiling needs cout % 64 == 0")
     unet.force_conv_variant(v)
     yield v
@@ -288,7 +289,7 @@ def test_conv_tuning_table_records_each_shape_once(gpu):
     rows = [r for r in unet.conv_tune_table() if (r[0]["h"], r[0]["w"], r[0]["c0"], r[0]["cout"]) == (40, 48, 64, 64)]
     assert len(rows) == 1
     shape, chosen, times = rows[0]
-    assert set(times) == {0, 1, 2, 3, 4} and chosen == min(times, key=times.get)
+    assert set(times) == {0, 1, 2, 3, 4, 5, 6, 7} and chosen == min(times, key=times.get)
     assert torch.equal(y1, y2)
 
 
