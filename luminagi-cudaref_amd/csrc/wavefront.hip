@@ -141,12 +141,24 @@ constexpr uint32_t kChunk = 64;
 // are copied into LDS once per block and read from there; kLds stack entries live in LDS.
 // kMinLanes: phase 1 hands over to the leaf tests once fewer than kMinLanes of the lanes still
 // traversing have no parked leaf (1: only when every such lane holds one).
-template <bool kAny, int kLds, int kTop, int kMinLanes>
+// kMode: kTraceClosest (queue q, n entries), kTraceShadow (shadow rays, n_sh entries) or
+// kTraceBoth: the shadow rays of one bounce and the closest-hit rays of the next in one launch
+// (queue positions [0, n_sh) are shadow rays, [n_sh, n_sh + n) closest-hit rays). Both depend
+// only on the shading of the bounce before, and one launch instead of two saves a persistent
+// kernel's tail, the time its last rays take after the queue has run dry.
+enum TraceMode { kTraceClosest = 0, kTraceShadow = 1, kTraceBoth = 2 };
+
+template <int kMode, int kLds, int kTop, int kMinLanes>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
+                                                     const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev,
                                                      float4* hits, uint8_t* nee, uint32_t* cursors, int* spill,
-                                                     ShadowSrc shadow, const uint32_t* n_dev) {
+                                                     ShadowSrc shadow) {
+    if (kMode == kTraceShadow) n = 0;
+    if (kMode == kTraceClosest) n_sh = 0;
     if (n_dev) n = min(n, (int)*n_dev);
+    if (n_sh_dev) n_sh = min(n_sh, (int)*n_sh_dev);
+    const int n_all = n_sh + n;
     __shared__ int s_stack[kLds * kBlock];
     __shared__ TNode s_top[kTop > 0 ? kTop : 1];
     if (kTop > 0) {
@@ -164,8 +176,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
     bool exhausted = false;
     const int n_seg = min(kSegments, (int)gridDim.x);
     int seg = (int)blockIdx.x % n_seg, tried = 0;
-    auto seg_lo = [&](int g) { return (uint32_t)((uint64_t)(uint32_t)n * (uint32_t)g / (uint32_t)n_seg); };
+    auto seg_lo = [&](int g) { return (uint32_t)((uint64_t)(uint32_t)n_all * (uint32_t)g / (uint32_t)n_seg); };
     bool active = false;
+    bool any = kMode == kTraceShadow;  // this lane's ray is a shadow ray (any hit ends it)
     // node: >= 0 internal node to visit, < 0 leaf (~first triangle), kNone when the stack is
     // exhausted; leaf: a parked leaf (0 = none)
     constexpr int kNone = INT_MIN;
@@ -202,7 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 uint32_t idx = 0xFFFFFFFFu;
                 if (rank < avail) idx = cbase + rank;
                 else if (nb != 0xFFFFFFFFu && rank - avail < nend - nb) idx = nb + (rank - avail);
-                if (idx != 0xFFFFFFFFu && kAny) {
+                const bool sh = kMode == kTraceShadow || (kMode == kTraceBoth && idx < (uint32_t)n_sh);
+                if (kMode == kTraceBoth) any = sh;
+                if (idx != 0xFFFFFFFFu && sh) {
                     const uint32_t j = shadow.ids[idx];
                     const int si = (int)(j / (uint32_t)shadow.nl), li = (int)(j % (uint32_t)shadow.nl);
                     const float4 p4 = shadow.vP[si], n4 = shadow.vN[si];
@@ -224,6 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                     sp = 0;
                     active = true;
                 } else if (idx != 0xFFFFFFFFu) {
+                    if (kMode == kTraceBoth) idx -= (uint32_t)n_sh;
                     const QRay e = q[idx];
                     qi = (int)idx;
                     slot = __float_as_int(e.o.w);
@@ -238,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                         leaf = 0;
                         sp = 0;
                         active = true;
-                    } else if (!kAny) {
+                    } else {
                         hits[idx] = make_float4(-1.f, 0.f, 0.f, __int_as_float(-1));
                     }
                 }
@@ -305,19 +321,19 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 float tt, uu, vv;
                 if (tri_hit(r, t, best_t, tt, uu, vv)) {
                     best_tri = ti; best_t = tt; best_u = uu; best_v = vv;
-                    if (kAny) break;
+                    if (any) break;
                 }
                 if (__float_as_int(t.v0.w) != 0) break;
                 ++ti;
             }
-            if (kAny && best_tri >= 0) {
+            if (any && best_tri >= 0) {
                 node = kNone;
                 sp = 0;
                 break;
             }
         }
         if (node == kNone && leaf == 0) {
-            if (kAny) {
+            if (any) {
                 if (best_tri >= 0) nee[slot] = 2;
             } else {
                 hits[qi] = make_float4(best_t, best_u, best_v, __int_as_float(best_tri));
@@ -595,19 +611,22 @@ constexpr int kTraceLdsStack = 8, kTraceTopNodes = 127;
 constexpr int kTraceMinLanes = 8, kTraceMinLanesAny = 8;
 static_assert(kTraceLdsStack >= kStackLdsMin, "spill area sized for kStackMax - kStackLdsMin entries");
 
-template <bool kAny>
-void launch_trace(int grid, hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
-                  float4* hits, uint8_t* nee, uint32_t* cursors, int* spill, const ShadowSrc& shadow,
-                  const uint32_t* n_dev) {
-    hipLaunchKernelGGL((k_wf_trace<kAny, kTraceLdsStack, kTraceTopNodes, kAny ? kTraceMinLanesAny : kTraceMinLanes>),
-                       dim3(grid), dim3(kBlock), 0, st, tn, n_tn, tv, q, n, hits, nee, cursors, spill, shadow, n_dev);
-}
-
 int trace_grid(int n) {
     // persistent: one resident round (16 KB of LDS per block -> 8 blocks = 32 waves per
     // CU on 256 CUs), never more blocks than the work needs
     return std::max(1, std::min(cdiv(n, kBlock), kTraceGridMax));
 }
+
+template <int kMode>
+void launch_trace(hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
+                  const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev, float4* hits, uint8_t* nee,
+                  uint32_t* cursors, int* spill, const ShadowSrc& shadow) {
+    constexpr int kMin = kMode == kTraceShadow ? kTraceMinLanesAny : kTraceMinLanes;
+    const int grid = trace_grid((kMode == kTraceShadow ? 0 : n) + (kMode == kTraceClosest ? 0 : n_sh));
+    hipLaunchKernelGGL((k_wf_trace<kMode, kTraceLdsStack, kTraceTopNodes, kMin>), dim3(grid), dim3(kBlock), 0, st, tn,
+                       n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow);
+}
+
 
 }  // namespace
 
@@ -694,13 +713,14 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         hipLaunchKernelGGL(k_wf_primary, dim3(cdiv(n_max, kBlock)), dim3(kBlock), 0, st, W, S, tiles_x, padded, qa);
         QRay* cur = qa;
         QRay* other = qb;
-        const uint32_t* n_dev = nullptr;  // bounce 0: all n_max entries
+        const uint32_t* n_dev = nullptr;  // live entries of this bounce's queue (bounce 0: all n_max)
+        if ((ms = mark(true)) != LGI_OK) return ms;
+        launch_trace<kTraceClosest>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, n_dev, 0, nullptr, hits,
+                                    nullptr, counters + kCursorStride, spill, shadow);
+        if ((ms = mark(false)) != LGI_OK) return ms;
         for (int b = 0; b <= P.bounces; ++b) {
             uint32_t* cnt = counters + kCounters * b;
-            if ((ms = mark(true)) != LGI_OK) return ms;
-            launch_trace<false>(trace_grid(n_max), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, hits,
-                                (uint8_t*)nullptr, cnt + kCursorStride, spill, shadow, n_dev);
-            if ((ms = mark(false)) != LGI_OK) return ms;
+            const bool last = b == P.bounces;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
                                sort ? keys_tmp : nullptr, ncont, blk_s, blk_n);
             // block offsets of the shadow-ray candidates and of the continuations
@@ -708,25 +728,33 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
             hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n_max, n_dev,
                                (const uint32_t*)blk_s, sids);
+            if (!last) {
+                // The next bounce's queue goes to cur (this bounce's queue is no longer read):
+                // with the sort, (key, slot) pairs of the continuations are compacted, sorted by
+                // key, and the rays gathered from shading's output in key order; without it, the
+                // rays are compacted.
+                if (sort) {
+                    hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, keys_tmp,
+                                       (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n, keys, idx);
+                    const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
+                    hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, other, sorted, cur, n_max, cnt + 2);
+                } else {
+                    hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, (const uint8_t*)ncont,
+                                       n_max, n_dev, (const uint32_t*)blk_n, cur);
+                }
+            }
+            // this bounce's shadow rays and, but for the last bounce, the next bounce's closest
+            // hits in the same persistent launch
+            uint32_t* cur_sh = cnt + kCursorStride * (1 + kSegments);
             if ((ms = mark(true)) != LGI_OK) return ms;
-            launch_trace<true>(trace_grid(n_max * nl), st, ctx->dev.tnodes, n_tn, ctx->dev.triv, (const QRay*)nullptr,
-                               n_max * nl, (float4*)nullptr, W.nee, cnt + kCursorStride * (1 + kSegments), spill,
-                               shadow, cnt + 1);
+            if (last)
+                launch_trace<kTraceShadow>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, nullptr, 0, nullptr, n_max * nl,
+                                           cnt + 1, nullptr, W.nee, cur_sh, spill, shadow);
+            else
+                launch_trace<kTraceBoth>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2, n_max * nl,
+                                         cnt + 1, hits, W.nee, cur_sh, spill, shadow);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
-            if (b == P.bounces) break;
-            // The next bounce's queue goes to cur (this bounce's queue is no longer read): with the
-            // sort, (key, slot) pairs of the continuations are compacted, sorted by key, and the
-            // rays gathered from shading's output in key order; without it, the rays are compacted.
-            if (sort) {
-                hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, keys_tmp, (const uint8_t*)ncont,
-                                   n_max, n_dev, (const uint32_t*)blk_n, keys, idx);
-                const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
-                hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, other, sorted, cur, n_max, cnt + 2);
-            } else {
-                hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, (const uint8_t*)ncont, n_max,
-                                   n_dev, (const uint32_t*)blk_n, cur);
-            }
             n_dev = cnt + 2;
         }
         const int n_acc = d_tiles ? n_tiles * 64 : npx;
