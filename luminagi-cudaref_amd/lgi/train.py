@@ -204,7 +204,9 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None
         flags, rt = pick_renderer(ctx, cam, bounces)
         res["renderer_ms_8spp"] = {RENDERERS[k]: v for k, v in rt.items()}
     res["renderer"] = RENDERERS.get(flags, str(flags))
-    # plain curve: progressive accumulation, timed per added chunk
+    # plain curve: progressive accumulation, timed per added chunk; a throw-away render at the
+    # largest chunk first, so that no timed chunk pays for growing the renderer's buffers
+    D.Frame(cam).render(ctx, 512, bounces, sample0=test0 + (1 << 25), flags=flags)
     f = D.Frame(cam)
     total_ms, spp = 0.0, 0
     for target in (8, 16, 32, 64, 128, 256, 512, 1024):
