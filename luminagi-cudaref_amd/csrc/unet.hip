@@ -292,24 +292,35 @@ __global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
         }
         __syncthreads();
     }
-    // epilogue: element j of lane l is pixel column 8*(j/4) + 4*(l/32) + j%4, channel l%32
-    for (int mi = 0; mi < RW; ++mi) {
-        const int y = y0 + wave * RW + mi;
-        if (y >= a.H) continue;
+    // Epilogue through LDS: element j of lane l is pixel column 8*(j/4) + 4*(l/32) + j%4,
+    // channel l%32; the tile is written to LDS as [row][column][channel] (in the weight slice's
+    // space for 64 channels, the halo's for 32, the last chunk's barrier above freed both) and
+    // then stored 8 channels (16 bytes) per thread.
+    static_assert(TN != 64 || TH * kTW32 * TN <= TN * kWRow, "64-channel tile fits the weight slice");
+    static_assert(TN != 32 || TH * kTW32 * TN <= HH * kHW32 * kCK, "32-channel tile fits the halo");
+    bf16* s_out = TN == 64 ? s_w : s_in;
+    for (int mi = 0; mi < RW; ++mi)
         for (int ng = 0; ng < NG; ++ng) {
-            const int co = co0 + ng * 32 + l32;
-            const float b = a.bias ? a.bias[co] : 0.0f;
-            bf16* out = co < a.Cs ? a.out0 : a.out1;
-            const int oc = co < a.Cs ? co : co - a.Cs;
-            const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+            const int c = ng * 32 + l32;
+            const float b = a.bias ? a.bias[co0 + c] : 0.0f;
             for (int j = 0; j < 16; ++j) {
-                const int x = x0 + 8 * (j >> 2) + 4 * kg + (j & 3);
-                if (x >= a.W) continue;
                 float v = acc[mi][ng][j] + b;
                 if (a.relu) v = fmaxf(v, 0.0f);
-                out[(((int64_t)n * a.H + y) * a.W + x) * Co + oc] = (bf16)v;
+                s_out[((wave * RW + mi) * kTW32 + 8 * (j >> 2) + 4 * kg + (j & 3)) * TN + c] = (bf16)v;
             }
         }
+    __syncthreads();
+    constexpr int kVecs = TH * kTW32 * TN / 8;
+    for (int t = tid; t < kVecs; t += kBlock) {
+        const int px = t / (TN / 8), cg = t % (TN / 8);
+        const int y = y0 + px / kTW32, x = x0 + px % kTW32;
+        if (y >= a.H || x >= a.W) continue;
+        const int co = co0 + cg * 8;  // 8 channels never straddle Cs (a multiple of 32)
+        bf16* out = co < a.Cs ? a.out0 : a.out1;
+        const int oc = co < a.Cs ? co : co - a.Cs;
+        const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+        *reinterpret_cast<uint4*>(out + (((int64_t)n * a.H + y) * a.W + x) * Co + oc) =
+            *reinterpret_cast<const uint4*>(&s_out[px * TN + cg * 8]);
     }
 }
 
@@ -461,20 +472,25 @@ __global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__
             acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
         }
     }
-    for (int mi = 0; mi < 2; ++mi) {
-        const int y = y0 + wave * 2 + mi;
-        if (y >= H) continue;
+    // epilogue through LDS: the 8 x 16 x 32 tile as [row][column][channel], then 16-byte stores
+    __shared__ __attribute__((aligned(16))) bf16 s_out[kTH * kTW * 32];
+    for (int mi = 0; mi < 2; ++mi)
         for (int ni = 0; ni < 2; ++ni) {
             const int co = ni * 16 + r16;
             const float b = bias ? bias[co] : 0.0f;
             for (int j = 0; j < 4; ++j) {
-                const int xx = x0 + 4 * g + j;
-                if (xx >= W) continue;
                 float v = acc[mi][ni][j] + b;
                 if (relu) v = fmaxf(v, 0.0f);
-                out[(((int64_t)n * H + y) * W + xx) * 32 + co] = (bf16)v;
+                s_out[((wave * 2 + mi) * kTW + 4 * g + j) * 32 + co] = (bf16)v;
             }
         }
+    __syncthreads();
+    for (int t = tid; t < kTH * kTW * 4; t += kBlock) {
+        const int px = t >> 2, cg = t & 3;
+        const int y = y0 + px / kTW, xx = x0 + px % kTW;
+        if (y >= H || xx >= W) continue;
+        *reinterpret_cast<uint4*>(out + (((int64_t)n * H + y) * W + xx) * 32 + cg * 8) =
+            *reinterpret_cast<const uint4*>(&s_out[px * 32 + cg * 8]);
     }
 }
 
