@@ -549,78 +549,67 @@ __global__ __launch_bounds__(64) void k_treelet(lgi_bvh_node* nodes, const int32
     }
 }
 
-// Breadth-first levels of the current tree in one workgroup: order[] lists the slots level by
-// level (children of a level in the order of their parents), level_start[l] is where level l
-// begins, level_start[levels] = m. Deterministic (block scans, no atomics). A structure that
-// is not a tree of m slots with at most max_levels levels stops early with *levels = -1.
-constexpr int kBfsThreads = 1024, kBfsItems = 4;
-__global__ __launch_bounds__(kBfsThreads) void k_bfs_levels(const lgi_bvh_node* nodes, int m, int max_levels,
-                                                            int32_t* order, int32_t* level_start, int32_t* levels) {
-    __shared__ uint32_t s_wave[kBfsThreads / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) {
-        order[0] = 0;
-        level_start[0] = 0;
+// Levels of the current tree, in parallel: every slot finds its depth by walking its parent
+// chain, then one stable 8-bit radix pass orders the slots by depth (by slot index within a
+// level). info[0] receives the deepest depth and info[1] an error flag: set when a child ref is
+// out of range, a slot other than the root has no parent or two, or a walk does not reach the
+// root within kMaxWalk steps (a cycle). Only a tree of m slots passes.
+constexpr int kMaxWalk = 63;
+__global__ __launch_bounds__(kBlock) void k_tree_init(int m, int32_t* parent, uint32_t* nparent, int32_t* info) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) {
+        parent[p] = -1;
+        nparent[p] = 0u;
     }
-    __syncthreads();
-    int a = 0, b = 1, nlev = 0;
-    bool ok = true;
-    while (a < b) {
-        if (nlev == max_levels) { ok = false; break; }
-        level_start[nlev++] = a;  // same value from every thread
-        int out = b;
-        for (int base = a; base < b; base += kBfsThreads * kBfsItems) {
-            // kBfsItems consecutive entries per thread, their loads in flight together
-            int x[kBfsItems], l[kBfsItems], r[kBfsItems];
-            uint32_t cnt = 0;
-            for (int k = 0; k < kBfsItems; ++k) {
-                const int i = base + tid * kBfsItems + k;
-                x[k] = i < b ? order[i] : -1;
-            }
-            for (int k = 0; k < kBfsItems; ++k) {
-                l[k] = -1;
-                if (x[k] >= 0 && nodes[x[k]].tri_count == 0) {
-                    l[k] = nodes[x[k]].left;
-                    r[k] = nodes[x[k]].right;
-                    ++cnt;
-                }
-            }
-            uint32_t v = cnt;
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(v, o, 64);
-                if (lane >= o) v += y;
-            }
-            if (lane == 63) s_wave[wave] = v;
-            __syncthreads();
-            uint32_t pre = 0, total = 0;
-            for (int w = 0; w < kBfsThreads / 64; ++w) {
-                pre += w < wave ? s_wave[w] : 0u;
-                total += s_wave[w];
-            }
-            uint32_t rank = pre + v - cnt;
-            if ((int64_t)out + 2 * (int64_t)total > m) { ok = false; break; }  // uniform
-            for (int k = 0; k < kBfsItems; ++k) {
-                if (l[k] < 0) continue;
-                order[out + 2 * rank] = l[k];
-                order[out + 2 * rank + 1] = r[k];
-                ++rank;
-            }
-            out += 2 * (int)total;
-            __syncthreads();
+    if (p < 2) info[p] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tree_parents(const lgi_bvh_node* nodes, int m, int32_t* parent,
+                                                         uint32_t* nparent, int32_t* info) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m || nodes[p].tri_count > 0) return;
+    for (int c : {nodes[p].left, nodes[p].right}) {
+        if (c <= 0 || c >= m) {
+            info[1] = 1;
+            continue;
         }
-        if (!ok) break;
-        a = b;
-        b = out;
-        __syncthreads();
-    }
-    if (tid == 0) {
-        level_start[nlev] = b;
-        *levels = (ok && b == m) ? nlev : -1;
+        parent[c] = p;
+        atomicAdd(&nparent[c], 1u);
     }
 }
 
-// Traversal numbering of the restructured tree: its internal nodes in breadth-first order
-// (order[] from k_bfs_levels), as the reference tree's are in its own array.
+__global__ __launch_bounds__(kBlock) void k_tree_depth(int m, const int32_t* parent, const uint32_t* nparent,
+                                                       uint32_t* depth, int32_t* slot, int32_t* info) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    int d = 0;
+    if (p > 0) {
+        if (nparent[p] != 1u) info[1] = 1;
+        int x = p;
+        while (x > 0 && d < kMaxWalk) {  // depth <= kMaxWalk: level starts fit [0, kMaxWalk + 1]
+            x = parent[x];
+            ++d;
+        }
+        if (x != 0) {
+            info[1] = 1;
+            d = 0;
+        }
+    }
+    depth[p] = (uint32_t)d;
+    slot[p] = p;
+    atomicMax(&info[0], d);
+}
+
+__global__ __launch_bounds__(kBlock) void k_depth_starts(const uint32_t* depth_sorted, int m, int32_t* level_start) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const int d = (int)depth_sorted[p];
+    if (p == 0 || (int)depth_sorted[p - 1] != d) level_start[d] = p;
+    if (p == m - 1) level_start[d + 1] = m;
+}
+
+// Traversal numbering of the restructured tree: its internal nodes level by level (order[] from
+// the depth sort), as the reference tree's are in its own array.
 __global__ __launch_bounds__(kBlock) void k_bfs_flags(const lgi_bvh_node* nodes, const int32_t* order, int m,
                                                       uint32_t* flags) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -885,11 +874,12 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
         lgi_bvh_node* onodes = (lgi_bvh_node*)S(20, (size_t)m * sizeof(lgi_bvh_node));
         float* cost = (float*)S(21, (size_t)m * 4);
         uint32_t* rank = (uint32_t*)S(22, (size_t)m * 4);
-        int32_t* order = (int32_t*)S(24, (size_t)m * 4);
-        constexpr int kBfsLevels = 64;
-        int32_t* lstart = (int32_t*)S(25, (kBfsLevels + 2) * 4);
-        if (!onodes || !cost || !rank || !order || !lstart)
+        int32_t* parent = (int32_t*)S(24, (size_t)m * 4);
+        constexpr int kBfsLevels = kMaxWalk + 1;
+        int32_t* lstart = (int32_t*)S(25, (kBfsLevels + 4) * 4);  // level starts, then depth and error
+        if (!onodes || !cost || !rank || !parent || !lstart)
             return lgi_set_err(ctx, LGI_ERR_HIP, "scratch allocation failed");
+        const int32_t* order = nullptr;
         LGI_HIP(ctx, hipMemcpyAsync(onodes, d.nodes, (size_t)m * sizeof(lgi_bvh_node), hipMemcpyDeviceToDevice, st));
         for (int lv = max_depth; lv >= 0; --lv)
             hipLaunchKernelGGL(k_sah_level, dim3(ceil_div(hls[lv + 1] - hls[lv], kBlock)), dim3(kBlock), 0, st, onodes,
@@ -898,22 +888,29 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
         // restructured tree no longer follows them (a root of one level may now lie below
         // another), so each later pass first takes the levels of the current tree.
         bool tree_ok = true;
-        int32_t lv_host[kBfsLevels + 2];  // level starts, then the level count at [kBfsLevels + 1]
+        int32_t lv_host[kBfsLevels + 4];
         const int32_t* lv_order = nullptr;
         int n_lev = max_depth + 1;
         std::copy(hls, hls + n_lev + 1, lv_host);
         // The levels are also taken after the last pass: the traversal uses the restructured tree
-        // only when its breadth-first walk covers the m slots within kMaxTraversalDepth levels
-        // below the root (the traversal stacks hold one entry per internal level).
+        // only when it is a tree of the m slots at most kMaxTraversalDepth levels deep (the
+        // traversal stacks hold one entry per internal level). The depth sort reuses the
+        // build's sort buffers.
+        int32_t* info = lstart + kBfsLevels + 1;
+        const int gm = ceil_div(m, kBlock);
         for (int pass = 0; pass <= passes; ++pass) {
             if (pass > 0) {
-                hipLaunchKernelGGL(k_bfs_levels, dim3(1), dim3(kBfsThreads), 0, st, onodes, (int)m, kBfsLevels, order,
-                                   lstart, lstart + kBfsLevels + 1);
+                hipLaunchKernelGGL(k_tree_init, dim3(gm), dim3(kBlock), 0, st, (int)m, parent, rank, info);
+                hipLaunchKernelGGL(k_tree_parents, dim3(gm), dim3(kBlock), 0, st, onodes, (int)m, parent, rank, info);
+                hipLaunchKernelGGL(k_tree_depth, dim3(gm), dim3(kBlock), 0, st, (int)m, parent, rank, sb.k[0], sb.v[0],
+                                   info);
+                const int cur = radix_sort(st, sb, (int)m, 8);
+                hipLaunchKernelGGL(k_depth_starts, dim3(gm), dim3(kBlock), 0, st, sb.k[cur], (int)m, lstart);
                 LGI_HIP(ctx, hipMemcpyAsync(lv_host, lstart, sizeof(lv_host), hipMemcpyDeviceToHost, st));
                 LGI_HIP(ctx, hipStreamSynchronize(st));
-                n_lev = lv_host[kBfsLevels + 1];
-                lv_order = order;
-                if (n_lev < 1) { tree_ok = false; break; }
+                order = lv_order = sb.v[cur];
+                if (lv_host[kBfsLevels + 2] != 0 || lv_host[kBfsLevels + 1] > kMaxWalk) { tree_ok = false; break; }
+                n_lev = lv_host[kBfsLevels + 1] + 1;
             }
             if (pass == passes) break;
             for (int lv = n_lev - 2; lv >= 0; --lv)
