@@ -264,6 +264,14 @@ LGI_API lgi_status lgi_read_png(const char* path, int32_t* w, int32_t* h, uint8_
 LGI_API lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1,
                                         const void* w, const float* bias, int32_t cout, int32_t n,
                                         int32_t h, int32_t w_, int32_t relu, void* out, void* stream);
+/* lgi_unet_conv3x3_fwd that also writes pool_out (n, h/2, w/2, cout) = MaxPool2d(2)(y), taken
+ * from the output tile while it is still on chip (ml/train.py:41-44 followed by :62, the
+ * encoder's second convolution and the pool after it). Bit-identical to lgi_unet_conv3x3_fwd
+ * then lgi_unet_maxpool_fwd; h and w must be even, as there. */
+LGI_API lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1,
+                                             const void* w, const float* bias, int32_t cout, int32_t n,
+                                             int32_t h, int32_t w_, int32_t relu, void* out, void* pool_out,
+                                             void* stream);
 /* First layer for inference: y = relu?(conv3x3(x) + bias) with x the FP32 3-channel NHWC
  * input (read directly, no padded copy), w [32][9][cin_w] BF16 of which channels 0..2 are used,
  * y (n, h, w, 32) BF16. Equals lgi_unet_conv3x3_fwd on lgi_unet_pack_input(x) up to FP32
@@ -288,7 +296,8 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
  * synchronizes that stream once) and later calls reuse the fastest; calls under stream
  * capture and mode 0 use a fixed choice. force_variant(v >= 0) makes every call use variant v
  * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
- * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, masked, relu, variant},
+ * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, kind, relu, variant} with kind
+ * 0 = forward, 1 = ReLU-masked input (data gradient), 2 = forward with a pooled copy,
  * us[8*i..] = microseconds per launch of each of the 8 variants (0 = does not fit); *n = entries. */
 LGI_API lgi_status lgi_unet_conv_tuning(int32_t mode);
 LGI_API lgi_status lgi_unet_conv_force_variant(int32_t variant);

@@ -53,9 +53,21 @@ struct ConvArgs {
     int C0, C1, Cout, Cs;
     int N, H, W;
     int relu;
+    bf16* pool_out;        // optional [N,H/2,W/2,Cout]: 2x2 max-pool of the output (k_conv32 only)
 };
 
 __device__ __forceinline__ uint4 zero16() { return make_uint4(0u, 0u, 0u, 0u); }
+
+// 8 bf16 channels (16 bytes) to and from fp32.
+__device__ __forceinline__ void unpack8(const uint4& v, float f[8]) {
+    const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+    for (int k = 0; k < 8; ++k) f[k] = (float)e[k];
+}
+__device__ __forceinline__ uint4 pack8(const float f[8]) {
+    bf16x8 e;
+    for (int k = 0; k < 8; ++k) e[k] = (bf16)f[k];
+    return *reinterpret_cast<const uint4*>(&e);
+}
 
 // Multiply 8 bf16 values by (mask > 0).
 __device__ __forceinline__ uint4 relu_mask(uint4 v, uint4 m) {
@@ -321,6 +333,30 @@ __global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
         const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
         *reinterpret_cast<uint4*>(out + (((int64_t)n * a.H + y) * a.W + x) * Co + oc) =
             *reinterpret_cast<const uint4*>(&s_out[px * TN + cg * 8]);
+    }
+    if (a.pool_out) {
+        // MaxPool2d(2) of the tile, as k_pool_fwd computes it (first maximum in row-major
+        // order, NaN propagates); H and W are even and tiles start at even rows and columns,
+        // so every window lies inside one tile
+        const int Hp = a.H >> 1, Wp = a.W >> 1;
+        constexpr int kPool = (TH / 2) * (kTW32 / 2) * (TN / 8);
+        for (int t = tid; t < kPool; t += kBlock) {
+            const int pp = t / (TN / 8), cg = t % (TN / 8);
+            const int ry = pp / (kTW32 / 2), rx = pp % (kTW32 / 2);
+            const int py = (y0 >> 1) + ry, pxo = (x0 >> 1) + rx;
+            if (py >= Hp || pxo >= Wp) continue;
+            float m[8], v[8];
+            unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx) * TN + cg * 8]), m);
+            const int offs[3] = {1, kTW32, kTW32 + 1};
+            for (int k = 0; k < 3; ++k) {
+                unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx + offs[k]) * TN + cg * 8]),
+                        v);
+                for (int e = 0; e < 8; ++e)
+                    if (v[e] > m[e] || v[e] != v[e]) m[e] = v[e];
+            }
+            *reinterpret_cast<uint4*>(a.pool_out + (((int64_t)n * Hp + py) * Wp + pxo) * a.Cout + co0 + cg * 8) =
+                pack8(m);
+        }
     }
 }
 
@@ -630,15 +666,6 @@ __global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* dy, const bf16
 
 // ------------------------------------------------------------------ pool / upsample
 // Elementwise layers move 8 channels (16 bytes) per thread; C is a multiple of 8.
-__device__ __forceinline__ void unpack8(const uint4& v, float f[8]) {
-    const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
-    for (int k = 0; k < 8; ++k) f[k] = (float)e[k];
-}
-__device__ __forceinline__ uint4 pack8(const float f[8]) {
-    bf16x8 e;
-    for (int k = 0; k < 8; ++k) e[k] = (bf16)f[k];
-    return *reinterpret_cast<const uint4*>(&e);
-}
 
 // Thread mapping of the per-pixel NHWC kernels below: blockIdx.y = n * rows + row, and the
 // threads along x cover the (column, 8-channel group) pairs of that row. Index math stays in
@@ -654,6 +681,10 @@ __device__ __forceinline__ bool row_thread(int rows, int cols, int C8, int& n, i
 }
 
 inline dim3 row_grid(int n, int rows, int cols, int C8) { return dim3(cdiv((int64_t)cols * C8, kBlock), n * rows); }
+// The row-mapped kernels need n * rows <= 65535 (grid y) and cols * C/8 < 2^31.
+static bool row_grid_fits(int64_t n, int64_t rows, int64_t cols, int64_t c8) {
+    return n > 0 && rows > 0 && cols > 0 && n * rows <= 65535 && cols * c8 < (int64_t(1) << 31);
+}
 
 // MaxPool2d(2): out[n,y,x,c] = max over the 2x2 window (first maximum in row-major order).
 __global__ __launch_bounds__(kBlock) void k_pool_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
@@ -857,6 +888,8 @@ std::map<std::array<int, 9>, TuneEntry> g_tune;
 std::atomic<int> g_tune_mode{1};
 std::atomic<int> g_force_variant{-1};
 
+// With pool_out, the k_conv32 variants write the pooled copy from their output tile and the
+// others are followed by a k_pool_fwd launch; the tuner times each pair as one candidate.
 bool variant_fits(int v, const ConvArgs& a, int taps) {
     if (v == kVarSimple) return true;
     if (taps != 9 || v < 0 || v >= kNumVariants) return false;
@@ -864,12 +897,19 @@ bool variant_fits(int v, const ConvArgs& a, int taps) {
 }
 
 int heuristic_variant(const ConvArgs& a, int taps) {
-    (void)a;
     (void)taps;
-    return kVarSimple;
+    return a.pool_out ? kVarM32r2n32 : kVarSimple;
 }
 
 void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
+    if (a.pool_out && v < kVarM32r2n64) {
+        ConvArgs c = a;
+        c.pool_out = nullptr;
+        launch_variant(v, c, taps, st);
+        hipLaunchKernelGGL(k_pool_fwd, row_grid(a.N, a.H / 2, a.W / 2, a.Cout / 8), dim3(kBlock), 0, st, a.out0, a.N,
+                           a.H, a.W, a.Cout, a.pool_out);
+        return;
+    }
     if (v == kVarSimple) {
         const int tiles = cdiv(a.W, kTW) * cdiv(a.H, kTH) * a.N;
         const bool tn64 = (a.Cout % 64 == 0) && (a.Cs % 64 == 0);
@@ -950,13 +990,18 @@ lgi_status conv_launch(const ConvArgs& a, int taps, hipStream_t st) {
         return LGI_ERR_INVALID_ARG;
     if (a.Cs < a.Cout && !a.out1) return LGI_ERR_INVALID_ARG;
     if (taps != 9 && taps != 1) return LGI_ERR_INVALID_ARG;
+    if (a.pool_out && (taps != 9 || a.Cs != a.Cout || a.mask0 || a.mask1 || a.H % 2 || a.W % 2 ||
+                       !row_grid_fits(a.N, a.H / 2, a.W / 2, a.Cout / 8)))
+        return LGI_ERR_INVALID_ARG;
     int v = g_force_variant.load();
     if (v >= 0) {
         if (!variant_fits(v, a, taps)) return LGI_ERR_UNSUPPORTED;
     } else {
         v = heuristic_variant(a, taps);
         if (taps == 9 && g_tune_mode.load() == 1) {
-            const std::array<int, 9> key{a.N, a.H, a.W, a.C0, a.C1, a.Cout, a.Cs, a.mask0 != nullptr, a.relu};
+            // input kind: 0 plain, 1 ReLU-masked (data gradient), 2 plain with a pooled copy
+            const int kind = a.mask0 != nullptr ? 1 : (a.pool_out ? 2 : 0);
+            const std::array<int, 9> key{a.N, a.H, a.W, a.C0, a.C1, a.Cout, a.Cs, kind, a.relu};
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
             if (it != g_tune.end()) {
@@ -991,6 +1036,17 @@ lgi_status lgi_unet_conv3x3_fwd(const void* src0, int32_t c0, const void* src1, 
     a.wk = (const bf16*)w; a.bias = bias; a.out0 = (bf16*)out; a.out1 = nullptr;
     a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu;
     if (!src0 || !w || !out) return LGI_ERR_INVALID_ARG;
+    return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1, const void* w,
+                                     const float* bias, int32_t cout, int32_t n, int32_t h, int32_t wd, int32_t relu,
+                                     void* out, void* pool_out, void* stream) {
+    ConvArgs a{};
+    a.src0 = (const bf16*)src0; a.src1 = (const bf16*)src1; a.C0 = c0; a.C1 = src1 ? c1 : 0;
+    a.wk = (const bf16*)w; a.bias = bias; a.out0 = (bf16*)out; a.out1 = nullptr;
+    a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu; a.pool_out = (bf16*)pool_out;
+    if (!src0 || !w || !out || !pool_out || h % 2 || wd % 2) return LGI_ERR_INVALID_ARG;
     return conv_launch(a, 9, (hipStream_t)stream);
 }
 
@@ -1076,11 +1132,6 @@ lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32
     }
     *n = i;
     return LGI_OK;
-}
-
-// The row-mapped kernels need n * rows <= 65535 (grid y) and cols * C/8 < 2^31.
-static bool row_grid_fits(int64_t n, int64_t rows, int64_t cols, int64_t c8) {
-    return n > 0 && rows > 0 && cols > 0 && n * rows <= 65535 && cols * c8 < (int64_t(1) << 31);
 }
 
 lgi_status lgi_unet_maxpool_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {

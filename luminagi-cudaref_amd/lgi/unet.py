@@ -68,7 +68,7 @@ def conv_tune_table():
     rows = np.zeros((max(cap, 1), 10), np.int32)
     us = np.zeros((max(cap, 1), len(CONV_VARIANTS)), np.float32)
     _call("lgi_unet_conv_tune_table", C.c_void_p(rows.ctypes.data), C.c_void_p(us.ctypes.data), cap, C.byref(n))
-    keys = ("n", "h", "w", "c0", "c1", "cout", "cout_split", "masked", "relu")
+    keys = ("n", "h", "w", "c0", "c1", "cout", "cout_split", "kind", "relu")
     return [(dict(zip(keys, map(int, r[:9]))), int(r[9]), {v: float(t) for v, t in enumerate(u) if t > 0})
             for r, u in zip(rows[:min(cap, n.value)], us)]
 
@@ -85,6 +85,21 @@ def conv3x3_fwd(src0, src1, w_bf16, bias, relu):
     _call("lgi_unet_conv3x3_fwd", _p(src0), c0, _p(src1), c1, _p(w_bf16), _p(bias), cout, n, h, w, int(relu),
           _p(out), _stream())
     return out
+
+
+def conv3x3_pool_fwd(src0, src1, w_bf16, bias, relu):
+    """conv3x3_fwd and its 2x2 max-pool from one kernel: returns (y, maxpool2(y))."""
+    n, h, w, c0 = src0.shape
+    c1 = src1.shape[-1] if src1 is not None else 0
+    cout = w_bf16.shape[0]
+    _check(src0, torch.bfloat16)
+    if src1 is not None:
+        _check(src1, torch.bfloat16)
+    out = torch.empty((n, h, w, cout), dtype=torch.bfloat16, device=src0.device)
+    pooled = torch.empty((n, h // 2, w // 2, cout), dtype=torch.bfloat16, device=src0.device)
+    _call("lgi_unet_conv3x3_pool_fwd", _p(src0), c0, _p(src1), c1, _p(w_bf16), _p(bias), cout, n, h, w, int(relu),
+          _p(out), _p(pooled), _stream())
+    return out, pooled
 
 
 def conv3x3_first_fwd(x, w_bf16, bias, relu=True):
@@ -289,15 +304,21 @@ class UNet(torch.nn.Module):
         x = x.contiguous()
         skips = []
         for i in range(len(ENC)):
-            if i == 0 and not torch.is_grad_enabled():
-                # inference: the first conv reads the 3-channel input directly
-                h = self.conv("enc0b", conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"]))
+            if not torch.is_grad_enabled():
+                # inference: the first conv reads the 3-channel input directly, and the second
+                # conv of each level writes its max-pooled copy from the same kernel
+                if i == 0:
+                    y = conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"])
+                else:
+                    y = self.conv(f"enc{i}a", h)
+                h, pooled = conv3x3_pool_fwd(y, None, self._wb(f"enc{i}b"), self.biases[f"enc{i}b"], True)
             else:
                 h = self._block(f"enc{i}", pack_input(x) if i == 0 else h)
+                pooled = None
             skips.append(h)
             if taps is not None:
                 taps[f"enc{i}"] = h
-            h = MaxPoolFn.apply(h)
+            h = MaxPoolFn.apply(h) if pooled is None else pooled
         h = self._block("mid_", h)
         if taps is not None:
             taps["mid"] = h

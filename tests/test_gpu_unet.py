@@ -312,3 +312,69 @@ def test_first_layer_kernel_matches_packed_path_and_fp64(gpu):
     d = (y1.float() - y0.float()).abs().max().item() / y0.float().abs().max().item()
     print(f"first layer: rel err {e1:.2e} (packed path {e0:.2e}), max diff between paths {d:.2e}", flush=True)
     assert e1 < BF16_TOL and d < 1e-2
+
+
+POOL_CASES = [  # n, h, w, c0, c1, cout
+    (2, 16, 32, 32, 0, 32),
+    (1, 14, 46, 64, 0, 64),     # ragged tiles in both directions
+    (1, 6, 70, 128, 0, 128),    # ragged 32-column tiles
+    (1, 10, 18, 32, 32, 64),    # concatenated input
+]
+
+
+@pytest.mark.parametrize("variant", [None, 0, 1, 2, 3, 4, 5, 6, 7],
+                         ids=lambda v: "tuned" if v is None else f"variant{v}")
+@pytest.mark.parametrize("case", POOL_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_pool_equals_conv_then_maxpool(gpu, variant, case):
+    """The conv with its pooled copy (written from the output tile by the 32x32-MFMA tilings,
+    by a separate pool launch after the others): y bit-identical to the plain conv with the
+    same tiling, the pooled copy bit-identical to MaxPool2d(2) of y."""
+    from lgi import unet
+    n, h, w, c0, c1, cout = case
+    if variant in (1, 4, 5) and cout % 64:
+        pytest.skip("64-channel tiling needs cout % 64 == 0")
+    x0 = bf16_rand(n, h, w, c0, seed=31).cuda()
+    x1 = bf16_rand(n, h, w, c1, seed=32).cuda() if c1 else None
+    wt = bf16_rand(cout, 9, c0 + c1, scale=(2.0 / (9 * (c0 + c1))) ** 0.5, seed=33).cuda()
+    b = (torch.randn(cout, generator=torch.Generator().manual_seed(34)) * 0.1).cuda()
+    unet.conv_tuning(True)
+    unet.force_conv_variant(variant)
+    try:
+        y, p = unet.conv3x3_pool_fwd(x0, x1, wt, b, True)
+        if variant is None:
+            chosen = [r for r in unet.conv_tune_table()
+                      if (r[0]["h"], r[0]["w"], r[0]["c0"], r[0]["c1"], r[0]["cout"], r[0]["kind"]) ==
+                      (h, w, c0, c1, cout, 2)]
+            assert len(chosen) == 1 and chosen[0][1] == min(chosen[0][2], key=chosen[0][2].get)
+            unet.force_conv_variant(chosen[0][1])
+        y0 = unet.conv3x3_fwd(x0, x1, wt, b, True)
+    finally:
+        unet.force_conv_variant(None)
+    p0 = unet.MaxPoolFn.apply(y0)
+    assert p.shape == (n, h // 2, w // 2, cout)
+    assert torch.equal(y, y0) and torch.equal(p, p0)
+    assert rel(y, ref_conv_nhwc(x0.cpu(), None if x1 is None else x1.cpu(), wt.cpu(), b.cpu(), True)) < BF16_TOL
+
+
+def test_conv3x3_pool_rejects_odd_sizes(gpu):
+    from lgi import unet
+    from lgi._abi import LgiError
+    w = torch.zeros(32, 9, 32, dtype=torch.bfloat16, device="cuda")
+    for h, wd in ((7, 16), (8, 15)):
+        with pytest.raises(LgiError, match="INVALID_ARG"):
+            unet.conv3x3_pool_fwd(torch.zeros(1, h, wd, 32, dtype=torch.bfloat16, device="cuda"), None, w,
+                                  torch.zeros(32, device="cuda"), True)
+
+
+def test_unet_inference_forward_matches_training_forward(gpu):
+    """Inference runs the first-layer kernel and the pooled convs; training runs the plain
+    layers. Both are BF16 paths of the same network, so they agree to BF16 rounding."""
+    model, x = make_model_and_input(seed=5, n=1, h=48, w=80)
+    x = x.cuda()
+    with torch.no_grad():
+        yi = model(x)
+    with torch.enable_grad():
+        yt = model(x).detach()
+    e = rel(yi, yt)
+    print(f"inference vs training forward: rel diff {e:.2e}", flush=True)
+    assert e < 2e-2
