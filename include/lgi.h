@@ -279,6 +279,12 @@ LGI_API lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const
 LGI_API lgi_status lgi_unet_conv3x3_first_fwd(const float* x, const void* w, int32_t cin_w, const float* bias,
                                               int32_t n, int32_t h, int32_t w_, int32_t relu, void* out,
                                               void* stream);
+/* lgi_unet_conv3x3_first_fwd on log1p(max(hdr, 0)) of the radiance image hdr, the transform
+ * applied inside the layer's input read: the denoiser's input conversion of tools/denoise.py
+ * (radiance in) without a separate pass over the frame. */
+LGI_API lgi_status lgi_unet_conv3x3_first_radiance_fwd(const float* hdr, const void* w, int32_t cin_w,
+                                                       const float* bias, int32_t n, int32_t h, int32_t w_,
+                                                       int32_t relu, void* out, void* stream);
 /* dx = conv3x3^T(dy * (ymask > 0)); channels [0, cin_split) -> dx0, the rest -> dx1.
  * ymask may be null (no ReLU); wt_scratch holds cout*9*cin BF16. */
 LGI_API lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout,
@@ -316,6 +322,10 @@ LGI_API lgi_status lgi_unet_pack_input(const float* x, int64_t npx, void* out, v
 /* out = x + W_t feat + b_t (1x1, 32 -> 3, FP32 out); backward gives dfeat, dW_t, db_t */
 LGI_API lgi_status lgi_unet_tail_fwd(const void* feat, const float* x, const float* w, const float* b, int64_t npx,
                                      float* out, void* stream);
+/* Inference tail in the radiance domain: out = max(expm1(log1p(max(hdr, 0)) + conv1x1(feat)), 0),
+ * the denoiser's output conversion of tools/denoise.py (radiance out) inside the tail. */
+LGI_API lgi_status lgi_unet_tail_radiance_fwd(const void* feat, const float* hdr, const float* w, const float* b,
+                                              int64_t npx, float* out, void* stream);
 LGI_API lgi_status lgi_unet_tail_bwd(const void* feat, const float* dout, const float* w, int64_t npx, void* dfeat,
                                      float* dw, float* db, float* scratch, int64_t scratch_floats, void* stream);
 

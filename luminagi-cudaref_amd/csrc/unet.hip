@@ -463,6 +463,16 @@ __global__ __launch_bounds__(kBlock, RW * NB >= 32 ? 1 : 2) void k_conv_rb(ConvA
 // one K = 27 (+5 zero) row, so a 16-pixel x 16-channel output fragment is a single MFMA
 // instead of the 9 taps x 32 (mostly zero) channels of the packed-input path. Output tile
 // 8 x 16 pixels x 32 channels; weights [32][9][cin_w] BF16 (channels 0..2 used).
+// Denoiser input transform: radiance -> log1p(max(radiance, 0)), NaN kept (torch's clamp_min
+// and log1p order), and its inverse on the output, max(expm1(v), 0).
+__device__ __forceinline__ float to_log(float v) { return log1pf(v < 0.f ? 0.f : v); }
+__device__ __forceinline__ float from_log(float v) {
+    const float r = expm1f(v);
+    return r < 0.f ? 0.f : r;
+}
+
+// kLog: x holds radiance and the layer reads log1p(max(x, 0)).
+template <bool kLog>
 __global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__ x, const bf16* __restrict__ wk,
                                                        int cin_w, const float* __restrict__ bias, int N, int H, int W,
                                                        int relu, bf16* __restrict__ out) {
@@ -481,6 +491,9 @@ __global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__
         if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
             const float* px = x + (((int64_t)n * H + gy) * W + gx) * 3;
             v0 = px[0]; v1 = px[1]; v2 = px[2];
+            if (kLog) {
+                v0 = to_log(v0); v1 = to_log(v1); v2 = to_log(v2);
+            }
         }
         s_in[t * 4 + 0] = (bf16)v0; s_in[t * 4 + 1] = (bf16)v1; s_in[t * 4 + 2] = (bf16)v2;
         s_in[t * 4 + 3] = (bf16)0.f;
@@ -799,6 +812,8 @@ __global__ __launch_bounds__(kBlock) void k_pack_input(const float* x, int64_t n
 }
 
 // out = x + W_t * feat + b_t (1x1 conv, 32 -> 3, FP32 math).
+// kRadiance: x holds radiance; the residual adds to log1p(max(x, 0)) and out is max(expm1(.), 0).
+template <bool kRadiance>
 __global__ __launch_bounds__(kBlock) void k_tail_fwd(const bf16* feat, const float* x, const float* w, const float* b,
                                                      int64_t npx, float* out) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -813,7 +828,8 @@ __global__ __launch_bounds__(kBlock) void k_tail_fwd(const bf16* feat, const flo
     for (int o = 0; o < 3; ++o) {
         float s = b[o];
         for (int c = 0; c < 32; ++c) s += w[o * 32 + c] * f[c];
-        out[p * 3 + o] = x[p * 3 + o] + s;
+        if (kRadiance) out[p * 3 + o] = from_log(to_log(x[p * 3 + o]) + s);
+        else out[p * 3 + o] = x[p * 3 + o] + s;
     }
 }
 
@@ -1050,13 +1066,28 @@ lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const void* s
     return conv_launch(a, 9, (hipStream_t)stream);
 }
 
-lgi_status lgi_unet_conv3x3_first_fwd(const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,
-                                      int32_t h, int32_t wd, int32_t relu, void* out, void* stream) {
+static lgi_status first_fwd(bool log_in, const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,
+                            int32_t h, int32_t wd, int32_t relu, void* out, void* stream) {
     if (!x || !w || !out || cin_w < 3 || n <= 0 || h <= 0 || wd <= 0) return LGI_ERR_INVALID_ARG;
     const int tiles = cdiv(wd, kTW) * cdiv(h, kTH) * n;
-    hipLaunchKernelGGL(k_conv_first, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w, cin_w,
-                       bias, n, h, wd, relu, (bf16*)out);
+    if (log_in)
+        hipLaunchKernelGGL(k_conv_first<true>, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
+                           cin_w, bias, n, h, wd, relu, (bf16*)out);
+    else
+        hipLaunchKernelGGL(k_conv_first<false>, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
+                           cin_w, bias, n, h, wd, relu, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_conv3x3_first_fwd(const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,
+                                      int32_t h, int32_t wd, int32_t relu, void* out, void* stream) {
+    return first_fwd(false, x, w, cin_w, bias, n, h, wd, relu, out, stream);
+}
+
+lgi_status lgi_unet_conv3x3_first_radiance_fwd(const float* hdr, const void* w, int32_t cin_w, const float* bias,
+                                               int32_t n, int32_t h, int32_t wd, int32_t relu, void* out,
+                                               void* stream) {
+    return first_fwd(true, hdr, w, cin_w, bias, n, h, wd, relu, out, stream);
 }
 
 lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const void* w, int32_t cout, int32_t cin,
@@ -1173,9 +1204,17 @@ lgi_status lgi_unet_pack_input(const float* x, int64_t npx, void* out, void* str
 
 lgi_status lgi_unet_tail_fwd(const void* feat, const float* x, const float* w, const float* b, int64_t npx, float* out,
                              void* stream) {
-    if (!feat || !x || !w || !b || !out) return LGI_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(k_tail_fwd, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, (const bf16*)feat, x, w,
-                       b, npx, out);
+    if (!feat || !x || !w || !b || !out || npx <= 0) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_tail_fwd<false>, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const bf16*)feat, x, w, b, npx, out);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_tail_radiance_fwd(const void* feat, const float* hdr, const float* w, const float* b, int64_t npx,
+                                      float* out, void* stream) {
+    if (!feat || !hdr || !w || !b || !out || npx <= 0) return LGI_ERR_INVALID_ARG;
+    hipLaunchKernelGGL(k_tail_fwd<true>, dim3(cdiv(npx, kBlock)), dim3(kBlock), 0, (hipStream_t)stream,
+                       (const bf16*)feat, hdr, w, b, npx, out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 

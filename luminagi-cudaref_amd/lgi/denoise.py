@@ -61,8 +61,8 @@ class Frame:
 def denoise(model, hdr):
     """UNet on log1p radiance; hdr HxWx3 (H, W multiples of 16) -> denoised radiance."""
     with torch.no_grad():
-        y = model(torch.log1p(hdr.clamp_min(0)).unsqueeze(0).contiguous())
-    return torch.expm1(y[0]).clamp_min(0)
+        # log1p(max(hdr, 0)) in, max(expm1(y), 0) out, both inside the first and last kernels
+        return model(hdr.unsqueeze(0).contiguous(), radiance=True)[0]
 
 
 def radiance_input(png_u8, hdr_path=None):

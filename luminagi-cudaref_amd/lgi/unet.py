@@ -102,14 +102,26 @@ def conv3x3_pool_fwd(src0, src1, w_bf16, bias, relu):
     return out, pooled
 
 
-def conv3x3_first_fwd(x, w_bf16, bias, relu=True):
-    """First layer straight from the FP32 3-channel input x [N,H,W,3] -> BF16 [N,H,W,32]."""
+def conv3x3_first_fwd(x, w_bf16, bias, relu=True, radiance=False):
+    """First layer straight from the FP32 3-channel input x [N,H,W,3] -> BF16 [N,H,W,32];
+    radiance=True: x is radiance and the layer reads log1p(max(x, 0))."""
     n, h, w, c = x.shape
     _check(x, torch.float32, 3)
     assert w_bf16.shape[0] == 32 and w_bf16.dtype == torch.bfloat16 and w_bf16.is_contiguous()
     out = torch.empty((n, h, w, 32), dtype=torch.bfloat16, device=x.device)
-    _call("lgi_unet_conv3x3_first_fwd", _p(x), _p(w_bf16), w_bf16.shape[2], _p(bias), n, h, w, int(relu), _p(out),
-          _stream())
+    name = "lgi_unet_conv3x3_first_radiance_fwd" if radiance else "lgi_unet_conv3x3_first_fwd"
+    _call(name, _p(x), _p(w_bf16), w_bf16.shape[2], _p(bias), n, h, w, int(relu), _p(out), _stream())
+    return out
+
+
+def tail_radiance_fwd(feat, hdr, w, b):
+    """Inference tail in the radiance domain: max(expm1(log1p(max(hdr, 0)) + conv1x1(feat)), 0)."""
+    n, h, wd, c = feat.shape
+    _check(feat, torch.bfloat16, 32)
+    _check(hdr, torch.float32, 3)
+    out = torch.empty((n, h, wd, 3), dtype=torch.float32, device=feat.device)
+    _call("lgi_unet_tail_radiance_fwd", _p(feat), _p(hdr), _p(w.contiguous()), _p(b.contiguous()), n * h * wd,
+          _p(out), _stream())
     return out
 
 
@@ -299,16 +311,20 @@ class UNet(torch.nn.Module):
             return checkpoint(self.block, prefix, a, b, use_reentrant=False)
         return self.block(prefix, a, b)
 
-    def forward(self, x, taps=None):
-        """taps (optional dict) receives every level's activation for per-layer error reports."""
+    def forward(self, x, taps=None, radiance=False):
+        """taps (optional dict) receives every level's activation for per-layer error reports.
+        radiance=True (inference only): x is radiance, and the log1p(max(x, 0)) input transform
+        and the max(expm1(y), 0) output transform run inside the first layer and the tail."""
         x = x.contiguous()
+        if radiance and torch.is_grad_enabled():
+            raise ValueError("radiance=True is an inference path; run it under torch.no_grad()")
         skips = []
         for i in range(len(ENC)):
             if not torch.is_grad_enabled():
                 # inference: the first conv reads the 3-channel input directly, and the second
                 # conv of each level writes its max-pooled copy from the same kernel
                 if i == 0:
-                    y = conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"])
+                    y = conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"], radiance=radiance)
                 else:
                     y = self.conv(f"enc{i}a", h)
                 h, pooled = conv3x3_pool_fwd(y, None, self._wb(f"enc{i}b"), self.biases[f"enc{i}b"], True)
@@ -326,6 +342,8 @@ class UNet(torch.nn.Module):
             h = self._block(f"dec{i}", UpsampleFn.apply(h), skips[i])
             if taps is not None:
                 taps[f"dec{i}"] = h
+        if radiance:
+            return tail_radiance_fwd(h, x, self.tail_w, self.tail_b)
         return TailFn.apply(h, x, self.tail_w, self.tail_b)
 
     def state_for_checkpoint(self):

@@ -378,3 +378,29 @@ def test_unet_inference_forward_matches_training_forward(gpu):
     e = rel(yi, yt)
     print(f"inference vs training forward: rel diff {e:.2e}", flush=True)
     assert e < 2e-2
+
+
+def test_radiance_entry_points_match_torch_transforms(gpu):
+    """The first layer reading log1p(max(hdr, 0)) and the tail writing max(expm1(.), 0) against
+    torch's log1p/clamp_min/expm1 around the log-domain network, with negative radiance in the
+    input (clamped) and a radiance range of 0..60."""
+    from lgi import unet
+    from lgi import denoise as D
+    model, _ = make_model_and_input(seed=7)
+    g = torch.Generator().manual_seed(8)
+    hdr = (torch.rand(48, 80, 3, generator=g) ** 4 * 60.0)
+    hdr[::7, ::5, 1] = -0.5
+    hdr = hdr.cuda()
+    with torch.no_grad():
+        y_new = D.denoise(model, hdr)
+        y_old = torch.expm1(model(torch.log1p(hdr.clamp_min(0)).unsqueeze(0).contiguous())[0]).clamp_min(0)
+        f0 = unet.conv3x3_first_fwd(torch.log1p(hdr.clamp_min(0)).unsqueeze(0).contiguous(), model._wb("enc0a"),
+                                    model.biases["enc0a"])
+        f1 = unet.conv3x3_first_fwd(hdr.unsqueeze(0).contiguous(), model._wb("enc0a"), model.biases["enc0a"],
+                                    radiance=True)
+    e = rel(y_new, y_old)
+    print(f"radiance path vs torch transforms: rel diff {e:.2e}, first layer equal {torch.equal(f0, f1)}", flush=True)
+    assert bool((y_new >= 0).all()) and e < 1e-3
+    assert torch.equal(f0, f1) or rel(f1, f0) < 1e-2
+    with pytest.raises(ValueError):
+        model(hdr.unsqueeze(0), radiance=True)
