@@ -196,6 +196,15 @@ LGI_API lgi_status lgi_last_render_ms(const lgi_ctx* ctx, float* ms);
  * shadow-ray launch; for the inline renderer the one megakernel launch) and their count. */
 LGI_API lgi_status lgi_last_trace_ms(const lgi_ctx* ctx, float* ms, int32_t* launches);
 
+/* Pass plan of the sorted (wavefront) renderer, host only; no reference counterpart (the
+ * reference renders 8-sample chunks, src/pathtracer.cu:673-704). For n_slots queue slots per
+ * sample (64 per 8x8 tile) and n_points point lights, *samples = the samples each pass of
+ * LGI_RENDER_SORTED holds (<= spp): about 100M paths in flight, at most 3 x that many
+ * shadow-ray candidates (slot x light, sun included) and never more than 2^31 - 1 of them.
+ * LGI_ERR_UNSUPPORTED when one sample alone exceeds 2^31 - 1 candidates (the sorted renderer
+ * then refuses the image with the same status). */
+LGI_API lgi_status lgi_wavefront_samples_per_pass(int64_t n_slots, int32_t n_points, int32_t spp, int32_t* samples);
+
 /* ---- many poses / selected tiles in one persistent launch ----
  * No reference counterpart beyond the --batch loop (src/main.cpp:158-189), which re-uploads the
  * scene and launches once per job. Work is split into (pose, 8x8-pixel tile, spp, first

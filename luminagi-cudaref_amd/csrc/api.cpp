@@ -199,9 +199,13 @@ lgi_status run_queue(lgi_ctx* ctx, const std::vector<lgi_work_item>& items, cons
     P.comp = comp;
     P.sumsq = sumsq;
     if ((st = attach_spill(ctx, P, (int64_t)kQueueGrid * 256)) != LGI_OK) return st;
-    auto* d_items = (lgi_work_item*)ctx->scratch_get(45, std::max<size_t>(1, items.size()) * sizeof(lgi_work_item));
-    auto* d_cams = (DevCamera*)ctx->scratch_get(46, std::max<size_t>(1, cams.size()) * sizeof(DevCamera));
-    auto* counter = (uint32_t*)ctx->scratch_get(47, 64);
+    // The captured graph reads its items, cameras and counter from slots of its own (72-74), so
+    // that direct launches (slots 45-47) never overwrite what a later replay reads.
+    const int slot0 = use_graph ? 72 : 45;
+    auto* d_items =
+        (lgi_work_item*)ctx->scratch_get(slot0, std::max<size_t>(1, items.size()) * sizeof(lgi_work_item));
+    auto* d_cams = (DevCamera*)ctx->scratch_get(slot0 + 1, std::max<size_t>(1, cams.size()) * sizeof(DevCamera));
+    auto* counter = (uint32_t*)ctx->scratch_get(slot0 + 2, 64);
     if (!d_items || !d_cams || !counter) return lgi_set_err(ctx, LGI_ERR_HIP, "queue allocation failed");
     // key of the captured graph: everything the kernels read by value or address
     std::string key((const char*)items.data(), items.size() * sizeof(lgi_work_item));
@@ -215,8 +219,12 @@ lgi_status run_queue(lgi_ctx* ctx, const std::vector<lgi_work_item>& items, cons
     if (use_graph && ctx->graph_exec && ctx->graph_key == key) {
         LGI_HIP(ctx, hipGraphLaunch((hipGraphExec_t)ctx->graph_exec, ctx->stream));
     } else {
-        LGI_HIP(ctx, hipMemcpy(d_items, items.data(), items.size() * sizeof(lgi_work_item), hipMemcpyHostToDevice));
-        LGI_HIP(ctx, hipMemcpy(d_cams, cams.data(), cams.size() * sizeof(DevCamera), hipMemcpyHostToDevice));
+        // Ordered on the context's stream after any earlier launch that still reads these slots.
+        // The host vectors are pageable, so each copy has been staged when the call returns.
+        LGI_HIP(ctx, hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(lgi_work_item),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        LGI_HIP(ctx, hipMemcpyAsync(d_cams, cams.data(), cams.size() * sizeof(DevCamera), hipMemcpyHostToDevice,
+                                    ctx->stream));
         if (!use_graph) {
             lgi_launch_render_queue(P, d_cams, d_items, (int)items.size(), counter, grid, ctx->stream);
         } else {

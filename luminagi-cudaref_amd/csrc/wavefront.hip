@@ -630,6 +630,28 @@ void launch_trace(hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, con
 
 }  // namespace
 
+extern "C" lgi_status lgi_wavefront_samples_per_pass(int64_t n_slots, int32_t n_points, int32_t spp,
+                                                     int32_t* samples) {
+    if (!samples || n_slots <= 0 || n_points < 0 || spp <= 0) return LGI_ERR_INVALID_ARG;
+    *samples = 0;
+    // Paths in flight per pass: up to ~100M (about 25 GB of queues and path state, of the 288 GB
+    // of HBM), so a 64-spp 1728x864 frame is one pass. More rays per launch make each XCD's
+    // queue segment and the ray sort denser: on the 1.05M-triangle frame, one 64-spp pass traces
+    // in 19.2 ms what four 16-spp passes traced in 26.2 ms.
+    constexpr int64_t kMaxPaths = int64_t(96) << 20;
+    // Shadow-ray candidates (slot x light) per pass: as many as the paths of a sun + 2-point-light
+    // scene, so a scene with more lights runs fewer samples per pass instead of more memory. The
+    // candidate ids slot * lights + light, the queue lengths and the launch sizes are 32-bit, so a
+    // pass never exceeds 2^31 - 1 candidates; when one sample does, the image is refused.
+    constexpr int64_t kMaxCandidates = 3 * kMaxPaths;
+    const int64_t nl = 1 + (int64_t)n_points;
+    const int64_t per_sample = n_slots * nl;
+    if (n_slots > INT32_MAX || per_sample > INT32_MAX) return LGI_ERR_UNSUPPORTED;
+    const int64_t s = std::min<int64_t>({spp, kMaxPaths / n_slots, kMaxCandidates / per_sample});
+    *samples = (int32_t)std::max<int64_t>(1, s);
+    return LGI_OK;
+}
+
 // Host driver of the wavefront renderer. Adds spp samples into P.sum/P.comp/P.sumsq.
 lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max,
                                 const int32_t* d_tiles, int n_tiles) {
@@ -639,12 +661,11 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     const int tiles_x = (Wd + 7) / 8, tiles_y = (Hd + 7) / 8;
     const int padded = (d_tiles ? n_tiles : tiles_x * tiles_y) * 64;
     const int nl = 1 + P.n_points;
-    // Samples per pass: up to ~100M paths in flight (about 25 GB of queues and path state, of the
-    // 288 GB of HBM), so a 64-spp 1728x864 frame is one pass. More rays per launch make each
-    // XCD's queue segment and the ray sort denser: on the 1.05M-triangle frame, one 64-spp pass
-    // traces in 19.2 ms what four 16-spp passes traced in 26.2 ms.
-    const int max_paths = 96 << 20;
-    const int S_max = std::max(1, std::min(P.spp, max_paths / std::max(1, padded)));
+    int32_t S_max = 0;
+    if (lgi_wavefront_samples_per_pass(padded, P.n_points, std::max(1, P.spp), &S_max) != LGI_OK)
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED,
+                           "wavefront renderer: one sample of this image times its lights exceeds 2^31 "
+                           "shadow-ray candidates");
     const int64_t cap = (int64_t)S_max * padded;
     auto S_ = [&](int slot, size_t bytes) { return ctx->scratch_get(slot, bytes); };
     QRay* qa = (QRay*)S_(50, cap * sizeof(QRay));

@@ -101,6 +101,7 @@ SIGNATURES = {
                                     C.c_int32]),
     "lgi_render_tiles": (C.c_int, [_P, C.POINTER(Camera), _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint32,
                                     _P, _P, _P]),
+    "lgi_wavefront_samples_per_pass": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_batch_parse": (C.c_int, [C.c_char_p, C.POINTER(BatchJob), C.c_int32, C.POINTER(C.c_int32)]),
     "lgi_write_png": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32]),
     "lgi_read_png": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _P]),

@@ -51,6 +51,14 @@ def write_png(path, rgb):
     _check(None, lib().lgi_write_png(str(path).encode(), ptr(rgb), w, h))
 
 
+def wavefront_samples_per_pass(n_slots, n_points, spp):
+    """Samples per pass of the sorted renderer for n_slots queue slots per sample (64 per 8x8
+    tile) and n_points point lights (host arithmetic only)."""
+    s = C.c_int32()
+    _check(None, lib().lgi_wavefront_samples_per_pass(n_slots, n_points, spp, C.byref(s)))
+    return s.value
+
+
 class Context:
     """One liblgi context: a scene, its device copy and BVH, and the render entry points."""
 

@@ -34,6 +34,18 @@
 //                     closed forms (sun on an unoccluded plane, point-light falloff),
 //                     statistical cos-hemisphere moments, FP64 sum agreement
 //   orc_gbuffer/tonemap pinned: closed-form encodings
+//   trace_path bounces pinned: closed interior of a sphere lit by a point light at its centre
+//                     (every hit has the same direct term D, so the 2-bounce mean is
+//                     D (1 + rho + rho^2) exactly when Russian roulette is unbiased)
+//   traverse_occluded pinned: shadow footprint of an opaque plate over a ground plane
+//
+// Planted mistakes: building with -DORC_MUTANT=k (k > 0) plants one known mistake so that the
+// tests can show the closed-form pins reject it (1: Russian roulette without the 1/q weight,
+// 2: sun shadow test inverted, 3: bounce origin offset along -N, 4: G-buffer normal not turned
+// towards the camera). The library the tests load is built with ORC_MUTANT = 0.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -351,7 +363,7 @@ inline V3 shade_direct(const SceneView& s, V3 P, V3 N, V3 albedo, Counters& c) {
         float ndotl = std::fmax(0.0f, dot(N, toLight));
         if (ndotl > 0.0f) {
             Ray shadow = make_ray(add(P, mul(N, kShadowBias)), toLight);
-            if (!traverse_occluded(s, shadow, 1e30f, c)) {
+            if (traverse_occluded(s, shadow, 1e30f, c) == (ORC_MUTANT == 2)) {
                 float I = s.sun.intensity * ndotl;
                 color = add(color, had(albedo, mul(vfrom(s.sun.color), I)));
             }
@@ -436,10 +448,10 @@ inline V3 trace_path(const SceneView& s, Ray ray, int maxBounces, Rng& rng, bool
         if (bounce >= 1 && !no_rr) {
             float q = std::fmin(0.95f, std::fmax(throughput.x, std::fmax(throughput.y, throughput.z)));
             if (rng.next() > q) break;
-            throughput = mul(throughput, 1.0f / q);
+            if (ORC_MUTANT != 1) throughput = mul(throughput, 1.0f / q);
         }
         V3 newDir = sample_cos_hemi(N, rng);
-        ray = make_ray(add(P, mul(N, 1e-3f)), newDir);
+        ray = make_ray(add(P, mul(N, ORC_MUTANT == 3 ? -1e-3f : 1e-3f)), newDir);
         throughput = had(throughput, albedo);
     }
     return L;
@@ -853,7 +865,7 @@ void orc_gbuffer(const OrcRenderArgs* a, uint8_t* alb, uint8_t* nrm, uint8_t* wp
                 wp[i] = wp[i + 1] = wp[i + 2] = 0;
                 continue;
             }
-            if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
+            if (dot(N, r.d) > 0.0f && ORC_MUTANT != 4) N = mul(N, -1.0f);
             const V3 A = sample_albedo(s, s.tris[tri].material, tu, tv);
             alb[i] = enc(A.x); alb[i + 1] = enc(A.y); alb[i + 2] = enc(A.z);
             nrm[i] = enc(N.x * 0.5f + 0.5f); nrm[i + 1] = enc(N.y * 0.5f + 0.5f); nrm[i + 2] = enc(N.z * 0.5f + 0.5f);

@@ -5,6 +5,7 @@ and only as the checker (or the timed CPU baseline) — never as a product path.
 Every function here forwards to oracle.cpp, which restates the reference algorithms with
 file:line citations.
 """
+import contextlib
 import ctypes as C
 import pathlib
 import subprocess
@@ -65,33 +66,58 @@ def lib():
     if _lib is None:
         if not LIB.exists():
             raise RuntimeError(f"{LIB} missing: build with `make -C {HERE}`")
-        l = C.CDLL(str(LIB))
-        V = C.c_void_p
-        sig = {
-            "orc_morton3": (C.c_uint32, [C.c_float] * 3),
-            "orc_tex_sample": (None, [V, V, V, V, C.c_int32, C.c_float, C.c_float, V]),
-            "orc_spread3": (C.c_uint32, [C.c_uint32]),
-            "orc_sincos": (None, [V, V, V, C.c_int64]),
-            "orc_pcg32_stream": (None, [C.c_uint64, C.c_uint64, V, C.c_int]),
-            "orc_pixel_randf": (None, [C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_int]),
-            "orc_xorshift_stream": (None, [C.c_uint32, V, C.c_int]),
-            "orc_sample_cos_hemi": (None, [V, C.c_uint64, C.c_uint64, V, C.c_int]),
-            "orc_morton_codes": (None, [V, C.c_int64, V]),
-            "orc_bvh_build": (C.c_int64, [V, C.c_int64, V, V, V]),
-            "orc_tan_half_fov": (C.c_float, [C.c_float]),
-            "orc_camera_look_at": (None, [V, V, V]),
-            "orc_render": (None, [C.POINTER(RenderArgs), V, V, V, V, V]),
-            "orc_primary_hits": (None, [C.POINTER(RenderArgs), V, V]),
-            "orc_brute_closest": (C.c_int32, [V, C.c_int64, V, V, V]),
-            "orc_bvh_closest": (C.c_int32, [V, V, V, V, V]),
-            "orc_gbuffer": (None, [C.POINTER(RenderArgs), V, V, V]),
-            "orc_tonemap": (None, [V, C.c_int64, C.c_float, V]),
-        }
-        for k, (r, a) in sig.items():
-            f = getattr(l, k)
-            f.restype, f.argtypes = r, a
-        _lib = l
+        _lib = _load(LIB)
     return _lib
+
+
+def build_mutant(k, out_dir):
+    """Builds oracle.cpp with planted mistake k (see ORC_MUTANT in oracle.cpp) into out_dir and
+    returns the library path. Only the oracle's own mutation tests use this."""
+    out = pathlib.Path(out_dir) / f"liblgi_oracle_mutant{k}.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-pthread",
+                    "-shared", f"-DORC_MUTANT={int(k)}", str(HERE / "oracle.cpp"), "-o", str(out)], check=True)
+    return out
+
+
+@contextlib.contextmanager
+def library(path):
+    """Routes every oracle call through another build of oracle.cpp while the block runs."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def _load(path):
+    l = C.CDLL(str(path))
+    V = C.c_void_p
+    sig = {
+        "orc_morton3": (C.c_uint32, [C.c_float] * 3),
+        "orc_tex_sample": (None, [V, V, V, V, C.c_int32, C.c_float, C.c_float, V]),
+        "orc_spread3": (C.c_uint32, [C.c_uint32]),
+        "orc_sincos": (None, [V, V, V, C.c_int64]),
+        "orc_pcg32_stream": (None, [C.c_uint64, C.c_uint64, V, C.c_int]),
+        "orc_pixel_randf": (None, [C.c_uint32, C.c_uint32, C.c_uint32, V, C.c_int]),
+        "orc_xorshift_stream": (None, [C.c_uint32, V, C.c_int]),
+        "orc_sample_cos_hemi": (None, [V, C.c_uint64, C.c_uint64, V, C.c_int]),
+        "orc_morton_codes": (None, [V, C.c_int64, V]),
+        "orc_bvh_build": (C.c_int64, [V, C.c_int64, V, V, V]),
+        "orc_tan_half_fov": (C.c_float, [C.c_float]),
+        "orc_camera_look_at": (None, [V, V, V]),
+        "orc_render": (None, [C.POINTER(RenderArgs), V, V, V, V, V]),
+        "orc_primary_hits": (None, [C.POINTER(RenderArgs), V, V]),
+        "orc_brute_closest": (C.c_int32, [V, C.c_int64, V, V, V]),
+        "orc_bvh_closest": (C.c_int32, [V, V, V, V, V]),
+        "orc_gbuffer": (None, [C.POINTER(RenderArgs), V, V, V]),
+        "orc_tonemap": (None, [V, C.c_int64, C.c_float, V]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(l, k)
+        f.restype, f.argtypes = r, a
+    return l
 
 
 def _p(a):

@@ -181,3 +181,30 @@ def test_treelet_passes_argument_checks():
     with pytest.raises(lgi.LgiError) as e:
         ctx.bvh_traversal()
     assert e.value.status == 7  # LGI_ERR_NO_BVH
+
+
+def test_wavefront_pass_plan_bounds_shadow_candidates():
+    """The sorted renderer's samples per pass (lgi_wavefront_samples_per_pass): ~100M paths, at most
+    3 x that many shadow-ray candidates (slot x light), never 2^31 or more of them; an image whose
+    single sample already exceeds that is refused."""
+    from lgi.tracer import wavefront_samples_per_pass as plan
+    max_paths = 96 << 20
+    slots = 216 * 108 * 64                     # 1728x864 in 8x8 tiles
+    assert plan(slots, 2, 64) == 64            # the bench frame is one pass
+    assert plan(slots, 2, 1000) == max_paths // slots
+    for n_points in (0, 2, 21, 22, 44, 45, 100, 1000):
+        s = plan(slots, n_points, 4096)
+        assert s >= 1
+        assert s * slots <= max_paths
+        assert s == 1 or s * slots * (1 + n_points) <= 3 * max_paths
+        assert s * slots * (1 + n_points) < 2 ** 31
+    assert plan(64, 0, 7) == 7
+    # one sample of a 2^30-pixel image with 2 lights exceeds 2^31 candidates: refused
+    with pytest.raises(lgi.LgiError) as e:
+        plan(1 << 30, 2, 1)
+    assert e.value.status == 8
+    with pytest.raises(lgi.LgiError) as e:
+        plan(slots, 20000, 1)
+    assert e.value.status == 8
+    with pytest.raises(lgi.LgiError):
+        plan(0, 2, 1)

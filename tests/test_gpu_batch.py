@@ -51,6 +51,7 @@ def test_batch_graph_replays_match_eager(gpu):
     torch.cuda.synchronize()
     s = torch.zeros(4, n * 3, device="cuda")
     c = torch.zeros(4, n * 3, device="cuda")
+    other = [torch.zeros(n * 3, device="cuda") for _ in range(2)]
     for rep in range(3):  # first call captures, later calls replay; eager work in between
         s.zero_()
         c.zero_()
@@ -58,6 +59,10 @@ def test_batch_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
         assert torch.equal(s, eager[0]) and torch.equal(c, eager[1]), f"replay {rep}"
         _ = (s * 2).sum().item()
+        # a direct queue launch with fewer items and another camera between replays: it must not
+        # change what the next replay reads
+        ctx.render_tiles([0, 3], 1 + rep, other[0], other[1], 1, camera=jobs[-1][0], flags=0)
+        ctx.render_batch(jobs[:2], eager[0][:2].clone(), eager[1][:2].clone(), 1, use_graph=False)
 
 
 @pytest.mark.parametrize("flags", [0, 2], ids=["queue", "sorted"])

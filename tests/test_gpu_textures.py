@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
+from parity import assert_close_images
 
 pytestmark = pytest.mark.gpu
 
@@ -39,11 +40,7 @@ def textured_scene(tmp_path, w=128, h=64):
 
 
 def compare(gpu_sum, cpu_sum, label):
-    g = gpu_sum.reshape(cpu_sum.shape).astype(np.float64)
-    c = cpu_sum.astype(np.float64)
-    bad = np.abs(g - c) > 1e-5 * np.abs(c) + 1e-12
-    print(f"{label}: exact {np.mean(g == c):.6f}, beyond 1e-5: {bad.sum()} of {bad.size}", flush=True)
-    assert bad.mean() <= 1e-4
+    assert_close_images(gpu_sum, cpu_sum, label)
 
 
 @pytest.mark.parametrize("spp,bounces,flags", [(1, 0, 0), (16, 2, 0), (16, 2, 2)],
