@@ -54,7 +54,7 @@ def test_bench_frame_matches_oracle(gpu):
     mism = gtri != otri
     print(f"primary hits: {int(mism.sum())} id mismatches of {mism.size}, t equal {np.mean(gt == ot):.6f}",
           flush=True)
-    assert (otri >= 0).mean() > 0.5
+    assert (otri >= 0).mean() > 0.3
     assert np.array_equal(gt, ot)             # hit distances bit-identical everywhere
     assert np.all(gt[mism] == ot[mism]) and mism.sum() <= 16   # other ids only at exact ties
     # 64 spp, 2 bounces: sorted wavefront and inline renderers give the same bits
