@@ -181,6 +181,21 @@ LGI_API lgi_status lgi_render(lgi_ctx* ctx, const lgi_camera* cam, int32_t spp, 
 LGI_API lgi_status lgi_render_accumulate(lgi_ctx* ctx, const lgi_camera* cam, int32_t spp,
                                          int32_t bounces, uint32_t flags, int32_t sample0,
                                          float* d_sum, float* d_comp, float* d_sumsq);
+/* lgi_render_accumulate without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments and sums as lgi_render_accumulate for
+ * flags without LGI_RENDER_SORTED (bit for bit): the whole image runs as one persistent queue
+ * of 8x8 tiles whose counter and traversal-stack spill slots live in a static device array of
+ * LGI_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends; its
+ * size does not depend on the image. The first call on a device resolves that device's array
+ * (hipGetSymbolAddress); no call allocates, frees or copies host memory. Calls that overlap on
+ * one GPU (other streams or threads) share the array and race; calls ordered on one stream do
+ * not. LGI_ERR_UNSUPPORTED for LGI_RENDER_SORTED (the wavefront renderer's queues take GBs) and
+ * when textures must be reloaded first (lgi_set_asset_root since the last render). */
+#define LGI_NOALLOC_SCRATCH_BYTES 25166080
+LGI_API lgi_status lgi_render_accumulate_noalloc(lgi_ctx* ctx, const lgi_camera* cam, int32_t spp,
+                                                 int32_t bounces, uint32_t flags, int32_t sample0,
+                                                 float* d_sum, float* d_comp, float* d_sumsq);
 /* Primary-hit triangle ids (original scene index, -1 on miss) and t at pixel centres. */
 LGI_API lgi_status lgi_primary_hits(lgi_ctx* ctx, const lgi_camera* cam, int32_t* d_tri, float* d_t);
 /* Device tonemap: u8 = (u8)(sqrt(clamp(sum*inv_spp,0,1))*255). */

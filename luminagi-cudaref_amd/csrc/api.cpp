@@ -56,6 +56,9 @@ lgi_status ensure_gpu(lgi_ctx* ctx) {
     LGI_HIP(ctx, hipEventCreate(&ctx->ev1));
     LGI_HIP(ctx, hipEventCreate(&ctx->ev_tree0));
     LGI_HIP(ctx, hipEventCreate(&ctx->ev_tree1));
+    // this device's static scratch of lgi_render_accumulate_noalloc (resolved once per device)
+    if (lgi_noalloc_scratch(ctx->device, &ctx->noalloc_scratch) != LGI_OK)
+        return lgi_set_err(ctx, LGI_ERR_HIP, "hipGetSymbolAddress of the allocation-free render's scratch failed");
     ctx->gpu_ready = true;
     return LGI_OK;
 }
@@ -511,6 +514,34 @@ lgi_status lgi_bvh_last_treelet_ms(const lgi_ctx* ctx, float* ms) {
 lgi_status lgi_bvh_last_build_ms(const lgi_ctx* ctx, float* ms) {
     if (!ctx || !ms) return LGI_ERR_INVALID_ARG;
     *ms = ctx->last_build_ms;
+    return LGI_OK;
+}
+
+lgi_status lgi_render_accumulate_noalloc(lgi_ctx* ctx, const lgi_camera* cam_in, int32_t spp, int32_t bounces,
+                                         uint32_t flags, int32_t sample0, float* d_sum, float* d_comp,
+                                         float* d_sumsq) {
+    if (!ctx) return LGI_ERR_INVALID_ARG;
+    const lgi_camera& cam = cam_in ? *cam_in : ctx->scene.camera;
+    if (flags & LGI_RENDER_SORTED)
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED,
+                           "allocation-free render: the sorted renderer's queues are allocated per pass");
+    if (ctx->tex_dirty)  // check_render_ready would reload (and allocate) the textures
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED,
+                           "allocation-free render: textures changed since the last render; render once first");
+    lgi_status st = check_render_ready(ctx, cam);
+    if (st != LGI_OK) return st;
+    if (spp < 0 || bounces < 0 || bounces > 64 || sample0 < 0 || !d_sum || !d_comp)
+        return lgi_set_err(ctx, LGI_ERR_INVALID_ARG, "bad render arguments");
+    if (spp == 0) return LGI_OK;
+    int* scratch_dev = ctx->noalloc_scratch;  // resolved when the context set up its device
+    if (!scratch_dev) return lgi_set_err(ctx, LGI_ERR_HIP, "allocation-free render: no scratch address");
+    TraceParams P = make_params(ctx, cam, spp, bounces, flags, sample0);
+    P.sum = d_sum;
+    P.comp = d_comp;
+    P.sumsq = d_sumsq;
+    ctx->kev_used = 0;
+    lgi_launch_render_all_tiles_noalloc(P, scratch_dev, ctx->stream);
+    LGI_HIP(ctx, hipGetLastError());
     return LGI_OK;
 }
 

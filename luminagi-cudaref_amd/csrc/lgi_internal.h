@@ -99,6 +99,7 @@ struct lgi_ctx {
     std::vector<hipEvent_t> kev;
     int kev_used = 0;           // pairs recorded by the last render (0: inline renderer)
     bool kev_overflow = false;  // more launches than pairs: the sum covers the first kev.size()/2
+    int* noalloc_scratch = nullptr;  // device address of this device's lgi_render_accumulate_noalloc scratch
     void* cap_stream = nullptr;  // hipStream_t used for graph capture
     void* graph_exec = nullptr;  // hipGraphExec_t of the last captured batch launch
     std::string graph_key;       // inputs the captured graph was built for

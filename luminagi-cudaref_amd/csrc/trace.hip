@@ -16,7 +16,9 @@
 // (one per Morton bit), so 32 entries never overflow.
 #include "trace_device.h"
 
+#include <algorithm>
 #include <climits>
+#include <mutex>
 
 using namespace lgi;
 
@@ -191,6 +193,9 @@ __global__ __launch_bounds__(kBlock, kRenderWavesPerSimd) void k_render(TracePar
 // items from a device queue (one atomic per item) until the queue is empty, so poses with
 // uneven spp budgets share one launch and no wave idles while items remain. Per-pose
 // accumulators are [pose][W*H*3] (sum, comp) and [pose][W*H] (sumsq).
+// kAllTiles: no item list; item i is every pixel of tile i of the one image P.cam describes,
+// with P.spp samples from P.sample0 (the whole-image render of lgi_render_accumulate_noalloc).
+template <bool kAllTiles>
 __global__ __launch_bounds__(kBlock, kRenderWavesPerSimd) void k_render_queue(TraceParams P, const DevCamera* cams,
                                                          const lgi_work_item* items, int n_items, uint32_t* counter) {
     __shared__ int s_stack[kStackLds * kBlock];
@@ -204,10 +209,10 @@ __global__ __launch_bounds__(kBlock, kRenderWavesPerSimd) void k_render_queue(Tr
         if (lane == 0) idx = atomicAdd(counter, 1u);
         idx = __shfl(idx, 0, 64);
         if (idx >= (uint32_t)n_items) break;
-        const lgi_work_item it = items[idx];
+        const lgi_work_item it = kAllTiles ? lgi_work_item{0, (int32_t)idx, P.spp, P.sample0} : items[idx];
         const int px = (it.tile % tiles_x) * 8 + (lane & 7), py = (it.tile / tiles_x) * 8 + (lane >> 3);
         if (px < W && py < H)
-            render_pixel(P, cams[it.pose], px, py, it.spp, it.sample0, P.sum + it.pose * npx * 3,
+            render_pixel(P, kAllTiles ? P.cam : cams[it.pose], px, py, it.spp, it.sample0, P.sum + it.pose * npx * 3,
                          P.comp + it.pose * npx * 3, P.sumsq ? P.sumsq + it.pose * npx : nullptr, stk);
     }
 }
@@ -280,7 +285,49 @@ void lgi_launch_render(const TraceParams& P, hipStream_t st) {
 void lgi_launch_render_queue(const TraceParams& P, const DevCamera* cams, const lgi_work_item* items, int n_items,
                              uint32_t* counter, int grid, hipStream_t st) {
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL(k_render_queue, dim3(grid), dim3(kBlock), 0, st, P, cams, items, n_items, counter);
+    hipLaunchKernelGGL(k_render_queue<false>, dim3(grid), dim3(kBlock), 0, st, P, cams, items, n_items, counter);
+}
+
+// ------------------------------------------------------------- allocation-free whole-image render
+// Scratch of lgi_render_accumulate_noalloc, one copy per device for the life of the process:
+// the queue counter (the first 64 ints, 256 B) and the traversal-stack spill slots of a
+// persistent grid of kNoAllocGrid blocks (kStackSpill entries per thread beyond the LDS part).
+// Host code never passes `scratch` itself to a kernel: its name is a host-side stand-in, so
+// lgi_noalloc_scratch resolves each device's address with hipGetSymbolAddress once.
+constexpr int kNoAllocGrid = 1024;  // 4 blocks of 256 threads per CU
+constexpr int64_t kNoAllocCounterInts = 64;
+constexpr int64_t kNoAllocScratchInts = kNoAllocCounterInts + (int64_t)kStackSpill * kNoAllocGrid * kBlock;
+static __device__ __align__(256) int scratch[kNoAllocScratchInts];
+static_assert(kNoAllocScratchInts * 4 == LGI_NOALLOC_SCRATCH_BYTES, "documented scratch size");
+
+lgi_status lgi_noalloc_scratch(int device, int** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex m;
+    static int* ptrs[kMaxDevices] = {};
+    if (!out || device < 0 || device >= kMaxDevices) return LGI_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptrs[device]) {
+        int prev = 0;
+        if (hipGetDevice(&prev) != hipSuccess) return LGI_ERR_HIP;
+        if (hipSetDevice(device) != hipSuccess) return LGI_ERR_HIP;
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+        if (e != hipSuccess || !p) return LGI_ERR_HIP;
+        ptrs[device] = (int*)p;
+    }
+    *out = ptrs[device];
+    return LGI_OK;
+}
+
+void lgi_launch_render_all_tiles_noalloc(TraceParams P, int* scratch_dev, hipStream_t st) {
+    const int n_tiles = ((P.cam.width + 7) / 8) * ((P.cam.height + 7) / 8);
+    const int grid = std::max(1, std::min(kNoAllocGrid, n_tiles));
+    uint32_t* counter = (uint32_t*)scratch_dev;
+    P.spill = scratch_dev + kNoAllocCounterInts;
+    P.spill_stride = grid * kBlock;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(k_render_queue<true>, dim3(grid), dim3(kBlock), 0, st, P, nullptr, nullptr, n_tiles, counter);
 }
 void lgi_launch_primary(const TraceParams& P, int32_t* tri, float* t, hipStream_t st) {
     dim3 grid((P.cam.width + 15) / 16, (P.cam.height + 15) / 16);

@@ -47,6 +47,11 @@ void lgi_launch_tonemap(const float* sum, int64_t n, float inv_spp, uint8_t* out
 // Persistent queue renderer over (pose, 8x8 tile, spp, sample0) items; resets the counter first.
 void lgi_launch_render_queue(const TraceParams& P, const DevCamera* cams, const lgi_work_item* items, int n_items,
                              uint32_t* counter, int grid, hipStream_t st);
+// Allocation-free whole-image render (lgi_render_accumulate_noalloc): the device address of this
+// device's static scratch array (resolved once per device), and the persistent queue launch over
+// every 8x8 tile of P.cam's image that uses it for its counter and stack spill slots.
+lgi_status lgi_noalloc_scratch(int device, int** out);
+void lgi_launch_render_all_tiles_noalloc(TraceParams P, int* scratch_dev, hipStream_t st);
 // Sorted wavefront renderer (wavefront.hip); bb = scene box used by the ray sort key.
 // d_tiles (device, n_tiles distinct row-major 8x8 tile ids) restricts the render to those tiles.
 lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_min, float3 bb_max,

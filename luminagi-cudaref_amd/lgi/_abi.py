@@ -92,6 +92,8 @@ SIGNATURES = {
     "lgi_render": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, _P, _P, _P, _P, _P]),
     "lgi_render_accumulate": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32, C.c_int32,
                                          _P, _P, _P]),
+    "lgi_render_accumulate_noalloc": (C.c_int, [_P, C.POINTER(Camera), C.c_int32, C.c_int32, C.c_uint32,
+                                                 C.c_int32, _P, _P, _P]),
     "lgi_primary_hits": (C.c_int, [_P, C.POINTER(Camera), _P, _P]),
     "lgi_tonemap": (C.c_int, [_P, _P, C.c_int32, C.c_float, _P]),
     "lgi_gbuffer": (C.c_int, [_P, C.POINTER(Camera), _P, _P, _P]),

@@ -212,6 +212,13 @@ class Context:
         self._c(lib().lgi_render_accumulate(self._h, C.byref(cam), spp, bounces, flags, sample0, _dp(d_sum),
                                             _dp(d_comp), _dp(d_sumsq)))
 
+    def accumulate_noalloc(self, d_sum, d_comp, spp, bounces, flags=0, sample0=0, d_sumsq=None, camera=None):
+        """accumulate() without any device allocation (lgi_render_accumulate_noalloc): same sums for
+        the non-sorted renderers; for callers that cannot allocate (graph capture, serving loops)."""
+        cam = camera if camera is not None else self.camera()
+        self._c(lib().lgi_render_accumulate_noalloc(self._h, C.byref(cam), spp, bounces, flags, sample0, _dp(d_sum),
+                                                    _dp(d_comp), _dp(d_sumsq)))
+
     def render_batch(self, jobs, d_sum, d_comp, bounces, d_sumsq=None, flags=0, use_graph=False):
         """jobs: list of (camera, spp, sample0); buffers [n][W*H*3] (sum, comp), [n][W*H] (sumsq)."""
         arr = (_abi.PoseJob * len(jobs))()
