@@ -10,7 +10,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <filesystem>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -37,41 +39,137 @@ bool write_gbuffer(const std::string& png, int W, int H, const std::vector<uint8
     return ok;
 }
 
+// Command line. The flag names, their values, the positional rule (first non-flag word = scene,
+// any later one = output PNG) and the defaults are the reference CLI's interface; a flag whose
+// values are missing at the end of argv is ignored, as an unknown flag is.
+struct Options {
+    std::string scene;
+    std::string png = "output/reference.png";
+    std::string asset_root, batch, hdr;
+    int spp = 64, bounces = 2;
+    uint32_t rng = LGI_RNG_PCG32;
+    bool sort = false, no_ray_sort = false, gbuffer = false;
+    bool has_pos = false, has_tgt = false;
+    float pos[3] = {0, 0, 0}, tgt[3] = {0, 0, 0};
+
+    uint32_t render_flags() const {
+        return rng | (sort ? LGI_RENDER_SORTED : 0u) | (sort && no_ray_sort ? LGI_RENDER_NO_RAY_SORT : 0u);
+    }
+};
+
+bool parse_options(int argc, char** argv, Options& o) {
+    using Values = const char* const*;
+    auto vec3 = [](Values v, float out[3]) {
+        for (int k = 0; k < 3; ++k) out[k] = (float)std::atof(v[k]);
+    };
+    struct Flag {
+        const char* name;
+        int values;
+        std::function<void(Values)> apply;
+    };
+    const Flag flags[] = {
+        {"--spp", 1, [&](Values v) { o.spp = std::atoi(v[0]); }},
+        {"--bounces", 1, [&](Values v) { o.bounces = std::atoi(v[0]); }},
+        {"--asset-root", 1, [&](Values v) { o.asset_root = v[0]; }},
+        {"-o", 1, [&](Values v) { o.png = v[0]; }},
+        {"--batch", 1, [&](Values v) { o.batch = v[0]; }},
+        {"--hdr", 1, [&](Values v) { o.hdr = v[0]; }},
+        {"--rng", 1, [&](Values v) { o.rng = std::strcmp(v[0], "xorshift") == 0 ? LGI_RNG_XORSHIFT_REF : LGI_RNG_PCG32; }},
+        {"--sort", 0, [&](Values) { o.sort = true; }},
+        {"--no-ray-sort", 0, [&](Values) { o.no_ray_sort = true; }},
+        {"--gbuffer", 0, [&](Values) { o.gbuffer = true; }},
+        {"--cam-pos", 3, [&](Values v) { vec3(v, o.pos); o.has_pos = true; }},
+        {"--cam-target", 3, [&](Values v) { vec3(v, o.tgt); o.has_tgt = true; }},
+    };
+    int positional = 0;
+    for (int i = 1; i < argc; ++i) {
+        const Flag* f = nullptr;
+        for (const Flag& c : flags)
+            if (std::strcmp(argv[i], c.name) == 0) f = &c;
+        if (f) {
+            if (i + f->values < argc) {
+                f->apply(argv + i + 1);
+                i += f->values;
+            }
+            continue;
+        }
+        if (argv[i][0] == '-') continue;
+        (positional++ == 0 ? o.scene : o.png) = argv[i];
+    }
+    return !o.scene.empty();
+}
+
+// One render per batch line (read_batch_file format), the scene and BVH loaded once; batch jobs
+// use the inline renderer, as the reference's batch loop does (src/main.cpp:170).
+int run_batch(lgi_ctx* ctx, const Options& o, const lgi_camera& cam, double bvh_ms, long long ntris) {
+    int32_t n = 0;
+    lgi_batch_parse(o.batch.c_str(), nullptr, 0, &n);
+    std::vector<lgi_batch_job> jobs(n);
+    if (n) lgi_batch_parse(o.batch.c_str(), jobs.data(), n, &n);
+    std::printf("batch: %zu jobs  (scene+bvh loaded once in %.1f ms)\n", jobs.size(), bvh_ms);
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const lgi_batch_job& job = jobs[k];
+        lgi_camera view = cam;
+        lgi_camera_look_at(&view, job.cam_pos, job.cam_tgt);
+        const size_t bytes = (size_t)view.width * view.height * 3;
+        std::vector<uint8_t> rgb(bytes), alb(o.gbuffer ? bytes : 0), nrm(alb.size()), wp(alb.size());
+        const auto start = std::chrono::steady_clock::now();
+        if (lgi_render(ctx, &view, job.spp, job.bounces, o.rng, rgb.data(), nullptr, o.gbuffer ? alb.data() : nullptr,
+                       o.gbuffer ? nrm.data() : nullptr, o.gbuffer ? wp.data() : nullptr) != LGI_OK)
+            return fail_gpu(ctx, "render");
+        float kernel_ms = 0;
+        lgi_last_render_ms(ctx, &kernel_ms);
+        std::printf("kernel: %.1f ms  (%d x %d, %d spp, %d bounces, %lld tris)\n", kernel_ms, view.width, view.height,
+                    job.spp, job.bounces, ntris);
+        const double wall_ms = ms_since(start);
+        const std::filesystem::path out(job.out_png);
+        if (out.has_parent_path()) std::filesystem::create_directories(out.parent_path());
+        lgi_write_png(job.out_png, rgb.data(), view.width, view.height);
+        if (o.gbuffer) write_gbuffer(job.out_png, view.width, view.height, alb, nrm, wp);
+        std::printf("[%zu/%zu] %s  spp=%d  render=%.1f ms\n", k + 1, jobs.size(), job.out_png, job.spp, wall_ms);
+    }
+    return 0;
+}
+
+// The scene camera's image to o.png (+ G-buffer PNGs, + raw FP32 mean radiance with --hdr).
+int run_single(lgi_ctx* ctx, const Options& o, const lgi_camera& cam, long long ntris) {
+    const int W = cam.width, H = cam.height;
+    const size_t bytes = (size_t)W * H * 3;
+    const bool gbuffer = o.gbuffer && !o.sort;  // the sorted path has no G-buffer (src/main.cpp:19-27)
+    std::vector<uint8_t> rgb(bytes), alb(gbuffer ? bytes : 0), nrm(alb.size()), wp(alb.size());
+    std::vector<float> hdr(o.hdr.empty() ? 0 : bytes);
+    const auto start = std::chrono::steady_clock::now();
+    if (lgi_render(ctx, nullptr, o.spp, o.bounces, o.render_flags(), rgb.data(), hdr.empty() ? nullptr : hdr.data(),
+                   gbuffer ? alb.data() : nullptr, gbuffer ? nrm.data() : nullptr, gbuffer ? wp.data() : nullptr) !=
+        LGI_OK)
+        return fail_gpu(ctx, "render");
+    float kernel_ms = 0;
+    lgi_last_render_ms(ctx, &kernel_ms);
+    std::printf("kernel%s: %.1f ms  (%d x %d, %d spp, %d bounces, %lld tris)\n", o.sort ? " (sorted)" : "", kernel_ms, W,
+                H, o.spp, o.bounces, ntris);
+    std::printf("render: %.1f ms (%d x %d)\n", ms_since(start), W, H);
+    const std::filesystem::path out(o.png);
+    if (out.has_parent_path()) std::filesystem::create_directories(out.parent_path());
+    if (lgi_write_png(o.png.c_str(), rgb.data(), W, H) != LGI_OK) {
+        std::fprintf(stderr, "failed to write %s\n", o.png.c_str());
+        return 3;
+    }
+    std::printf("wrote %s\n", o.png.c_str());
+    if (gbuffer) write_gbuffer(o.png, W, H, alb, nrm, wp);
+    if (!hdr.empty()) {
+        if (FILE* f = std::fopen(o.hdr.c_str(), "wb")) {
+            std::fwrite(hdr.data(), 4, hdr.size(), f);
+            std::fclose(f);
+        }
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-    const char* jsonPath = nullptr;
-    const char* pngPath = "output/reference.png";
-    std::string assetRoot, batchFile, hdrPath;
-    int spp = 64, bounces = 2;
-    bool useSort = false, noRaySort = false, gbuffer = false, haveCamPos = false, haveCamTgt = false;
-    uint32_t rngFlag = LGI_RNG_PCG32;
-    float camPos[3] = {0, 0, 0}, camTgt[3] = {0, 0, 0};
-    auto read_vec3 = [&](int& i, float out[3]) -> bool {
-        if (i + 3 >= argc) return false;
-        out[0] = (float)std::atof(argv[++i]);
-        out[1] = (float)std::atof(argv[++i]);
-        out[2] = (float)std::atof(argv[++i]);
-        return true;
-    };
-    for (int i = 1; i < argc; ++i) {
-        std::string a = argv[i];
-        if (a == "--spp" && i + 1 < argc) spp = std::atoi(argv[++i]);
-        else if (a == "--bounces" && i + 1 < argc) bounces = std::atoi(argv[++i]);
-        else if (a == "--asset-root" && i + 1 < argc) assetRoot = argv[++i];
-        else if (a == "-o" && i + 1 < argc) pngPath = argv[++i];
-        else if (a == "--sort") useSort = true;
-        else if (a == "--no-ray-sort") noRaySort = true;
-        else if (a == "--gbuffer") gbuffer = true;
-        else if (a == "--cam-pos") haveCamPos = read_vec3(i, camPos);
-        else if (a == "--cam-target") haveCamTgt = read_vec3(i, camTgt);
-        else if (a == "--batch" && i + 1 < argc) batchFile = argv[++i];
-        else if (a == "--hdr" && i + 1 < argc) hdrPath = argv[++i];
-        else if (a == "--rng" && i + 1 < argc) rngFlag = std::string(argv[++i]) == "xorshift" ? LGI_RNG_XORSHIFT_REF : LGI_RNG_PCG32;
-        else if (a[0] != '-' && !jsonPath) jsonPath = argv[i];
-        else if (a[0] != '-') pngPath = argv[i];
-    }
-    if (!jsonPath) {
+    Options o;
+    if (!parse_options(argc, argv, o)) {
         std::fprintf(stderr, "usage: %s <scene.json> [output.png] [--spp N] [--bounces N] [--asset-root DIR]\n",
                      argv[0]);
         return 1;
@@ -82,95 +180,34 @@ int main(int argc, char** argv) {
         return 4;
     }
     // a .lgis path is the binary scene file (lgi_scene_save_bin), anything else scene.json
-    const std::string sp(jsonPath);
-    const bool bin = sp.size() > 5 && sp.compare(sp.size() - 5, 5, ".lgis") == 0;
-    if ((bin ? lgi_scene_load_bin(ctx, jsonPath, 1) : lgi_scene_load_json(ctx, jsonPath, 1)) != LGI_OK) {
+    const bool binary = std::filesystem::path(o.scene).extension() == ".lgis";
+    if ((binary ? lgi_scene_load_bin(ctx, o.scene.c_str(), 1) : lgi_scene_load_json(ctx, o.scene.c_str(), 1)) !=
+        LGI_OK) {
         lgi_destroy(ctx);
         return 2;
     }
-    if (!assetRoot.empty()) lgi_set_asset_root(ctx, assetRoot.c_str());
+    if (!o.asset_root.empty()) lgi_set_asset_root(ctx, o.asset_root.c_str());
     lgi_camera cam;
     lgi_scene_get(ctx, &cam, nullptr, nullptr, nullptr, nullptr);
-    if (haveCamPos && haveCamTgt) {
-        lgi_camera_look_at(&cam, camPos, camTgt);
+    if (o.has_pos && o.has_tgt) {
+        lgi_camera_look_at(&cam, o.pos, o.tgt);
         lgi_scene_set_camera(ctx, &cam);
-        std::printf("camera override: pos (%.2f,%.2f,%.2f) -> target (%.2f,%.2f,%.2f)\n", camPos[0], camPos[1],
-                    camPos[2], camTgt[0], camTgt[1], camTgt[2]);
+        std::printf("camera override: pos (%.2f,%.2f,%.2f) -> target (%.2f,%.2f,%.2f)\n", o.pos[0], o.pos[1], o.pos[2],
+                    o.tgt[0], o.tgt[1], o.tgt[2]);
     }
-    const uint32_t flags =
-        rngFlag | (useSort ? LGI_RENDER_SORTED : 0u) | (useSort && noRaySort ? LGI_RENDER_NO_RAY_SORT : 0u);
-    auto t0 = std::chrono::steady_clock::now();
+    const auto start = std::chrono::steady_clock::now();
     if (lgi_bvh_build(ctx) != LGI_OK) return fail_gpu(ctx, "bvh build");
-    int64_t nn = 0;
-    lgi_bvh_node_count(ctx, &nn);
-    const double bvh_ms = ms_since(t0);
-    int64_t ntris = 0;
+    const double bvh_ms = ms_since(start);
+    int64_t nodes = 0, ntris = 0;
+    lgi_bvh_node_count(ctx, &nodes);
     lgi_scene_counts(ctx, &ntris, nullptr, nullptr, nullptr);
-
-    if (!batchFile.empty()) {
-        int32_t n = 0;
-        lgi_batch_parse(batchFile.c_str(), nullptr, 0, &n);
-        std::vector<lgi_batch_job> jobs(n);
-        if (n) lgi_batch_parse(batchFile.c_str(), jobs.data(), n, &n);
-        std::printf("batch: %zu jobs  (scene+bvh loaded once in %.1f ms)\n", jobs.size(), bvh_ms);
-        for (size_t k = 0; k < jobs.size(); ++k) {
-            const lgi_batch_job& j = jobs[k];
-            lgi_camera jc = cam;
-            lgi_camera_look_at(&jc, j.cam_pos, j.cam_tgt);
-            const int W = jc.width, H = jc.height;
-            std::vector<uint8_t> rgb((size_t)W * H * 3), alb, nrm, wp;
-            if (gbuffer) { alb.resize(rgb.size()); nrm.resize(rgb.size()); wp.resize(rgb.size()); }
-            auto tj = std::chrono::steady_clock::now();
-            // batch mode renders with the inline path, as the reference (src/main.cpp:170)
-            if (lgi_render(ctx, &jc, j.spp, j.bounces, rngFlag, rgb.data(), nullptr, gbuffer ? alb.data() : nullptr,
-                           gbuffer ? nrm.data() : nullptr, gbuffer ? wp.data() : nullptr) != LGI_OK)
-                return fail_gpu(ctx, "render");
-            float kms = 0;
-            lgi_last_render_ms(ctx, &kms);
-            std::printf("kernel: %.1f ms  (%d x %d, %d spp, %d bounces, %lld tris)\n", kms, W, H, j.spp, j.bounces,
-                        (long long)ntris);
-            const double rms = ms_since(tj);
-            std::filesystem::path op(j.out_png);
-            if (op.has_parent_path()) std::filesystem::create_directories(op.parent_path());
-            lgi_write_png(j.out_png, rgb.data(), W, H);
-            if (gbuffer) write_gbuffer(j.out_png, W, H, alb, nrm, wp);
-            std::printf("[%zu/%zu] %s  spp=%d  render=%.1f ms\n", k + 1, jobs.size(), j.out_png, j.spp, rms);
-        }
-        lgi_destroy(ctx);
-        return 0;
-    }
-    std::printf("bvh: %.1f ms (%lld nodes)\n", bvh_ms, (long long)nn);
-    const int W = cam.width, H = cam.height;
-    std::vector<uint8_t> rgb((size_t)W * H * 3), alb, nrm, wp;
-    std::vector<float> hdr;
-    if (!hdrPath.empty()) hdr.resize(rgb.size());
-    const bool gb = gbuffer && !useSort;
-    if (gb) { alb.resize(rgb.size()); nrm.resize(rgb.size()); wp.resize(rgb.size()); }
-    auto t2 = std::chrono::steady_clock::now();
-    if (lgi_render(ctx, nullptr, spp, bounces, flags, rgb.data(), hdr.empty() ? nullptr : hdr.data(),
-                   gb ? alb.data() : nullptr, gb ? nrm.data() : nullptr, gb ? wp.data() : nullptr) != LGI_OK)
-        return fail_gpu(ctx, "render");
-    float kms = 0;
-    lgi_last_render_ms(ctx, &kms);
-    std::printf("kernel%s: %.1f ms  (%d x %d, %d spp, %d bounces, %lld tris)\n", useSort ? " (sorted)" : "", kms, W, H,
-                spp, bounces, (long long)ntris);
-    std::printf("render: %.1f ms (%d x %d)\n", ms_since(t2), W, H);
-    std::filesystem::path pp(pngPath);
-    if (pp.has_parent_path()) std::filesystem::create_directories(pp.parent_path());
-    if (lgi_write_png(pngPath, rgb.data(), W, H) != LGI_OK) {
-        std::fprintf(stderr, "failed to write %s\n", pngPath);
-        lgi_destroy(ctx);
-        return 3;
-    }
-    std::printf("wrote %s\n", pngPath);
-    if (gb) write_gbuffer(pngPath, W, H, alb, nrm, wp);
-    if (!hdr.empty()) {
-        FILE* f = std::fopen(hdrPath.c_str(), "wb");
-        if (f) {
-            std::fwrite(hdr.data(), 4, hdr.size(), f);
-            std::fclose(f);
-        }
+    int rc;
+    if (!o.batch.empty()) {
+        rc = run_batch(ctx, o, cam, bvh_ms, (long long)ntris);
+    } else {
+        std::printf("bvh: %.1f ms (%lld nodes)\n", bvh_ms, (long long)nodes);
+        rc = run_single(ctx, o, cam, (long long)ntris);
     }
     lgi_destroy(ctx);
-    return 0;
+    return rc;
 }

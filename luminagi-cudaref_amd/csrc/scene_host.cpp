@@ -897,6 +897,9 @@ lgi_status read_png_rgba(const char* path, int& w, int& h, std::vector<uint8_t>&
     if (!(depth == 8 || (depth == 16 && ctype != 3))) return bad("unsupported bit depth");
     if (ctype == 3 && pal.size() < 3) return bad("palette image without PLTE");
     const size_t bpp = (size_t)ch * depth / 8, stride = bpp * (size_t)w;
+    // deflate expands at most ~1032:1, so a header promising far more pixels than the compressed
+    // data can hold is corrupt; refusing it here avoids allocating for it
+    if (((size_t)stride + 1) * h > idat.size() * 1032 + 1024) return bad("image data too short for its size");
     std::vector<uint8_t> raw(((size_t)stride + 1) * h);
     uLongf rlen = raw.size();
     if (uncompress(raw.data(), &rlen, idat.data(), idat.size()) != Z_OK || rlen != raw.size())
@@ -992,6 +995,11 @@ struct FileCloser {
     }
 };
 
+// fwrite/fread of n records; n = 0 passes (an empty vector's data() may be null, which the C
+// library does not accept even for zero records).
+bool put_records(FILE* f, const void* p, size_t size, size_t n) { return n == 0 || std::fwrite(p, size, n, f) == n; }
+bool get_records(FILE* f, void* p, size_t size, size_t n) { return n == 0 || std::fread(p, size, n, f) == n; }
+
 }  // namespace
 
 lgi_status scene_save_bin(const char* path, const HostScene& s, std::string& err) {
@@ -1015,14 +1023,14 @@ lgi_status scene_save_bin(const char* path, const HostScene& s, std::string& err
     bool ok = std::fwrite(&h, sizeof h, 1, fc.f) == 1;
     ok = ok && std::fwrite(&s.camera, sizeof(lgi_camera), 1, fc.f) == 1;
     ok = ok && std::fwrite(&s.sun, sizeof(lgi_dir_light), 1, fc.f) == 1;
-    ok = ok && std::fwrite(s.dirs.data(), sizeof(lgi_dir_light), s.dirs.size(), fc.f) == s.dirs.size();
-    ok = ok && std::fwrite(s.points.data(), sizeof(lgi_point_light), s.points.size(), fc.f) == s.points.size();
+    ok = ok && put_records(fc.f, s.dirs.data(), sizeof(lgi_dir_light), s.dirs.size());
+    ok = ok && put_records(fc.f, s.points.data(), sizeof(lgi_point_light), s.points.size());
     for (const HostMaterial& m : s.materials) {
         const uint32_t len = (uint32_t)m.diffuse_tex.size();
         ok = ok && std::fwrite(m.albedo, 4, 3, fc.f) == 3 && std::fwrite(m.emissive, 4, 3, fc.f) == 3;
-        ok = ok && std::fwrite(&len, 4, 1, fc.f) == 1 && std::fwrite(m.diffuse_tex.data(), 1, len, fc.f) == len;
+        ok = ok && std::fwrite(&len, 4, 1, fc.f) == 1 && put_records(fc.f, m.diffuse_tex.data(), 1, len);
     }
-    ok = ok && std::fwrite(s.tris.data(), sizeof(lgi_triangle), s.tris.size(), fc.f) == s.tris.size();
+    ok = ok && put_records(fc.f, s.tris.data(), sizeof(lgi_triangle), s.tris.size());
     ok = std::fclose(fc.f) == 0 && ok;
     fc.f = nullptr;
     if (!ok) {
@@ -1064,18 +1072,18 @@ lgi_status scene_load_bin(const char* path, HostScene& out, std::string& err, bo
     s.tris.resize(h.n_tris);
     bool ok = std::fread(&s.camera, sizeof(lgi_camera), 1, fc.f) == 1;
     ok = ok && std::fread(&s.sun, sizeof(lgi_dir_light), 1, fc.f) == 1;
-    ok = ok && std::fread(s.dirs.data(), sizeof(lgi_dir_light), h.n_dirs, fc.f) == h.n_dirs;
-    ok = ok && std::fread(s.points.data(), sizeof(lgi_point_light), h.n_points, fc.f) == h.n_points;
+    ok = ok && get_records(fc.f, s.dirs.data(), sizeof(lgi_dir_light), h.n_dirs);
+    ok = ok && get_records(fc.f, s.points.data(), sizeof(lgi_point_light), h.n_points);
     for (HostMaterial& m : s.materials) {
         uint32_t len = 0;
         ok = ok && std::fread(m.albedo, 4, 3, fc.f) == 3 && std::fread(m.emissive, 4, 3, fc.f) == 3;
         ok = ok && std::fread(&len, 4, 1, fc.f) == 1 && len <= (uint64_t)fsize;
         if (ok) {
             m.diffuse_tex.resize(len);
-            ok = std::fread(&m.diffuse_tex[0], 1, len, fc.f) == len;
+            ok = get_records(fc.f, &m.diffuse_tex[0], 1, len);
         }
     }
-    ok = ok && std::fread(s.tris.data(), sizeof(lgi_triangle), h.n_tris, fc.f) == h.n_tris;
+    ok = ok && get_records(fc.f, s.tris.data(), sizeof(lgi_triangle), h.n_tris);
     if (!ok || std::ftell(fc.f) != fsize) return bad("truncated or inconsistent");
     out = std::move(s);
     if (verbose) {

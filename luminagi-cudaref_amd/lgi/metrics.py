@@ -125,3 +125,39 @@ def html_report(engine, ref, result, engine_path, ref_path, gain=4.0, title="Eng
                           engine=html.escape(pathlib.Path(engine_path).resolve().as_posix()),
                           ref=html.escape(pathlib.Path(ref_path).resolve().as_posix()),
                           img_engine=b64(engine), img_ref=b64(ref), img_heat=b64(result["heat"]))
+
+
+def read_rgb(path, size=None):
+    """HxWx3 uint8 of an image file; with size=(w, h) and another size on disk, resampled to it
+    with PIL's bilinear filter (the reference tool resizes the engine image this way)."""
+    from PIL import Image
+    with Image.open(path) as im:
+        im = im.convert("RGB")
+        if size is not None and im.size != tuple(size):
+            im = im.resize(tuple(size), Image.BILINEAR)
+        return np.array(im, dtype=np.uint8)
+
+
+def report_line(r):
+    """The diff tool's one-line summary (tools/diff.py's report format)."""
+    return f"PSNR {r['psnr']:.2f} dB | SSIM {r['ssim']:.4f} | MAD {r['mad']:.2f}"
+
+
+def diff_files(engine_path, ref_path, out_html, gain=4.0, warn=None):
+    """Engine screenshot vs reference image on the GPU: metrics and heatmap of the two files (the
+    engine image resampled to the reference's size when they differ), written as out_html and
+    its <stem>.heatmap.png sidecar. Returns compare()'s dict plus the two images."""
+    from .tracer import write_png
+    ref = read_rgb(ref_path)
+    engine = read_rgb(engine_path)
+    if engine.shape != ref.shape:
+        if warn:
+            warn(f"resizing engine {engine.shape[1::-1]} -> {ref.shape[1::-1]}")
+        engine = read_rgb(engine_path, size=ref.shape[1::-1])
+    r = compare(engine, ref, gain=gain)
+    out = pathlib.Path(out_html)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    write_png(out.with_suffix(".heatmap.png"), r["heat"])
+    out.write_text(html_report(engine, ref, r, engine_path, ref_path, gain=gain), encoding="utf-8")
+    r["engine"], r["ref"] = engine, ref
+    return r

@@ -1,34 +1,52 @@
 """validate.py <scene.json> [--open] [--out-dir DIR]
 
-Drop-in for the reference's tools/validate.py: renders the scene with bin/cuda_ref into
-<out-dir>/ref_<stem>.png, then runs diff.py on the engine screenshot saved next to the scene
-(<scene>.png) against it, writing <out-dir>/report_<stem>.html. The output directory defaults
-to output/ under the package root. --open shows the report in the default browser.
+Checks an engine capture against the path tracer in one process. A capture is a scene file plus
+the engine's screenshot saved beside it under the same stem (<scene>.png). The scene is loaded
+and rendered through liblgi with the cuda_ref defaults (64 spp, 2 bounces), the render is written
+to <out-dir>/ref_<stem>.png, and the screenshot is compared with it on the GPU, giving
+<out-dir>/report_<stem>.html and its heatmap. The arguments and the output file names are those
+of the reference's tools/validate.py; --out-dir defaults to output/ under the package root.
 """
 import argparse
+import contextlib
 import pathlib
-import subprocess
 import sys
 import time
 
 PKG = pathlib.Path(__file__).resolve().parent.parent
-CUDA_REF = PKG / "bin" / "cuda_ref"
-DIFF_PY = PKG / "tools" / "diff.py"
+sys.path.insert(0, str(PKG))
+
+REF_SPP, REF_BOUNCES = 64, 2  # cuda_ref's defaults
 
 
-def run(cmd, label, cwd):
-    print(f"--- {label} ---")
-    print("  " + " ".join(f'"{c}"' if " " in str(c) else str(c) for c in cmd))
+@contextlib.contextmanager
+def stage(name):
     t0 = time.perf_counter()
-    r = subprocess.run([str(c) for c in cmd], cwd=cwd, capture_output=True, text=True)
-    dt = time.perf_counter() - t0
-    if r.stdout:
-        print(r.stdout.rstrip())
-    if r.stderr:
-        print(r.stderr.rstrip(), file=sys.stderr)
-    if r.returncode != 0:
-        sys.exit(f"{label} failed (exit {r.returncode})")
-    print(f"  ({dt:.1f}s)")
+    yield
+    print(f"[validate] {name}: {time.perf_counter() - t0:.2f} s")
+
+
+def capture_paths(scene_json, out_dir):
+    """(scene, screenshot, reference PNG, report) paths of one capture; raises FileNotFoundError
+    naming every input that is not there."""
+    scene = pathlib.Path(scene_json).resolve()
+    shot = scene.with_suffix(".png")
+    absent = [str(p) for p in (scene, shot) if not p.is_file()]
+    if absent:
+        raise FileNotFoundError("capture incomplete, not found: " + ", ".join(absent))
+    out = pathlib.Path(out_dir).resolve()
+    return scene, shot, out / f"ref_{scene.stem}.png", out / f"report_{scene.stem}.html"
+
+
+def render_reference(scene, out_png, spp=REF_SPP, bounces=REF_BOUNCES):
+    import lgi
+    with lgi.Context() as ctx:
+        ctx.load_json(scene)
+        ctx.build_bvh()
+        img = ctx.render(spp, bounces)["rgb"]
+    out_png.parent.mkdir(parents=True, exist_ok=True)
+    lgi.write_png(out_png, img)
+    return img
 
 
 def main(argv=None):
@@ -37,26 +55,18 @@ def main(argv=None):
     ap.add_argument("--open", action="store_true", help="open the HTML report after running")
     ap.add_argument("--out-dir", default=str(PKG / "output"))
     args = ap.parse_args(argv)
+    try:
+        scene, shot, ref_png, report = capture_paths(args.scene_json, args.out_dir)
+    except FileNotFoundError as e:
+        sys.exit(f"validate: {e}")
 
-    scene = pathlib.Path(args.scene_json).resolve()
-    if not scene.is_file():
-        sys.exit(f"no such file: {scene}")
-    engine_png = scene.with_suffix(".png")
-    if not engine_png.is_file():
-        sys.exit(f"engine screenshot missing: {engine_png}")
-    if not CUDA_REF.is_file():
-        sys.exit(f"{CUDA_REF} is not built (make -C {PKG})")
+    from lgi import metrics
 
-    out_dir = pathlib.Path(args.out_dir).resolve()
-    out_dir.mkdir(parents=True, exist_ok=True)
-    stem = scene.stem
-    ref_png = out_dir / f"ref_{stem}.png"
-    report = out_dir / f"report_{stem}.html"
-
-    run([CUDA_REF, scene, ref_png], "path-traced reference render", PKG)
-    run([sys.executable, DIFF_PY, engine_png, ref_png, "--out", report], "image diff", PKG)
-
-    print()
+    with stage(f"reference render ({REF_SPP} spp, {REF_BOUNCES} bounces) -> {ref_png.name}"):
+        render_reference(scene, ref_png)
+    with stage(f"comparison -> {report.name}"):
+        r = metrics.diff_files(shot, ref_png, report, warn=lambda msg: print(msg, file=sys.stderr))
+    print(metrics.report_line(r))
     print(f"report: {report}")
     if args.open:
         import webbrowser

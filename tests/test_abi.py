@@ -149,14 +149,15 @@ def test_cli_usage_and_load_failure(tmp_path):
 
 
 def test_validate_tool_argument_errors(tmp_path):
-    """validate.py stops with the reference's messages before any GPU work."""
+    """validate.py names every missing input of a capture and stops before any GPU work."""
     import sys
     tool = PKG / "tools" / "validate.py"
     r = subprocess.run([sys.executable, str(tool), str(tmp_path / "none.json")], capture_output=True, text=True)
-    assert r.returncode == 1 and "no such file:" in r.stderr
+    assert r.returncode == 1 and "not found" in r.stderr
+    assert "none.json" in r.stderr and "none.png" in r.stderr
     (tmp_path / "s.json").write_text("{}")
     r = subprocess.run([sys.executable, str(tool), str(tmp_path / "s.json")], capture_output=True, text=True)
-    assert r.returncode == 1 and "engine screenshot missing:" in r.stderr
+    assert r.returncode == 1 and "s.png" in r.stderr and "s.json" not in r.stderr
 
 
 def test_probe_rejects_bad_arguments():

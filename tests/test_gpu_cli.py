@@ -160,8 +160,9 @@ def test_tools_diff_resizes_engine(gpu, tmp_path, capsys):
 
 
 def test_tools_validate_flow(gpu, tmp_path, capsys):
-    """validate.py <scene.json>: cuda_ref renders the reference next to the engine screenshot
-    (<scene>.png), diff.py compares them; the report's numbers are the oracle's on those files."""
+    """validate.py <scene.json>: the reference render (cuda_ref defaults) is written next to the
+    report and compared with the engine screenshot (<scene>.png); the report's numbers are the
+    oracle's on those files."""
     import lgi
     ctx, scene = small_scene(tmp_path)
     ctx.build_bvh()
@@ -170,7 +171,8 @@ def test_tools_validate_flow(gpu, tmp_path, capsys):
     out = capsys.readouterr().out
     ref_png = tmp_path / "out" / "ref_scene.png"
     assert report == tmp_path / "out" / "report_scene.html" and report.is_file()
-    assert "--- path-traced reference render ---" in out and "--- image diff ---" in out
+    assert "[validate] reference render (64 spp, 2 bounces)" in out and "[validate] comparison" in out
+    assert (tmp_path / "out" / "report_scene.heatmap.png").is_file()
     assert fmt_line(read_png(scene.with_suffix(".png")), read_png(ref_png)) in out
     ref = lgi.Context()
     ref.load_json(scene)
