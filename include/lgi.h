@@ -328,7 +328,9 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
  * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
  * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, kind, relu, variant} with kind
  * 0 = forward, 1 = ReLU-masked input (data gradient), 2 = forward with a pooled copy,
- * us[8*i..] = microseconds per launch of each of the 8 variants (0 = does not fit); *n = entries. */
+ * us[LGI_UNET_CONV_VARIANTS*i..] = microseconds per launch of each variant (0 = does not fit);
+ * *n = entries. */
+#define LGI_UNET_CONV_VARIANTS 11
 LGI_API lgi_status lgi_unet_conv_tuning(int32_t mode);
 LGI_API lgi_status lgi_unet_conv_force_variant(int32_t variant);
 LGI_API lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32_t* n);

@@ -207,6 +207,66 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
 // in lane n + 32*((m/4)%2), element 4*(m/8) + m%4.
 constexpr int kTW32 = 32, kHW32 = kTW32 + 2;
 
+// Epilogue of the 32x32-MFMA convolutions: bias, ReLU, BF16 output tile of (4*RW rows x 32
+// columns) x TN channels through LDS, optional 2x2 max-pool of it. Element j of lane l is pixel
+// column 8*(j/4) + 4*(l/32) + j%4, channel l%32; the tile is written to s_out as [row][column]
+// [channel] and then stored 8 channels (16 bytes) per thread. The caller's last barrier must
+// have freed s_out (TH * 32 * TN BF16).
+template <int TN, int RW>
+__device__ __forceinline__ void conv32_epilogue(const ConvArgs& a, const f32x16 (&acc)[RW][TN / 32], bf16* s_out,
+                                                int n, int y0, int x0, int co0) {
+    constexpr int NG = TN / 32, TH = RW * (kBlock / 64);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, kg = lane >> 5;
+    for (int mi = 0; mi < RW; ++mi)
+        for (int ng = 0; ng < NG; ++ng) {
+            const int c = ng * 32 + l32;
+            const float b = a.bias ? a.bias[co0 + c] : 0.0f;
+            for (int j = 0; j < 16; ++j) {
+                float v = acc[mi][ng][j] + b;
+                if (a.relu) v = fmaxf(v, 0.0f);
+                s_out[((wave * RW + mi) * kTW32 + 8 * (j >> 2) + 4 * kg + (j & 3)) * TN + c] = (bf16)v;
+            }
+        }
+    __syncthreads();
+    constexpr int kVecs = TH * kTW32 * TN / 8;
+    for (int t = tid; t < kVecs; t += kBlock) {
+        const int px = t / (TN / 8), cg = t % (TN / 8);
+        const int y = y0 + px / kTW32, x = x0 + px % kTW32;
+        if (y >= a.H || x >= a.W) continue;
+        const int co = co0 + cg * 8;  // 8 channels never straddle Cs (a multiple of 32)
+        bf16* out = co < a.Cs ? a.out0 : a.out1;
+        const int oc = co < a.Cs ? co : co - a.Cs;
+        const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
+        *reinterpret_cast<uint4*>(out + (((int64_t)n * a.H + y) * a.W + x) * Co + oc) =
+            *reinterpret_cast<const uint4*>(&s_out[px * TN + cg * 8]);
+    }
+    if (a.pool_out) {
+        // MaxPool2d(2) of the tile, as k_pool_fwd computes it (first maximum in row-major
+        // order, NaN propagates); H and W are even and tiles start at even rows and columns,
+        // so every window lies inside one tile
+        const int Hp = a.H >> 1, Wp = a.W >> 1;
+        constexpr int kPool = (TH / 2) * (kTW32 / 2) * (TN / 8);
+        for (int t = tid; t < kPool; t += kBlock) {
+            const int pp = t / (TN / 8), cg = t % (TN / 8);
+            const int ry = pp / (kTW32 / 2), rx = pp % (kTW32 / 2);
+            const int py = (y0 >> 1) + ry, pxo = (x0 >> 1) + rx;
+            if (py >= Hp || pxo >= Wp) continue;
+            float m[8], v[8];
+            unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx) * TN + cg * 8]), m);
+            const int offs[3] = {1, kTW32, kTW32 + 1};
+            for (int k = 0; k < 3; ++k) {
+                unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx + offs[k]) * TN + cg * 8]),
+                        v);
+                for (int e = 0; e < 8; ++e)
+                    if (v[e] > m[e] || v[e] != v[e]) m[e] = v[e];
+            }
+            *reinterpret_cast<uint4*>(a.pool_out + (((int64_t)n * Hp + py) * Wp + pxo) * a.Cout + co0 + cg * 8) =
+                pack8(m);
+        }
+    }
+}
+
 template <int TN, int RW, bool MASK>
 __global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
     constexpr int NG = TN / 32;
@@ -304,60 +364,157 @@ __global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
         }
         __syncthreads();
     }
-    // Epilogue through LDS: element j of lane l is pixel column 8*(j/4) + 4*(l/32) + j%4,
-    // channel l%32; the tile is written to LDS as [row][column][channel] (in the weight slice's
-    // space for 64 channels, the halo's for 32, the last chunk's barrier above freed both) and
-    // then stored 8 channels (16 bytes) per thread.
     static_assert(TN != 64 || TH * kTW32 * TN <= TN * kWRow, "64-channel tile fits the weight slice");
     static_assert(TN != 32 || TH * kTW32 * TN <= HH * kHW32 * kCK, "32-channel tile fits the halo");
-    bf16* s_out = TN == 64 ? s_w : s_in;
+    conv32_epilogue<TN, RW>(a, acc, TN == 64 ? s_w : s_in, n, y0, x0, co0);
+}
+
+// 3x3 convolution with the operands staged by direct global-to-LDS loads (buffer_load_dwordx4
+// ... lds): no staging registers, two LDS buffers, so chunk c + 1 streams into one buffer while
+// the MFMAs of chunk c read the other, and one barrier per 32-channel chunk. A load instruction
+// writes lane i's 16 bytes at the wave's LDS base + 16 i (checked on the device), so an LDS
+// "slot" (16 bytes) is filled by the lane whose global address the kernel picks for it; lanes
+// outside the image or past the data get an offset beyond the buffer's range and write zeros
+// (the padding of the convolution).
+// Slot layouts, with the 4 16-byte parts of a pixel (32 channels) or of a weight row (co, tap)
+// XOR-swizzled by bits 2..3 of the pixel / output channel, so that the 32-lane fragment reads
+// (16 lanes per LDS cycle, consecutive pixels or channels) hit distinct banks:
+//   halo   slot p*4 + (part ^ ((p >> 2) & 3)), pixel p = row * 34 + column
+//   weight slot tap*TN*4 + co*4 + (part ^ ((co >> 2) & 3))
+// Register blocking over kernel rows: for one kernel column kx and K half kh a wave reads the
+// RW + 2 halo rows it needs once and 3 weight fragments per 32 channels, and issues 3*RW*NG
+// MFMAs (0.5 fragment reads per MFMA at RW 4, TN 64, against 1 in k_conv32).
+constexpr unsigned kDlOut = 0x80000000u;  // buffer offset past the range: the load returns 0
+constexpr int kDlRsrcWord3 = 0x00020000;  // raw buffer, 32-bit data format
+
+template <int TN, int RW>
+struct DlShape {
+    static constexpr int NG = TN / 32, TH = RW * (kBlock / 64), HH = TH + 2;
+    static constexpr int kHalo = HH * kHW32 * 4;
+    static constexpr int kHaloSlots = (kHalo + kBlock - 1) / kBlock * kBlock;
+    static constexpr int kW = TN * 36;
+    static constexpr int kWSlots = (kW + kBlock - 1) / kBlock * kBlock;
+    static constexpr int kBuf = kHaloSlots + kWSlots;  // 16-byte slots per buffer
+};
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+template <int TN, int RW>
+__global__ __launch_bounds__(kBlock, 1) void k_conv_dl(ConvArgs a) {
+    using S = DlShape<TN, RW>;
+    constexpr int NG = S::NG, TH = S::TH;
+    static_assert(2 * S::kBuf * 16 <= 160 * 1024, "two buffers fit the 160 KB of LDS");
+    static_assert(TH * kTW32 * TN * 2 <= S::kBuf * 16, "output tile fits one buffer");
+    // Two buffers as two LDS objects, each access naming one of them: the compiler's wait
+    // counting then sees that loads into one buffer cannot alias reads of the other and does not
+    // wait for them before the MFMA loop's LDS reads.
+    __shared__ __attribute__((aligned(16))) uint4 s_b0[S::kBuf];
+    __shared__ __attribute__((aligned(16))) uint4 s_b1[S::kBuf];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.W + kTW32 - 1) / kTW32, tiles_y = (a.H + TH - 1) / TH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * TH, x0 = tx * kTW32;
+    const int co0 = blockIdx.y * TN;
+    const int Cin = a.C0 + a.C1;
+    const int nchunks = Cin / kCK;
+    // buffer resources: byte ranges of the two sources and of the weights (sizes < 2^31 checked
+    // by the launcher)
+    const __amdgpu_buffer_rsrc_t r0 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)a.src0, 0, (int)((int64_t)a.N * a.H * a.W * a.C0 * 2), kDlRsrcWord3);
+    const __amdgpu_buffer_rsrc_t r1 = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.src1 ? a.src1 : a.src0), 0, (int)((int64_t)a.N * a.H * a.W * a.C1 * 2), kDlRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rw = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)a.wk, 0, (int)((int64_t)a.Cout * 9 * Cin * 2), kDlRsrcWord3);
+    auto issue = [&](int c, uint4* base) {
+        const int ci0 = c * kCK;
+        const bool first = ci0 < a.C0;
+        const unsigned Cs = first ? a.C0 : a.C1;
+        const unsigned cc0 = first ? ci0 : ci0 - a.C0;
+#pragma unroll
+        for (int i = 0; i < S::kHaloSlots / kBlock; ++i) {
+            const int t = i * kBlock + tid;
+            unsigned off = kDlOut;
+            if (t < S::kHalo) {
+                const int p = t >> 2, part = (t & 3) ^ ((p >> 2) & 3);
+                const int hy = p / kHW32, hx = p % kHW32;
+                const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                    off = ((((unsigned)n * a.H + gy) * a.W + gx) * Cs + cc0 + part * 8) * 2u;
+            }
+            lds_void_t* dst = (lds_void_t*)(base + i * kBlock + wave * 64);
+            if (first) __builtin_amdgcn_raw_ptr_buffer_load_lds(r0, dst, 16, off, 0, 0, 0);
+            else __builtin_amdgcn_raw_ptr_buffer_load_lds(r1, dst, 16, off, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < S::kWSlots / kBlock; ++i) {
+            const int t = i * kBlock + tid;
+            unsigned off = kDlOut;
+            if (t < S::kW) {
+                const int tap = t / (TN * 4), r = t % (TN * 4), co = r >> 2, part = (r & 3) ^ ((co >> 2) & 3);
+                off = ((((unsigned)(co0 + co) * 9 + tap) * Cin) + ci0 + part * 8) * 2u;
+            }
+            lds_void_t* dst = (lds_void_t*)(base + S::kHaloSlots + i * kBlock + wave * 64);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rw, dst, 16, off, 0, 0, 0);
+        }
+    };
+    f32x16 acc[RW][NG];
     for (int mi = 0; mi < RW; ++mi)
-        for (int ng = 0; ng < NG; ++ng) {
-            const int c = ng * 32 + l32;
-            const float b = a.bias ? a.bias[co0 + c] : 0.0f;
-            for (int j = 0; j < 16; ++j) {
-                float v = acc[mi][ng][j] + b;
-                if (a.relu) v = fmaxf(v, 0.0f);
-                s_out[((wave * RW + mi) * kTW32 + 8 * (j >> 2) + 4 * kg + (j & 3)) * TN + c] = (bf16)v;
+        for (int ng = 0; ng < NG; ++ng)
+            for (int j = 0; j < 16; ++j) acc[mi][ng][j] = 0.f;
+    const int l32 = lane & 31, kg = lane >> 5;
+    auto compute = [&](const uint4* sh) {
+        const uint4* sw = sh + S::kHaloSlots;
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+#pragma unroll
+            for (int kh = 0; kh < 2; ++kh) {
+                const int k = kh * 2 + kg;
+                bf16x8 A[RW + 2], B[3][NG];
+#pragma unroll
+                for (int r = 0; r < RW + 2; ++r) {
+                    const int p = (wave * RW + r) * kHW32 + l32 + kx;
+                    const uint4 v = sh[p * 4 + (k ^ ((p >> 2) & 3))];
+                    A[r] = *reinterpret_cast<const bf16x8*>(&v);
+                }
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                    for (int ng = 0; ng < NG; ++ng) {
+                        const int co = ng * 32 + l32;
+                        const uint4 v = sw[(ky * 3 + kx) * TN * 4 + co * 4 + (k ^ ((co >> 2) & 3))];
+                        B[ky][ng] = *reinterpret_cast<const bf16x8*>(&v);
+                    }
+#pragma unroll
+                for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                    for (int mi = 0; mi < RW; ++mi)
+#pragma unroll
+                        for (int ng = 0; ng < NG; ++ng)
+                            acc[mi][ng] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[mi + ky], B[ky][ng], acc[mi][ng],
+                                                                                  0, 0, 0);
             }
         }
+    };
+    issue(0, s_b0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    constexpr int kVecs = TH * kTW32 * TN / 8;
-    for (int t = tid; t < kVecs; t += kBlock) {
-        const int px = t / (TN / 8), cg = t % (TN / 8);
-        const int y = y0 + px / kTW32, x = x0 + px % kTW32;
-        if (y >= a.H || x >= a.W) continue;
-        const int co = co0 + cg * 8;  // 8 channels never straddle Cs (a multiple of 32)
-        bf16* out = co < a.Cs ? a.out0 : a.out1;
-        const int oc = co < a.Cs ? co : co - a.Cs;
-        const int Co = co < a.Cs ? a.Cs : a.Cout - a.Cs;
-        *reinterpret_cast<uint4*>(out + (((int64_t)n * a.H + y) * a.W + x) * Co + oc) =
-            *reinterpret_cast<const uint4*>(&s_out[px * TN + cg * 8]);
+    // chunk c reads buffer c % 2 while chunk c + 1 streams into the other; each step ends with
+    // this thread's loads waited for and a barrier (every load landed, nobody reads the buffer
+    // the next step refills)
+    for (int c = 0; c < nchunks; c += 2) {
+        if (c + 1 < nchunks) issue(c + 1, s_b1);
+        compute(s_b0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (c + 1 >= nchunks) break;
+        if (c + 2 < nchunks) issue(c + 2, s_b0);
+        compute(s_b1);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
     }
-    if (a.pool_out) {
-        // MaxPool2d(2) of the tile, as k_pool_fwd computes it (first maximum in row-major
-        // order, NaN propagates); H and W are even and tiles start at even rows and columns,
-        // so every window lies inside one tile
-        const int Hp = a.H >> 1, Wp = a.W >> 1;
-        constexpr int kPool = (TH / 2) * (kTW32 / 2) * (TN / 8);
-        for (int t = tid; t < kPool; t += kBlock) {
-            const int pp = t / (TN / 8), cg = t % (TN / 8);
-            const int ry = pp / (kTW32 / 2), rx = pp % (kTW32 / 2);
-            const int py = (y0 >> 1) + ry, pxo = (x0 >> 1) + rx;
-            if (py >= Hp || pxo >= Wp) continue;
-            float m[8], v[8];
-            unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx) * TN + cg * 8]), m);
-            const int offs[3] = {1, kTW32, kTW32 + 1};
-            for (int k = 0; k < 3; ++k) {
-                unpack8(*reinterpret_cast<const uint4*>(&s_out[((2 * ry) * kTW32 + 2 * rx + offs[k]) * TN + cg * 8]),
-                        v);
-                for (int e = 0; e < 8; ++e)
-                    if (v[e] > m[e] || v[e] != v[e]) m[e] = v[e];
-            }
-            *reinterpret_cast<uint4*>(a.pool_out + (((int64_t)n * Hp + py) * Wp + pxo) * a.Cout + co0 + cg * 8) =
-                pack8(m);
-        }
-    }
+    conv32_epilogue<TN, RW>(a, acc, reinterpret_cast<bf16*>(s_b0), n, y0, x0, co0);
 }
 
 // Register-blocked 3x3 convolution. Each wave owns RW output rows x 16 columns x 16*NB output
@@ -890,10 +1047,12 @@ __global__ void k_tail_reduce(const float* partial, int blocks, float* dw, float
 enum ConvVariant {
     kVarSimple = 0, kVarRb4x4, kVarRb8x2, kVarRb4x2, kVarRb8x4,
     kVarM32r2n64, kVarM32r2n32, kVarM32r4n32,  // k_conv32<TN, RW> (RW 4 with TN 64 spills)
+    kVarDl4n64, kVarDl2n64, kVarDl4n32,        // k_conv_dl<TN, RW> (direct-to-LDS staging)
     kNumVariants
 };
-constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8, 2, 2, 4};
-constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4, 4, 2, 2};  // output channels / 16
+static_assert(kNumVariants == LGI_UNET_CONV_VARIANTS, "variant count in the C ABI");
+constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8, 2, 2, 4, 4, 2, 4};
+constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2};  // output channels / 16
 
 struct TuneEntry {
     int variant;
@@ -909,6 +1068,13 @@ std::atomic<int> g_force_variant{-1};
 bool variant_fits(int v, const ConvArgs& a, int taps) {
     if (v == kVarSimple) return true;
     if (taps != 9 || v < 0 || v >= kNumVariants) return false;
+    if (v >= kVarDl4n64) {
+        // direct loads stage raw input (no ReLU-mask multiply) through 32-bit buffer offsets
+        const int64_t px = (int64_t)a.N * a.H * a.W, lim = int64_t(1) << 31;
+        if (a.mask0 || a.mask1 || px * a.C0 * 2 >= lim || px * a.C1 * 2 >= lim ||
+            (int64_t)a.Cout * 9 * (a.C0 + a.C1) * 2 >= lim)
+            return false;
+    }
     return a.Cout % (16 * kVarNB[v]) == 0;
 }
 
@@ -943,6 +1109,16 @@ void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
         return;
     }
     const int th = kVarRW[v] * (kBlock / 64), tn = 16 * kVarNB[v];
+    if (v >= kVarDl4n64) {
+        const dim3 grid(cdiv(a.W, kTW32) * cdiv(a.H, th) * a.N, a.Cout / tn);
+        switch (v) {
+            case kVarDl4n64: hipLaunchKernelGGL((k_conv_dl<64, 4>), grid, dim3(kBlock), 0, st, a); break;
+            case kVarDl2n64: hipLaunchKernelGGL((k_conv_dl<64, 2>), grid, dim3(kBlock), 0, st, a); break;
+            case kVarDl4n32: hipLaunchKernelGGL((k_conv_dl<32, 4>), grid, dim3(kBlock), 0, st, a); break;
+            default: break;
+        }
+        return;
+    }
     if (v >= kVarM32r2n64) {
         const dim3 grid(cdiv(a.W, kTW32) * cdiv(a.H, th) * a.N, a.Cout / tn);
         const bool m = a.mask0 || a.mask1;

@@ -45,7 +45,9 @@ def _check(t, dtype, channels=None):
 # ------------------------------------------------------------------ tile-shape selection
 CONV_VARIANTS = {0: "8x16 px, 32/64 ch", 1: "16x16 px, 64 ch, rb", 2: "32x16 px, 32 ch, rb",
                  3: "16x16 px, 32 ch, rb", 4: "32x16 px, 64 ch, rb", 5: "8x32 px, 64 ch, 32x32 MFMA",
-                 6: "8x32 px, 32 ch, 32x32 MFMA", 7: "16x32 px, 32 ch, 32x32 MFMA"}
+                 6: "8x32 px, 32 ch, 32x32 MFMA", 7: "16x32 px, 32 ch, 32x32 MFMA",
+                 8: "16x32 px, 64 ch, direct LDS loads", 9: "8x32 px, 64 ch, direct LDS loads",
+                 10: "16x32 px, 32 ch, direct LDS loads"}
 
 
 def conv_tuning(enabled=True):
