@@ -296,6 +296,16 @@ LGI_API lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const
                                              const void* w, const float* bias, int32_t cout, int32_t n,
                                              int32_t h, int32_t w_, int32_t relu, void* out, void* pool_out,
                                              void* stream);
+/* The denoiser's last convolution and its tail in one kernel (inference): y = relu?(conv3x3(src0) +
+ * bias) with 32 output channels, then out (n, h, w, 3) FP32 = max(expm1(log1p(max(hdr, 0)) +
+ * tail_w . y + tail_b), 0), tail_w [3][32], tail_b [3] (ml/train.py's last ConvBlock, tail and
+ * residual, in the radiance domain). y is never written: the tail reads it from the output tile
+ * on chip. Bit-identical to lgi_unet_conv3x3_fwd followed by lgi_unet_tail_radiance_fwd with the
+ * same tiling. */
+LGI_API lgi_status lgi_unet_conv3x3_tail_radiance_fwd(const void* src0, int32_t c0, const void* w,
+                                                      const float* bias, int32_t n, int32_t h, int32_t w_,
+                                                      int32_t relu, const float* tail_w, const float* tail_b,
+                                                      const float* hdr, float* out, void* stream);
 /* First layer for inference: y = relu?(conv3x3(x) + bias) with x the FP32 3-channel NHWC
  * input (read directly, no padded copy), w [32][9][cin_w] BF16 of which channels 0..2 are used,
  * y (n, h, w, 32) BF16. Equals lgi_unet_conv3x3_fwd on lgi_unet_pack_input(x) up to FP32
@@ -327,7 +337,8 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
  * capture and mode 0 use a fixed choice. force_variant(v >= 0) makes every call use variant v
  * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
  * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, kind, relu, variant} with kind
- * 0 = forward, 1 = ReLU-masked input (data gradient), 2 = forward with a pooled copy,
+ * 0 = forward, 1 = ReLU-masked input (data gradient), 2 = forward with a pooled copy, 3 = forward
+ * with the fused denoiser tail,
  * us[LGI_UNET_CONV_VARIANTS*i..] = microseconds per launch of each variant (0 = does not fit);
  * *n = entries. */
 #define LGI_UNET_CONV_VARIANTS 11

@@ -54,6 +54,12 @@ struct ConvArgs {
     int N, H, W;
     int relu;
     bf16* pool_out;        // optional [N,H/2,W/2,Cout]: 2x2 max-pool of the output (k_conv32 only)
+    // optional denoiser tail on the output (Cout 32, 32x32-MFMA tilings): tail_out[N,H,W,3] =
+    // max(expm1(log1p(max(tail_x, 0)) + tail_w . y + tail_b), 0); the 32-channel y is not stored
+    const float* tail_w;   // [3][32]
+    const float* tail_b;   // [3]
+    const float* tail_x;   // [N,H,W,3] radiance
+    float* tail_out;
 };
 
 __device__ __forceinline__ uint4 zero16() { return make_uint4(0u, 0u, 0u, 0u); }
@@ -207,6 +213,14 @@ __global__ __launch_bounds__(kBlock) void k_conv(ConvArgs a) {
 // in lane n + 32*((m/4)%2), element 4*(m/8) + m%4.
 constexpr int kTW32 = 32, kHW32 = kTW32 + 2;
 
+// Denoiser input transform: radiance -> log1p(max(radiance, 0)), NaN kept (torch's clamp_min
+// and log1p order), and its inverse on the output, max(expm1(v), 0).
+__device__ __forceinline__ float to_log(float v) { return log1pf(v < 0.f ? 0.f : v); }
+__device__ __forceinline__ float from_log(float v) {
+    const float r = expm1f(v);
+    return r < 0.f ? 0.f : r;
+}
+
 // Epilogue of the 32x32-MFMA convolutions: bias, ReLU, BF16 output tile of (4*RW rows x 32
 // columns) x TN channels through LDS, optional 2x2 max-pool of it. Element j of lane l is pixel
 // column 8*(j/4) + 4*(l/32) + j%4, channel l%32; the tile is written to s_out as [row][column]
@@ -229,6 +243,22 @@ __device__ __forceinline__ void conv32_epilogue(const ConvArgs& a, const f32x16 
             }
         }
     __syncthreads();
+    if (TN == 32 && a.tail_out) {
+        // the tail on the BF16 features of each pixel, in k_tail_fwd's order of operations
+        for (int px = tid; px < TH * kTW32; px += kBlock) {
+            const int y = y0 + px / kTW32, x = x0 + px % kTW32;
+            if (y >= a.H || x >= a.W) continue;
+            float f[32];
+            for (int q = 0; q < 4; ++q) unpack8(*reinterpret_cast<const uint4*>(&s_out[px * TN + q * 8]), f + q * 8);
+            const int64_t p = ((int64_t)n * a.H + y) * a.W + x;
+            for (int o = 0; o < 3; ++o) {
+                float acc_t = a.tail_b[o];
+                for (int c = 0; c < 32; ++c) acc_t += a.tail_w[o * 32 + c] * f[c];
+                a.tail_out[p * 3 + o] = from_log(to_log(a.tail_x[p * 3 + o]) + acc_t);
+            }
+        }
+        return;
+    }
     constexpr int kVecs = TH * kTW32 * TN / 8;
     for (int t = tid; t < kVecs; t += kBlock) {
         const int px = t / (TN / 8), cg = t % (TN / 8);
@@ -620,14 +650,6 @@ __global__ __launch_bounds__(kBlock, RW * NB >= 32 ? 1 : 2) void k_conv_rb(ConvA
 // one K = 27 (+5 zero) row, so a 16-pixel x 16-channel output fragment is a single MFMA
 // instead of the 9 taps x 32 (mostly zero) channels of the packed-input path. Output tile
 // 8 x 16 pixels x 32 channels; weights [32][9][cin_w] BF16 (channels 0..2 used).
-// Denoiser input transform: radiance -> log1p(max(radiance, 0)), NaN kept (torch's clamp_min
-// and log1p order), and its inverse on the output, max(expm1(v), 0).
-__device__ __forceinline__ float to_log(float v) { return log1pf(v < 0.f ? 0.f : v); }
-__device__ __forceinline__ float from_log(float v) {
-    const float r = expm1f(v);
-    return r < 0.f ? 0.f : r;
-}
-
 // kLog: x holds radiance and the layer reads log1p(max(x, 0)).
 template <bool kLog>
 __global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__ x, const bf16* __restrict__ wk,
@@ -1066,6 +1088,8 @@ std::atomic<int> g_force_variant{-1};
 // With pool_out, the k_conv32 variants write the pooled copy from their output tile and the
 // others are followed by a k_pool_fwd launch; the tuner times each pair as one candidate.
 bool variant_fits(int v, const ConvArgs& a, int taps) {
+    // the fused tail lives in the epilogue of the 32-channel 32x32-MFMA tilings
+    if (a.tail_out && !(v == kVarM32r2n32 || v == kVarM32r4n32 || v == kVarDl4n32)) return false;
     if (v == kVarSimple) return true;
     if (taps != 9 || v < 0 || v >= kNumVariants) return false;
     if (v >= kVarDl4n64) {
@@ -1080,7 +1104,7 @@ bool variant_fits(int v, const ConvArgs& a, int taps) {
 
 int heuristic_variant(const ConvArgs& a, int taps) {
     (void)taps;
-    return a.pool_out ? kVarM32r2n32 : kVarSimple;
+    return (a.pool_out || a.tail_out) ? kVarM32r2n32 : kVarSimple;
 }
 
 void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
@@ -1185,14 +1209,18 @@ lgi_status conv_launch(const ConvArgs& a, int taps, hipStream_t st) {
     if (a.pool_out && (taps != 9 || a.Cs != a.Cout || a.mask0 || a.mask1 || a.H % 2 || a.W % 2 ||
                        !row_grid_fits(a.N, a.H / 2, a.W / 2, a.Cout / 8)))
         return LGI_ERR_INVALID_ARG;
+    if (a.tail_out && (taps != 9 || a.Cout != 32 || a.Cs != 32 || a.mask0 || a.mask1 || a.pool_out || !a.tail_w ||
+                       !a.tail_b || !a.tail_x))
+        return LGI_ERR_INVALID_ARG;
     int v = g_force_variant.load();
     if (v >= 0) {
         if (!variant_fits(v, a, taps)) return LGI_ERR_UNSUPPORTED;
     } else {
         v = heuristic_variant(a, taps);
         if (taps == 9 && g_tune_mode.load() == 1) {
-            // input kind: 0 plain, 1 ReLU-masked (data gradient), 2 plain with a pooled copy
-            const int kind = a.mask0 != nullptr ? 1 : (a.pool_out ? 2 : 0);
+            // input kind: 0 plain, 1 ReLU-masked (data gradient), 2 plain with a pooled copy,
+            // 3 with the denoiser tail in the epilogue
+            const int kind = a.mask0 != nullptr ? 1 : (a.pool_out ? 2 : (a.tail_out ? 3 : 0));
             const std::array<int, 9> key{a.N, a.H, a.W, a.C0, a.C1, a.Cout, a.Cs, kind, a.relu};
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
@@ -1239,6 +1267,18 @@ lgi_status lgi_unet_conv3x3_pool_fwd(const void* src0, int32_t c0, const void* s
     a.wk = (const bf16*)w; a.bias = bias; a.out0 = (bf16*)out; a.out1 = nullptr;
     a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu; a.pool_out = (bf16*)pool_out;
     if (!src0 || !w || !out || !pool_out || h % 2 || wd % 2) return LGI_ERR_INVALID_ARG;
+    return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_conv3x3_tail_radiance_fwd(const void* src0, int32_t c0, const void* w, const float* bias,
+                                              int32_t n, int32_t h, int32_t wd, int32_t relu, const float* tail_w,
+                                              const float* tail_b, const float* hdr, float* out, void* stream) {
+    ConvArgs a{};
+    a.src0 = (const bf16*)src0; a.src1 = nullptr; a.C0 = c0; a.C1 = 0;
+    a.wk = (const bf16*)w; a.bias = bias; a.out0 = nullptr; a.out1 = nullptr;
+    a.Cout = 32; a.Cs = 32; a.N = n; a.H = h; a.W = wd; a.relu = relu;
+    a.tail_w = tail_w; a.tail_b = tail_b; a.tail_x = hdr; a.tail_out = out;
+    if (!src0 || !w || !out || !tail_w || !tail_b || !hdr) return LGI_ERR_INVALID_ARG;
     return conv_launch(a, 9, (hipStream_t)stream);
 }
 

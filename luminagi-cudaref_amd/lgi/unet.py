@@ -116,6 +116,19 @@ def conv3x3_first_fwd(x, w_bf16, bias, relu=True, radiance=False):
     return out
 
 
+def conv3x3_tail_radiance_fwd(src0, w_bf16, bias, tail_w, tail_b, hdr, relu=True):
+    """Last decoder convolution (32 channels) with the radiance-domain tail in its epilogue:
+    max(expm1(log1p(max(hdr, 0)) + conv1x1(relu(conv3x3(src0)))), 0) without writing the features."""
+    n, h, w, c0 = src0.shape
+    _check(src0, torch.bfloat16)
+    _check(hdr, torch.float32, 3)
+    assert w_bf16.shape[0] == 32
+    out = torch.empty((n, h, w, 3), dtype=torch.float32, device=src0.device)
+    _call("lgi_unet_conv3x3_tail_radiance_fwd", _p(src0), c0, _p(w_bf16), _p(bias), n, h, w, int(relu),
+          _p(tail_w.contiguous()), _p(tail_b.contiguous()), _p(hdr), _p(out), _stream())
+    return out
+
+
 def tail_radiance_fwd(feat, hdr, w, b):
     """Inference tail in the radiance domain: max(expm1(log1p(max(hdr, 0)) + conv1x1(feat)), 0)."""
     n, h, wd, c = feat.shape
@@ -341,6 +354,10 @@ class UNet(torch.nn.Module):
         if taps is not None:
             taps["mid"] = h
         for i in reversed(range(len(ENC))):
+            if radiance and i == 0 and taps is None:
+                # the last convolution with the tail in its epilogue: dec0's output never leaves the chip
+                y = self.conv("dec0a", UpsampleFn.apply(h), skips[0])
+                return conv3x3_tail_radiance_fwd(y, self._wb("dec0b"), self.biases["dec0b"], self.tail_w, self.tail_b, x)
             h = self._block(f"dec{i}", UpsampleFn.apply(h), skips[i])
             if taps is not None:
                 taps[f"dec{i}"] = h

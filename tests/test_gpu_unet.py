@@ -410,3 +410,32 @@ def test_radiance_entry_points_match_torch_transforms(gpu):
     assert torch.equal(f0, f1) or rel(f1, f0) < 1e-2
     with pytest.raises(ValueError):
         model(hdr.unsqueeze(0), radiance=True)
+
+
+@pytest.mark.parametrize("variant", [None, 6, 7, 10], ids=lambda v: "tuned" if v is None else f"variant{v}")
+@pytest.mark.parametrize("shape", [(1, 16, 64, 64), (2, 13, 45, 32)], ids=lambda s: "x".join(map(str, s)))
+def test_conv3x3_tail_equals_conv_then_tail(gpu, variant, shape):
+    """The last convolution with the radiance-domain tail in its epilogue: bit-identical to the
+    same tiling's convolution followed by the tail kernel (ragged tiles, negative radiance)."""
+    from lgi import unet
+    n, h, w, c0 = shape
+    x0 = bf16_rand(n, h, w, c0, seed=51).cuda()
+    wt = bf16_rand(32, 9, c0, scale=(2.0 / (9 * c0)) ** 0.5, seed=52).cuda()
+    b = (torch.randn(32, generator=torch.Generator().manual_seed(53)) * 0.1).cuda()
+    g = torch.Generator().manual_seed(54)
+    tw, tb = (torch.randn(3, 32, generator=g) * 0.1).cuda(), (torch.randn(3, generator=g) * 0.1).cuda()
+    hdr = (torch.rand(n, h, w, 3, generator=g) * 3.0 - 0.2).cuda()
+    unet.conv_tuning(True)
+    unet.force_conv_variant(variant)
+    try:
+        out = unet.conv3x3_tail_radiance_fwd(x0, wt, b, tw, tb, hdr)
+        if variant is None:
+            chosen = [r for r in unet.conv_tune_table()
+                      if (r[0]["h"], r[0]["w"], r[0]["c0"], r[0]["cout"], r[0]["kind"]) == (h, w, c0, 32, 3)]
+            assert len(chosen) == 1 and chosen[0][1] in (6, 7, 10)
+            unet.force_conv_variant(chosen[0][1])
+        y = unet.conv3x3_fwd(x0, None, wt, b, True)
+    finally:
+        unet.force_conv_variant(None)
+    ref = unet.tail_radiance_fwd(y, hdr, tw, tb)
+    assert out.shape == (n, h, w, 3) and torch.equal(out, ref)
