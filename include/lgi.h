@@ -306,6 +306,16 @@ LGI_API lgi_status lgi_unet_conv3x3_tail_radiance_fwd(const void* src0, int32_t 
                                                       const float* bias, int32_t n, int32_t h, int32_t w_,
                                                       int32_t relu, const float* tail_w, const float* tail_b,
                                                       const float* hdr, float* out, void* stream);
+/* lgi_unet_conv3x3_fwd (or _pool_fwd with pool_out non-null) on FP8 E4M3 operands: the BF16
+ * inputs are quantized on chip with the per-tensor act_scale, q = e4m3(clamp(x / act_scale, +-448))
+ * rounded to nearest even; w_e4m3 holds the weights [cout][9][c0 + c1] quantized per output
+ * channel (w ~= q * w_scale[co]); FP32 accumulation, then * act_scale * w_scale[co] + bias,
+ * ReLU and BF16 output. c0 and c1 multiples of 64. The reference UNet runs in FP32 (ml/train.py);
+ * this is the denoiser's reduced-precision inference path. */
+LGI_API lgi_status lgi_unet_conv3x3_fp8_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1,
+                                            const void* w_e4m3, const float* w_scale, float act_scale,
+                                            const float* bias, int32_t cout, int32_t n, int32_t h, int32_t w_,
+                                            int32_t relu, void* out, void* pool_out, void* stream);
 /* First layer for inference: y = relu?(conv3x3(x) + bias) with x the FP32 3-channel NHWC
  * input (read directly, no padded copy), w [32][9][cin_w] BF16 of which channels 0..2 are used,
  * y (n, h, w, 32) BF16. Equals lgi_unet_conv3x3_fwd on lgi_unet_pack_input(x) up to FP32
@@ -338,10 +348,10 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask
  * (LGI_ERR_UNSUPPORTED where it does not fit), -1 restores selection. tune_table copies up to
  * cap entries: rows[10*i..] = {n, h, w, c0, c1, cout, cout_split, kind, relu, variant} with kind
  * 0 = forward, 1 = ReLU-masked input (data gradient), 2 = forward with a pooled copy, 3 = forward
- * with the fused denoiser tail,
+ * with the fused denoiser tail, each + 8 with FP8 operands,
  * us[LGI_UNET_CONV_VARIANTS*i..] = microseconds per launch of each variant (0 = does not fit);
  * *n = entries. */
-#define LGI_UNET_CONV_VARIANTS 11
+#define LGI_UNET_CONV_VARIANTS 14
 LGI_API lgi_status lgi_unet_conv_tuning(int32_t mode);
 LGI_API lgi_status lgi_unet_conv_force_variant(int32_t variant);
 LGI_API lgi_status lgi_unet_conv_tune_table(int32_t* rows, float* us, int32_t cap, int32_t* n);

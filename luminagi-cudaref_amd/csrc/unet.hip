@@ -60,6 +60,11 @@ struct ConvArgs {
     const float* tail_b;   // [3]
     const float* tail_x;   // [N,H,W,3] radiance
     float* tail_out;
+    // FP8 operands (k_conv_f8): wk holds E4M3 weights [Cout][9][Cin] (bytes), w_scale[Cout] their
+    // per-output-channel scales, act_scale the per-tensor scale the BF16 inputs are quantized with
+    const float* w_scale;
+    float act_scale;
+    int fp8;
 };
 
 __device__ __forceinline__ uint4 zero16() { return make_uint4(0u, 0u, 0u, 0u); }
@@ -397,6 +402,133 @@ __global__ __launch_bounds__(kBlock) void k_conv32(ConvArgs a) {
     static_assert(TN != 64 || TH * kTW32 * TN <= TN * kWRow, "64-channel tile fits the weight slice");
     static_assert(TN != 32 || TH * kTW32 * TN <= HH * kHW32 * kCK, "32-channel tile fits the halo");
     conv32_epilogue<TN, RW>(a, acc, TN == 64 ? s_w : s_in, n, y0, x0, co0);
+}
+
+// 3x3 convolution on FP8 (E4M3) operands: v_mfma_scale_f32_32x32x64_f8f6f4 with unit block
+// scales, FP32 accumulation. One MFMA multiplies 32 pixels x 64 input channels by 64 input
+// channels x 32 output channels: twice the multiply-adds per operand byte of the BF16 32x32x16
+// instruction and twice its rate. The BF16 input is quantized while it is staged into LDS,
+// q = e4m3(clamp(x / act_scale, +-448)) rounded to nearest even (the converter turns values
+// beyond 448 into NaN, hence the clamp); the weights arrive quantized per output channel,
+// w ~= q_w * w_scale[co]; the epilogue multiplies the FP32 sums by act_scale * w_scale[co] before
+// the bias, ReLU and BF16 store. Operand layout (checked on the device): A byte j of lane
+// m + 32 h holds k = 32 h + j of pixel m; B the same for output channel n; D as in k_conv32.
+// Tile and pipeline as k_conv32: (4*RW rows x 32 columns) x TN channels, 64 input channels per
+// chunk, the next chunk's loads in registers during the current chunk's MFMAs.
+constexpr int kCK8 = 64;                 // input channels per FP8 chunk
+constexpr int kWRow8 = 9 * kCK8 + 16;    // LDS weight row (bytes), padded against bank conflicts
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint2 quant8_e4m3(const uint4& v, float inv_scale) {
+    float f[8];
+    unpack8(v, f);
+    float c[8];
+    for (int k = 0; k < 8; ++k) c[k] = fminf(fmaxf(f[k] * inv_scale, -448.f), 448.f);
+    int w0 = __builtin_amdgcn_cvt_pk_fp8_f32(c[0], c[1], 0, false);   // bytes 0, 1
+    w0 = __builtin_amdgcn_cvt_pk_fp8_f32(c[2], c[3], w0, true);        // bytes 2, 3
+    int w1 = __builtin_amdgcn_cvt_pk_fp8_f32(c[4], c[5], 0, false);
+    w1 = __builtin_amdgcn_cvt_pk_fp8_f32(c[6], c[7], w1, true);
+    return make_uint2((unsigned)w0, (unsigned)w1);
+}
+
+template <int TN, int RW>
+__global__ __launch_bounds__(kBlock) void k_conv_f8(ConvArgs a) {
+    constexpr int NG = TN / 32;
+    constexpr int TH = RW * (kBlock / 64), HH = TH + 2;
+    constexpr int kHalo = HH * kHW32 * 8;   // 16-byte BF16 loads (8 channels) of one chunk's halo
+    constexpr int kWLd = TN * 9 * 4;        // 16-byte FP8 loads (16 channels) of one chunk's weights
+    constexpr int kHIt = (kHalo + kBlock - 1) / kBlock, kWIt = (kWLd + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) uint8_t s_in[HH * kHW32 * kCK8];
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[TN * kWRow8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (a.W + kTW32 - 1) / kTW32, tiles_y = (a.H + TH - 1) / TH;
+    const int tile = blockIdx.x;
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int y0 = ty * TH, x0 = tx * kTW32;
+    const int co0 = blockIdx.y * TN;
+    const int Cin = a.C0 + a.C1;
+    const float inv_sa = 1.0f / a.act_scale;
+    const uint8_t* wq = reinterpret_cast<const uint8_t*>(a.wk);
+    f32x16 acc[RW][NG];
+    for (int mi = 0; mi < RW; ++mi)
+        for (int ng = 0; ng < NG; ++ng)
+            for (int j = 0; j < 16; ++j) acc[mi][ng][j] = 0.f;
+    const int l32 = lane & 31, kg = lane >> 5;
+    uint4 hv[kHIt], wv[kWIt];
+    auto load_chunk = [&](int ci0) {
+        const bool first = ci0 < a.C0;
+        const bf16* src = first ? a.src0 : a.src1;
+        const int Cs = first ? a.C0 : a.C1;
+        const int cc0 = first ? ci0 : ci0 - a.C0;
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            hv[i] = zero16();
+            if (t < kHalo) {
+                const int px = t >> 3, part = t & 7;
+                const int hy = px / kHW32, hx = px % kHW32;
+                const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                    hv[i] = *reinterpret_cast<const uint4*>(src + (((int64_t)n * a.H + gy) * a.W + gx) * Cs + cc0 +
+                                                            part * 8);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            wv[i] = zero16();
+            if (t < kWLd) {
+                const int co = t / 36, r = t % 36, tap = r >> 2, part = r & 3;
+                wv[i] = *reinterpret_cast<const uint4*>(wq + ((int64_t)(co0 + co) * 9 + tap) * Cin + ci0 + part * 16);
+            }
+        }
+    };
+    load_chunk(0);
+    for (int ci0 = 0; ci0 < Cin; ci0 += kCK8) {
+#pragma unroll
+        for (int i = 0; i < kHIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kHalo) *reinterpret_cast<uint2*>(&s_in[(t >> 3) * kCK8 + (t & 7) * 8]) = quant8_e4m3(hv[i], inv_sa);
+        }
+#pragma unroll
+        for (int i = 0; i < kWIt; ++i) {
+            const int t = tid + i * kBlock;
+            if (t < kWLd) {
+                const int co = t / 36, r = t % 36, tap = r >> 2, part = r & 3;
+                *reinterpret_cast<uint4*>(&s_w[co * kWRow8 + tap * kCK8 + part * 16]) = wv[i];
+            }
+        }
+        __syncthreads();
+        if (ci0 + kCK8 < Cin) load_chunk(ci0 + kCK8);
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            i32x8 A[RW], B[NG];
+            for (int mi = 0; mi < RW; ++mi) {
+                const int row = wave * RW + mi + ky;
+                const uint4* q = reinterpret_cast<const uint4*>(&s_in[(row * kHW32 + l32 + kx) * kCK8 + kg * 32]);
+                const uint4 lo = q[0], hi = q[1];
+                A[mi] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+            for (int ng = 0; ng < NG; ++ng) {
+                const uint4* q = reinterpret_cast<const uint4*>(&s_w[(ng * 32 + l32) * kWRow8 + tap * kCK8 + kg * 32]);
+                const uint4 lo = q[0], hi = q[1];
+                B[ng] = i32x8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+            for (int mi = 0; mi < RW; ++mi)
+                for (int ng = 0; ng < NG; ++ng)
+                    acc[mi][ng] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[mi], B[ng], acc[mi][ng], 0, 0, 0,
+                                                                                  127, 0, 127);
+        }
+        __syncthreads();
+    }
+    for (int ng = 0; ng < NG; ++ng) {
+        const float sc = a.act_scale * a.w_scale[co0 + ng * 32 + l32];
+        for (int mi = 0; mi < RW; ++mi)
+            for (int j = 0; j < 16; ++j) acc[mi][ng][j] *= sc;
+    }
+    static_assert(TH * kTW32 * TN * 2 <= TN * kWRow8, "output tile fits the weight slice");
+    conv32_epilogue<TN, RW>(a, acc, reinterpret_cast<bf16*>(s_w), n, y0, x0, co0);
 }
 
 // 3x3 convolution with the operands staged by direct global-to-LDS loads (buffer_load_dwordx4
@@ -1070,11 +1202,12 @@ enum ConvVariant {
     kVarSimple = 0, kVarRb4x4, kVarRb8x2, kVarRb4x2, kVarRb8x4,
     kVarM32r2n64, kVarM32r2n32, kVarM32r4n32,  // k_conv32<TN, RW> (RW 4 with TN 64 spills)
     kVarDl4n64, kVarDl2n64, kVarDl4n32,        // k_conv_dl<TN, RW> (direct-to-LDS staging)
+    kVarF8r2n64, kVarF8r1n64, kVarF8r2n32,     // k_conv_f8<TN, RW> (FP8 operands only)
     kNumVariants
 };
 static_assert(kNumVariants == LGI_UNET_CONV_VARIANTS, "variant count in the C ABI");
-constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8, 2, 2, 4, 4, 2, 4};
-constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2};  // output channels / 16
+constexpr int kVarRW[kNumVariants] = {0, 4, 8, 4, 8, 2, 2, 4, 4, 2, 4, 2, 1, 2};
+constexpr int kVarNB[kNumVariants] = {0, 4, 2, 2, 4, 4, 2, 2, 4, 4, 2, 4, 4, 2};  // output channels / 16
 
 struct TuneEntry {
     int variant;
@@ -1088,6 +1221,11 @@ std::atomic<int> g_force_variant{-1};
 // With pool_out, the k_conv32 variants write the pooled copy from their output tile and the
 // others are followed by a k_pool_fwd launch; the tuner times each pair as one candidate.
 bool variant_fits(int v, const ConvArgs& a, int taps) {
+    // FP8 operands run only on the FP8 kernel, which needs 64-channel chunks of one source
+    const bool f8v = v >= kVarF8r2n64 && v < kNumVariants;
+    if (a.fp8 != 0 || f8v)
+        return a.fp8 != 0 && f8v && taps == 9 && !a.mask0 && !a.mask1 && a.C0 % kCK8 == 0 && a.C1 % kCK8 == 0 &&
+               a.Cout % (16 * kVarNB[v]) == 0;
     // the fused tail lives in the epilogue of the 32-channel 32x32-MFMA tilings
     if (a.tail_out && !(v == kVarM32r2n32 || v == kVarM32r4n32 || v == kVarDl4n32)) return false;
     if (v == kVarSimple) return true;
@@ -1104,6 +1242,7 @@ bool variant_fits(int v, const ConvArgs& a, int taps) {
 
 int heuristic_variant(const ConvArgs& a, int taps) {
     (void)taps;
+    if (a.fp8) return a.Cout % 64 == 0 ? kVarF8r2n64 : kVarF8r2n32;
     return (a.pool_out || a.tail_out) ? kVarM32r2n32 : kVarSimple;
 }
 
@@ -1133,6 +1272,16 @@ void launch_variant(int v, const ConvArgs& a, int taps, hipStream_t st) {
         return;
     }
     const int th = kVarRW[v] * (kBlock / 64), tn = 16 * kVarNB[v];
+    if (v >= kVarF8r2n64) {
+        const dim3 grid(cdiv(a.W, kTW32) * cdiv(a.H, th) * a.N, a.Cout / tn);
+        switch (v) {
+            case kVarF8r2n64: hipLaunchKernelGGL((k_conv_f8<64, 2>), grid, dim3(kBlock), 0, st, a); break;
+            case kVarF8r1n64: hipLaunchKernelGGL((k_conv_f8<64, 1>), grid, dim3(kBlock), 0, st, a); break;
+            case kVarF8r2n32: hipLaunchKernelGGL((k_conv_f8<32, 2>), grid, dim3(kBlock), 0, st, a); break;
+            default: break;
+        }
+        return;
+    }
     if (v >= kVarDl4n64) {
         const dim3 grid(cdiv(a.W, kTW32) * cdiv(a.H, th) * a.N, a.Cout / tn);
         switch (v) {
@@ -1219,8 +1368,8 @@ lgi_status conv_launch(const ConvArgs& a, int taps, hipStream_t st) {
         v = heuristic_variant(a, taps);
         if (taps == 9 && g_tune_mode.load() == 1) {
             // input kind: 0 plain, 1 ReLU-masked (data gradient), 2 plain with a pooled copy,
-            // 3 with the denoiser tail in the epilogue
-            const int kind = a.mask0 != nullptr ? 1 : (a.pool_out ? 2 : (a.tail_out ? 3 : 0));
+            // 3 with the denoiser tail in the epilogue; + 8 with FP8 operands
+            const int kind = (a.mask0 != nullptr ? 1 : (a.pool_out ? 2 : (a.tail_out ? 3 : 0))) + (a.fp8 ? 8 : 0);
             const std::array<int, 9> key{a.N, a.H, a.W, a.C0, a.C1, a.Cout, a.Cs, kind, a.relu};
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tune.find(key);
@@ -1279,6 +1428,19 @@ lgi_status lgi_unet_conv3x3_tail_radiance_fwd(const void* src0, int32_t c0, cons
     a.Cout = 32; a.Cs = 32; a.N = n; a.H = h; a.W = wd; a.relu = relu;
     a.tail_w = tail_w; a.tail_b = tail_b; a.tail_x = hdr; a.tail_out = out;
     if (!src0 || !w || !out || !tail_w || !tail_b || !hdr) return LGI_ERR_INVALID_ARG;
+    return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_conv3x3_fp8_fwd(const void* src0, int32_t c0, const void* src1, int32_t c1, const void* w_e4m3,
+                                    const float* w_scale, float act_scale, const float* bias, int32_t cout, int32_t n,
+                                    int32_t h, int32_t wd, int32_t relu, void* out, void* pool_out, void* stream) {
+    ConvArgs a{};
+    a.src0 = (const bf16*)src0; a.src1 = (const bf16*)src1; a.C0 = c0; a.C1 = src1 ? c1 : 0;
+    a.wk = (const bf16*)w_e4m3; a.w_scale = w_scale; a.act_scale = act_scale; a.fp8 = 1;
+    a.bias = bias; a.out0 = (bf16*)out; a.out1 = nullptr;
+    a.Cout = cout; a.Cs = cout; a.N = n; a.H = h; a.W = wd; a.relu = relu; a.pool_out = (bf16*)pool_out;
+    if (!src0 || !w_e4m3 || !w_scale || !out || !(act_scale > 0.0f) || (pool_out && (h % 2 || wd % 2)))
+        return LGI_ERR_INVALID_ARG;
     return conv_launch(a, 9, (hipStream_t)stream);
 }
 

@@ -119,6 +119,8 @@ SIGNATURES = {
                                             C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),
     "lgi_unet_conv3x3_tail_radiance_fwd": (C.c_int, [_P, C.c_int32, _P, _P, C.c_int32, C.c_int32, C.c_int32,
                                                      C.c_int32, _P, _P, _P, _P, _P]),
+    "lgi_unet_conv3x3_fp8_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, C.c_float, _P, C.c_int32, C.c_int32,
+                                           C.c_int32, C.c_int32, C.c_int32, _P, _P, _P]),
     "lgi_unet_conv3x3_first_radiance_fwd": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,
                                                       C.c_int32, _P, _P]),
     "lgi_unet_tail_radiance_fwd": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P]),

@@ -47,7 +47,8 @@ CONV_VARIANTS = {0: "8x16 px, 32/64 ch", 1: "16x16 px, 64 ch, rb", 2: "32x16 px,
                  3: "16x16 px, 32 ch, rb", 4: "32x16 px, 64 ch, rb", 5: "8x32 px, 64 ch, 32x32 MFMA",
                  6: "8x32 px, 32 ch, 32x32 MFMA", 7: "16x32 px, 32 ch, 32x32 MFMA",
                  8: "16x32 px, 64 ch, direct LDS loads", 9: "8x32 px, 64 ch, direct LDS loads",
-                 10: "16x32 px, 32 ch, direct LDS loads"}
+                 10: "16x32 px, 32 ch, direct LDS loads", 11: "8x32 px, 64 ch, FP8 operands",
+                 12: "4x32 px, 64 ch, FP8 operands", 13: "8x32 px, 32 ch, FP8 operands"}
 
 
 def conv_tuning(enabled=True):
@@ -102,6 +103,41 @@ def conv3x3_pool_fwd(src0, src1, w_bf16, bias, relu):
     _call("lgi_unet_conv3x3_pool_fwd", _p(src0), c0, _p(src1), c1, _p(w_bf16), _p(bias), cout, n, h, w, int(relu),
           _p(out), _p(pooled), _stream())
     return out, pooled
+
+
+E4M3_MAX = 448.0
+
+
+def quantize_weight_e4m3(w):
+    """FP32 weights [cout, 9, cin] -> (E4M3 codes as uint8 [cout, 9, cin], FP32 scale [cout]) with
+    w ~= code * scale[co]: each output channel's largest magnitude maps to 448, rounding to nearest
+    even (torch's float8_e4m3fn cast)."""
+    w = w.detach().float()
+    scale = (w.abs().amax(dim=(1, 2)) / E4M3_MAX).clamp_min(1e-30)
+    q = (w / scale[:, None, None]).clamp(-E4M3_MAX, E4M3_MAX).to(torch.float8_e4m3fn)
+    return q.view(torch.uint8).contiguous(), scale.contiguous()
+
+
+def quantize_act_e4m3(x, act_scale):
+    """The kernels' on-chip activation quantization, restated in torch: e4m3(clamp(x / s, +-448))."""
+    return (x.float() / act_scale).clamp(-E4M3_MAX, E4M3_MAX).to(torch.float8_e4m3fn)
+
+
+def conv3x3_fp8_fwd(src0, src1, wq, w_scale, act_scale, bias, relu, pool=False):
+    """conv3x3_fwd (or conv3x3_pool_fwd with pool=True) on FP8 operands: inputs quantized with the
+    per-tensor act_scale inside the kernel, wq/w_scale from quantize_weight_e4m3."""
+    n, h, w, c0 = src0.shape
+    c1 = src1.shape[-1] if src1 is not None else 0
+    cout = wq.shape[0]
+    _check(src0, torch.bfloat16)
+    if src1 is not None:
+        _check(src1, torch.bfloat16)
+    assert wq.dtype == torch.uint8 and wq.is_contiguous() and w_scale.dtype == torch.float32
+    out = torch.empty((n, h, w, cout), dtype=torch.bfloat16, device=src0.device)
+    pooled = torch.empty((n, h // 2, w // 2, cout), dtype=torch.bfloat16, device=src0.device) if pool else None
+    _call("lgi_unet_conv3x3_fp8_fwd", _p(src0), c0, _p(src1), c1, _p(wq), _p(w_scale), C.c_float(float(act_scale)),
+          _p(bias), cout, n, h, w, int(relu), _p(out), _p(pooled), _stream())
+    return (out, pooled) if pool else out
 
 
 def conv3x3_first_fwd(x, w_bf16, bias, relu=True, radiance=False):
@@ -300,6 +336,8 @@ class UNet(torch.nn.Module):
         self.tail_b = torch.nn.Parameter(torch.zeros(3))
         self.recompute = recompute
         self._wcache = {}  # name -> ((version, data_ptr), BF16 copy) for inference
+        self._fp8 = {}     # name -> (E4M3 weights, weight scales, activation scale) when FP8 is on
+        self._amax = None  # name -> largest input magnitude, while calibrating
 
     def _wb(self, name):
         """BF16 copy of a 3x3 weight, rebuilt only when the FP32 weight has changed."""
@@ -311,10 +349,52 @@ class UNet(torch.nn.Module):
             self._wcache[name] = hit
         return hit[1]
 
-    def conv(self, name, a, b=None, relu=True):
+    def conv(self, name, a, b=None, relu=True, pool=False):
+        """relu(conv3x3(cat(a, b))), and with pool=True also its 2x2 max-pool (inference only)."""
         if not torch.is_grad_enabled():
+            if self._amax is not None:
+                m = a.abs().amax() if b is None else torch.maximum(a.abs().amax(), b.abs().amax())
+                self._amax[name] = max(self._amax.get(name, 0.0), float(m))
+            if name in self._fp8:
+                wq, ws, sa = self._fp8[name]
+                return conv3x3_fp8_fwd(a, b, wq, ws, sa, self.biases[name], relu, pool=pool)
+            if pool:
+                return conv3x3_pool_fwd(a, b, self._wb(name), self.biases[name], relu)
             return conv3x3_fwd(a, b, self._wb(name), self.biases[name], relu)
+        assert not pool
         return Conv3x3Fn.apply(a, b, self.weights[name], self.biases[name], relu)
+
+    def fp8_layers(self):
+        """The convolutions that can run on FP8 operands: 64-channel input chunks of one source."""
+        out = []
+        for name, cin, cout in self.shapes:
+            if name in ("enc0a", "enc1a"):
+                continue  # 32-channel inputs
+            srcs = (cin - ENC[int(name[3])], ENC[int(name[3])]) if name.endswith("a") and name.startswith("dec") \
+                else (cin,)
+            if all(c % 64 == 0 for c in srcs):
+                out.append(name)
+        return out
+
+    @torch.no_grad()
+    def enable_fp8(self, calib_x, margin=4.0, layers=None, radiance=False):
+        """Switches the FP8-capable convolutions (fp8_layers(), or `layers`) to E4M3 operands: per
+        output channel weight scales, and per-tensor activation scales of margin x the largest input
+        magnitude a BF16 forward of calib_x produces (larger inputs saturate at 448 x scale)."""
+        self._fp8 = {}
+        self._amax = {}
+        try:
+            self.forward(calib_x, radiance=radiance)
+        finally:
+            amax, self._amax = self._amax, None
+        for name in (layers if layers is not None else self.fp8_layers()):
+            wq, ws = quantize_weight_e4m3(self.weights[name])
+            sa = max(amax[name], 1e-6) * margin / E4M3_MAX
+            self._fp8[name] = (wq.to(calib_x.device), ws.to(calib_x.device), sa)
+        return dict(self._fp8)
+
+    def disable_fp8(self):
+        self._fp8 = {}
 
     def block(self, prefix, a, b=None):
         y = self.conv(prefix + "a", a, b)
@@ -342,7 +422,7 @@ class UNet(torch.nn.Module):
                     y = conv3x3_first_fwd(x, self._wb("enc0a"), self.biases["enc0a"], radiance=radiance)
                 else:
                     y = self.conv(f"enc{i}a", h)
-                h, pooled = conv3x3_pool_fwd(y, None, self._wb(f"enc{i}b"), self.biases[f"enc{i}b"], True)
+                h, pooled = self.conv(f"enc{i}b", y, pool=True)
             else:
                 h = self._block(f"enc{i}", pack_input(x) if i == 0 else h)
                 pooled = None

@@ -209,3 +209,14 @@ def test_wavefront_pass_plan_bounds_shadow_candidates():
     assert e.value.status == 8
     with pytest.raises(lgi.LgiError):
         plan(0, 2, 1)
+
+
+def test_conv_variant_table_matches_the_header():
+    """lgi_unet_conv_tune_table writes LGI_UNET_CONV_VARIANTS timings per row into the caller's
+    array; the Python front end sizes that array from its variant table."""
+    import re
+    from lgi import unet
+    hdr = (PKG.parent / "include" / "lgi.h").read_text()
+    n = int(re.search(r"#define LGI_UNET_CONV_VARIANTS (\d+)", hdr).group(1))
+    assert len(unet.CONV_VARIANTS) == n and sorted(unet.CONV_VARIANTS) == list(range(n))
+

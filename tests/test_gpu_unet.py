@@ -439,3 +439,50 @@ def test_conv3x3_tail_equals_conv_then_tail(gpu, variant, shape):
         unet.force_conv_variant(None)
     ref = unet.tail_radiance_fwd(y, hdr, tw, tb)
     assert out.shape == (n, h, w, 3) and torch.equal(out, ref)
+
+
+FP8_CASES = [  # n, h, w, c0, c1, cout, pool
+    (1, 16, 64, 64, 0, 64, False),
+    (2, 12, 40, 128, 64, 64, True),     # concat of two sources, ragged tiles, pooled copy
+    (1, 9, 33, 64, 0, 96, False),       # Cout not a multiple of 64
+    (1, 8, 32, 256, 256, 128, True),
+]
+
+
+@pytest.mark.parametrize("variant", [None, 11, 12, 13], ids=lambda v: "tuned" if v is None else f"variant{v}")
+@pytest.mark.parametrize("case", FP8_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_fp8(gpu, variant, case):
+    """FP8 (E4M3) convolution. Against the FP64 convolution of the same quantized operands (torch's
+    float8_e4m3fn rounding of x / act_scale and of the per-channel scaled weights): only the FP32
+    sums and the BF16 output round, so the BF16 bar holds. Against the unquantized FP64 reference:
+    the 3-bit E4M3 mantissa bounds the error. The pooled copy is MaxPool2d(2) of the output."""
+    from lgi import unet
+    n, h, w, c0, c1, cout, pool = case
+    if variant in (11, 12) and cout % 64:
+        pytest.skip("64-channel tiling needs cout % 64 == 0")
+    g = torch.Generator().manual_seed(61)
+    x0 = (torch.rand(n, h, w, c0, generator=g) * 3.0).to(torch.bfloat16)
+    x1 = (torch.rand(n, h, w, c1, generator=g) * 2.0).to(torch.bfloat16) if c1 else None
+    wt = torch.randn(cout, 9, c0 + c1, generator=g) * (2.0 / (9 * (c0 + c1))) ** 0.5
+    b = torch.randn(cout, generator=g) * 0.1
+    sa = 3.0 * 2 / unet.E4M3_MAX
+    wq, ws = unet.quantize_weight_e4m3(wt)
+    unet.conv_tuning(True)
+    unet.force_conv_variant(variant)
+    try:
+        r = unet.conv3x3_fp8_fwd(x0.cuda(), None if x1 is None else x1.cuda(), wq.cuda(), ws.cuda(), sa, b.cuda(), True,
+                                 pool=pool)
+    finally:
+        unet.force_conv_variant(None)
+    y, p = r if pool else (r, None)
+    # FP64 of the quantized operands
+    xq0 = unet.quantize_act_e4m3(x0, sa).double() * sa
+    xq1 = unet.quantize_act_e4m3(x1, sa).double() * sa if c1 else None
+    wdq = wq.view(torch.float8_e4m3fn).double() * ws.double()[:, None, None]
+    ref_q = ref_conv_nhwc(xq0, xq1, wdq, b, True)
+    ref = ref_conv_nhwc(x0, x1, wt, b, True)
+    eq, e = rel(y, ref_q), rel(y, ref)
+    print(f"fp8 conv {case} variant {variant}: vs quantized operands {eq:.2e}, vs FP64 {e:.2e}", flush=True)
+    assert y.shape == ref.shape and eq < BF16_TOL and e < 8e-2
+    if pool:
+        assert torch.equal(p, unet.MaxPoolFn.apply(y))
