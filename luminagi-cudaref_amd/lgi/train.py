@@ -192,11 +192,20 @@ def pick_renderer(ctx, cam, bounces=2, spp=8):
     return min(times, key=times.get), times
 
 
-def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None, log=print):
+def fp8_calibration_frame(ctx, spp=8, bounces=2):
+    """Radiance (1, H, W, 3) of a training pose at 8 spp: the frame the FP8 activation scales are
+    calibrated on (the first orbit pose of the training set, not the evaluated view)."""
+    (pos, tgt, _), = orbit_poses(1, seed=0)
+    return render_mean(ctx, camera_for(ctx.camera(), pos, tgt), spp, 1 << 26, bounces).unsqueeze(0).contiguous()
+
+
+def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None, fp8=True, log=print):
     """PSNR/SSIM (8-bit sqrt-tonemapped domain, as tools/diff.py) of plain renders at growing spp
     and of 8 spp + UNet against a ref_spp plain render; times to 40 dB. flags selects the
     renderer used for the timed renders (None: the faster one on this scene, pick_renderer;
-    both give the same bits)."""
+    both give the same bits). fp8=True also measures the UNet on FP8 operands (activation scales
+    calibrated on a training pose, fp8_calibration_frame). Every method that reaches 40 dB is
+    timed; "unet_ms" is the fastest of them."""
     gt = D.tonemap(ctx, render_mean(ctx, cam, ref_spp, 0, bounces, batch=256))
     test0 = 1 << 24  # sample streams disjoint from the reference's
     res = {"ref_spp": ref_spp, "plain": []}
@@ -239,6 +248,28 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None
     res["noisy8"] = {"psnr": D.psnr_from_mse(mse8), "ssim": ssim8}
     log(f"8 spp raw: {D.psnr_from_mse(mse8):.2f} dB; 8 spp + UNet: {res['unet']['psnr']:.2f} dB ssim {ssim:.4f} "
         f"({t_render:.1f} + {t_den_g:.1f} ms; eager denoise {t_den:.1f} ms; one-graph frame {t_frame:.1f} ms)")
+    if fp8:
+        # the same 8-spp frame through the UNet on FP8 operands (before the fallback below adds
+        # samples to some of its tiles)
+        model.enable_fp8(fp8_calibration_frame(ctx, bounces=bounces), radiance=True)
+        try:
+            D.denoise(model, fr.mean())
+            den8, t8 = timed(lambda: D.denoise(model, fr.mean()))
+            g8 = D.DenoiseGraph(model, cam.height, cam.width)
+            g8(fr.mean())
+            den8_g, t8_g = timed(lambda: g8(fr.mean()))
+            fg8 = D.FrameGraph(ctx, model, cam, spp=8, bounces=bounces, sample0=test0 + (1 << 20), flags=flags)
+            fg8.replay()
+            (hdr8, _), t_frame8 = timed(fg8.replay)
+            assert torch.equal(den8_g, den8) and torch.equal(hdr8, den8), "FP8 graphs differ from the eager forward"
+        finally:
+            model.disable_fp8()
+        mse, ssim, _ = D.image_metrics(D.tonemap(ctx, den8), gt)
+        res["unet_fp8"] = {"spp": 8, "render_ms": t_render, "denoise_ms": t8_g, "denoise_eager_ms": t8,
+                           "frame_graph_ms": t_frame8, "psnr": D.psnr_from_mse(mse), "ssim": ssim,
+                           "ms": min(t_render + t8_g, t_frame8)}
+        log(f"8 spp + UNet (FP8): {res['unet_fp8']['psnr']:.2f} dB ssim {ssim:.4f} "
+            f"({t_render:.1f} + {t8_g:.2f} ms; one-graph frame {t_frame8:.2f} ms)")
     if fallback:
         warm = D.Frame(cam)  # warm-up of the tile path on a throw-away frame
         warm.render(ctx, 8, bounces, sample0=test0 + (1 << 22), flags=flags)
@@ -251,21 +282,39 @@ def evaluate(ctx, model, cam, ref_spp=4096, bounces=2, fallback=True, flags=None
     spp40 = plain_spp_for(res["plain"], 40.0)
     res["plain_spp_40db"] = spp40
     res["plain_ms_40db"] = None if spp40 is None else spp40 * ms_per_spp
-    # the cheapest UNet variant that reaches 40 dB (plain UNet first, then with the fallback)
     res["unet"]["ms"] = min(res["unet"]["render_ms"] + res["unet"]["denoise_ms"], res["unet"]["frame_graph_ms"])
-    cands = [(res[k]["ms"], k) for k in ("unet", "unet_fallback") if k in res and res[k]["psnr"] >= 40.0]
+    # every method: its time, the plain tracer's time to its PSNR, and both speedups
+    methods = {}
+    for k in ("unet", "unet_fp8", "unet_fallback"):
+        if k not in res:
+            continue
+        m = {"ms": res[k]["ms"], "psnr": res[k]["psnr"], "reaches_40db": res[k]["psnr"] >= 40.0}
+        if m["reaches_40db"] and res["plain_ms_40db"]:
+            m["speedup_40db"] = res["plain_ms_40db"] / m["ms"]
+        spp_eq = plain_spp_for(res["plain"], m["psnr"])
+        if spp_eq is not None:
+            m["plain_spp_same_psnr"] = spp_eq
+            m["plain_ms_same_psnr"] = spp_eq * ms_per_spp
+            m["speedup_same_psnr"] = m["plain_ms_same_psnr"] / m["ms"]
+        methods[k] = m
+    res["methods"] = methods
+    cands = [(m["ms"], k) for k, m in methods.items() if m["reaches_40db"]]
     res["unet_reaches_40db"] = bool(cands)
     if cands:
         res["unet_ms"], res["unet_variant_40db"] = min(cands)
         if res["plain_ms_40db"]:
             res["speedup_40db"] = res["plain_ms_40db"] / res["unet_ms"]
-    # plain time to the PSNR that 8 spp + UNet reaches
-    spp_eq = plain_spp_for(res["plain"], res["unet"]["psnr"])
-    if spp_eq is not None:
-        res["plain_spp_same_psnr"] = spp_eq
-        res["plain_ms_same_psnr"] = spp_eq * ms_per_spp
-        res["speedup_same_psnr"] = res["plain_ms_same_psnr"] / res["unet"]["ms"]
+    # plain time to the PSNR that the BF16 8 spp + UNet frame reaches
+    if "plain_spp_same_psnr" in methods["unet"]:
+        for k in ("plain_spp_same_psnr", "plain_ms_same_psnr", "speedup_same_psnr"):
+            res[k] = methods["unet"][k]
     return res
+
+
+def off_orbit_camera(ctx):
+    """A view the training set has no neighbour of: 6 m from the scene centre at 2.5 m height,
+    looking across it (the training poses stand on a 15 m orbit, 3-12 m high)."""
+    return camera_for(ctx.camera(), (-4.2, 4.3, 2.5), (3.0, -2.0, 3.0))
 
 
 def held_out_camera(ctx):
