@@ -187,10 +187,23 @@ def unet_workload(ckpt, reps=20):
     eager = best(lambda: D.denoise(model, x))
     graph = best(lambda: g(x))
     assert torch.equal(g(x), D.denoise(model, x)), "graph replay differs from the eager forward"
-    return {"workload": "UNet 32/64/128/256 + 512 bottleneck, one 1728x864 frame, BF16 MFMA convolutions",
-            "checkpoint": os.path.relpath(ckpt, ROOT), "gflop": UNET_GFLOP,
-            "eager_ms": round(eager, 3), "graph_ms": round(graph, 3),
-            "tflops_graph": round(UNET_GFLOP / graph, 1), "frac_of_bf16_peak": round(UNET_GFLOP / graph / BF16_PEAK_TFLOPS, 4)}
+    out = {"workload": "UNet 32/64/128/256 + 512 bottleneck, one 1728x864 frame, BF16 MFMA convolutions",
+           "checkpoint": os.path.relpath(ckpt, ROOT), "gflop": UNET_GFLOP,
+           "eager_ms": round(eager, 3), "graph_ms": round(graph, 3),
+           "tflops_graph": round(UNET_GFLOP / graph, 1),
+           "frac_of_bf16_peak": round(UNET_GFLOP / graph / BF16_PEAK_TFLOPS, 4)}
+    # the same frame with the 13 FP8-capable convolutions on E4M3 operands (activation scales
+    # calibrated on a training pose of the 16-mesh scene)
+    with torch.no_grad():
+        model.enable_fp8(T.fp8_calibration_frame(T.make_scene(1, 1, 1728, 864)), radiance=True)
+    for _ in range(3):
+        D.denoise(model, x)
+    g8 = D.DenoiseGraph(model, 864, 1728)
+    eager8 = best(lambda: D.denoise(model, x))
+    graph8 = best(lambda: g8(x))
+    out["fp8"] = {"eager_ms": round(eager8, 3), "graph_ms": round(graph8, 3), "tflops_graph": round(UNET_GFLOP / graph8, 1),
+                  "frac_of_bf16_peak": round(UNET_GFLOP / graph8 / BF16_PEAK_TFLOPS, 4)}
+    return out
 
 
 def _summary(r):
