@@ -424,7 +424,20 @@ def main():
                                   "sample": "CPU oracle, same scene and PCG32 streams, %d spp on every %dth row "
                                             "(%d samples, %.1f s), %s" % (args.cpu_spp, args.cpu_row_step, cpu_samples,
                                                                           cpu_s, cpu_model()),
-                                  "bvh_build_ms_1thread": round(bvh_s * 1e3, 1)}
+                                  "bvh_build_ms_1thread": round(bvh_s * 1e3, 1),
+                                  "gpu_lbvh_build_ms": result["lbvh_build_ms"]}
+        from oracle import oracle as orc
+        if orc.ref_bvh_available():
+            # the reference's own BuildBVH (src/bvh.cpp, compiled from its sources into
+            # oracle/_ref) on the same triangles, one thread; its node array against the GPU's
+            import tempfile
+            with tempfile.TemporaryDirectory() as td:
+                ctx.save_bin(pathlib.Path(td) / "scene.lgis")
+                ref_nodes, _, ref_ms = orc.ref_bvh(pathlib.Path(td) / "scene.lgis", pathlib.Path(td) / "ref.bin",
+                                                   timed=True)
+            result["cpu_baseline"]["bvh_build_ms_reference"] = round(ref_ms, 1)
+            result["cpu_baseline"]["reference_bvh_nodes_equal_gpu"] = bool(ref_nodes.tobytes() ==
+                                                                           ctx.bvh()[0].tobytes())
         result["oracle_counters_per_sample"] = {k: round(v / cpu_samples, 2) for k, v in counters.items()}
         result["oracle_counters_per_sample_reference_tree"] = {k: round(v / cpu_samples, 2) for k, v in c_ref.items()}
         # rays per sample (closest-hit + shadow) from the oracle's deterministic counts

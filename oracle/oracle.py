@@ -146,13 +146,17 @@ def ref_bvh_available():
     return REF_BVH.exists()
 
 
-def ref_bvh(scene_json, tmp_bin):
-    """Reference BuildBVH compiled from the reference sources: (nodes, perm)."""
-    subprocess.run([str(REF_BVH), str(scene_json), str(tmp_bin)], check=True, stdout=subprocess.DEVNULL)
+def ref_bvh(scene_path, tmp_bin, timed=False):
+    """Reference BuildBVH compiled from the reference sources, on a scene.json or a binary .lgis
+    scene file: (nodes, perm), and with timed=True also the milliseconds of the BuildBVH call."""
+    r = subprocess.run([str(REF_BVH), str(scene_path), str(tmp_bin)], check=True, capture_output=True, text=True)
     raw = pathlib.Path(tmp_bin).read_bytes()
     nt, nn = np.frombuffer(raw[:16], np.int64)
     nodes = np.frombuffer(raw[16:16 + 40 * nn], NODE_DTYPE).copy()
     perm = np.frombuffer(raw[16 + 40 * nn:16 + 40 * nn + 4 * nt], np.int32).copy()
+    if timed:
+        ms = float(r.stdout.split("BuildBVH")[1].split("ms")[0])
+        return nodes, perm, ms
     return nodes, perm
 
 
