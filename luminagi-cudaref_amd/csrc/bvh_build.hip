@@ -261,14 +261,20 @@ __global__ __launch_bounds__(kBlock) void k_morton(const lgi_triangle* tris, int
     vals[i] = i;
 }
 
-// Sorted vertex array (9 floats per triangle) in Morton order.
+// Sorted vertex array (9 floats per triangle) in Morton order. Each thread gathers one
+// triangle's vertices into LDS; the block then writes its 256 x 9 floats as contiguous rows.
 __global__ __launch_bounds__(kBlock) void k_gather_verts(const lgi_triangle* tris, const int32_t* perm, int n,
                                                          float* sv) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const lgi_triangle& t = tris[perm[i]];
-    for (int c = 0; c < 3; ++c)
-        for (int k = 0; k < 3; ++k) sv[(size_t)i * 9 + c * 3 + k] = t.v[c][k];
+    __shared__ float s_v[kBlock * 9];
+    const int i0 = blockIdx.x * kBlock, i = i0 + threadIdx.x;
+    if (i < n) {
+        const lgi_triangle& t = tris[perm[i]];
+        for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 3; ++k) s_v[threadIdx.x * 9 + c * 3 + k] = t.v[c][k];
+    }
+    __syncthreads();
+    const int count = min(kBlock, n - i0) * 9;
+    for (int j = threadIdx.x; j < count; j += kBlock) sv[(size_t)i0 * 9 + j] = s_v[j];
 }
 
 // ------------------------------------------------------------------ Karras tree
@@ -418,9 +424,14 @@ __global__ __launch_bounds__(kBlock) void k_level_starts(const uint32_t* bfs_key
 // children's, so a ray enters a leaf's ancestors whenever it enters the leaf, and the closest
 // hit is the same as on the reference tree except where visit order decides it: exact
 // distance ties, and a hit inside a box the slab test rounds to missed.
-// Treelet roots are processed one BFS level of the reference tree per launch, deepest first:
-// a root's treelet lies inside its own subtree, so the treelets of one launch are disjoint
-// and each launch sees its subtrees already restructured. One wave per treelet.
+// Every internal node of the tree as it stands at the start of a pass is a treelet root once in
+// that pass, after all of its descendants: a root's treelet lies inside its own subtree (whose
+// node set restructuring below it never changes), so roots in disjoint subtrees commute and
+// every children-first order gives the same tree. The passes here take the levels of the tree,
+// deepest first, one launch per level and one wave per treelet. (A single bottom-up launch per
+// pass, where the second child to arrive at a node continues upwards, gives the same tree but
+// ran 4x slower: every step up needs an agent-scope release and acquire, which write back and
+// invalidate the L2 of the XCD.)
 constexpr float kSahNode = 1.2f, kSahTri = 1.0f;
 constexpr int kTreeletLeaves = 7;
 constexpr int kSubsets = 1 << kTreeletLeaves;
@@ -430,123 +441,264 @@ __device__ __forceinline__ float half_area(const float* mn, const float* mx) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-// Subtree cost of every node of BFS level [p0, p1), children first (the level below ran before).
-__global__ __launch_bounds__(kBlock) void k_sah_level(const lgi_bvh_node* nodes, int p0, int p1, float* cost) {
-    const int p = p0 + blockIdx.x * kBlock + threadIdx.x;
-    if (p >= p1) return;
-    const lgi_bvh_node nd = nodes[p];
-    const float a = half_area(nd.bb_min, nd.bb_max);
-    cost[p] = nd.tri_count > 0 ? a * kSahTri * (float)nd.tri_count : a * kSahNode + cost[nd.left] + cost[nd.right];
+// A treelet as its root's lane formed it: the internal slots (in[0] = the root) and the
+// leaves, each leaf's box and subtree cost, and the root's refreshed cost. 65 words, so that
+// the 64 lanes' records start in different LDS banks.
+struct TreeletForm {
+    int leaf[kTreeletLeaves], in[kTreeletLeaves - 1], n;
+    float root_cost;
+    float mn[kTreeletLeaves][3], mx[kTreeletLeaves][3], c[kTreeletLeaves];
+    int pad;
+};
+// Working state while a treelet is formed: the root's subtree to depth 3, prefetched by the
+// wave in heap order (slot 1 = the root, slots 2h and 2h+1 = the children of slot h; valid only
+// below an internal node), and per treelet leaf its children, area, whether it is internal,
+// its subtree cost and its heap slot (0: outside the prefetched levels).
+constexpr int kHeapSlots = 16;
+struct TreeletGrow {
+    lgi_bvh_node node[kHeapSlots];
+    float cost[kHeapSlots];
+    int kid[kTreeletLeaves][2];
+    float area[kTreeletLeaves];
+    int open[kTreeletLeaves];
+    int heap[kTreeletLeaves];
+};
+// Dynamic programming over the leaf subsets of the treelet being optimized.
+struct TreeletDp {
+    float mn[kSubsets][3], mx[kSubsets][3], a[kSubsets], c[kSubsets];
+    uint8_t part[kSubsets];
+};
+
+// The subsets of 7 treelet leaves ordered by size: sub[start[s] + i] is the i-th of size s.
+struct SubsetsBySize {
+    uint8_t sub[kSubsets];
+    uint8_t start[kTreeletLeaves + 2];
+};
+constexpr SubsetsBySize subsets_by_size() {
+    SubsetsBySize t{};
+    int k = 0;
+    for (int s = 0; s <= kTreeletLeaves; ++s) {
+        t.start[s] = (uint8_t)k;
+        for (int v = 0; v < kSubsets; ++v) {
+            int c = 0;
+            for (int b = v; b; b &= b - 1) ++c;
+            if (c == s) t.sub[k++] = (uint8_t)v;
+        }
+    }
+    t.start[kTreeletLeaves + 1] = (uint8_t)k;
+    return t;
+}
+__constant__ SubsetsBySize kBySize = subsets_by_size();
+
+// Records node x (loaded as c, subtree cost cx, heap slot h or 0) as treelet leaf `at`.
+__device__ __forceinline__ void treelet_leaf(TreeletForm& F, TreeletGrow& G, int at, int x, const lgi_bvh_node& c,
+                                             float cx, int h) {
+    F.leaf[at] = x;
+    for (int k = 0; k < 3; ++k) {
+        F.mn[at][k] = c.bb_min[k];
+        F.mx[at][k] = c.bb_max[k];
+    }
+    F.c[at] = cx;
+    G.open[at] = c.tri_count == 0;
+    G.area[at] = half_area(c.bb_min, c.bb_max);
+    G.kid[at][0] = c.left;
+    G.kid[at][1] = c.right;
+    G.heap[at] = h;
 }
 
-// Restructures the treelet rooted at order[p0 + blockIdx.x] (p0 + blockIdx.x when order is
-// null); the subtree cost is kept current for every node it rewires.
-__global__ __launch_bounds__(64) void k_treelet(lgi_bvh_node* nodes, const int32_t* order, int p0, float* cost) {
-    const int p = order ? order[p0 + blockIdx.x] : p0 + blockIdx.x;  // one treelet root per 64-thread block
-    if (nodes[p].tri_count > 0) return;
-    __shared__ int s_leaf[kTreeletLeaves], s_int[kTreeletLeaves - 1], s_n;
-    __shared__ float s_mn[kSubsets][3], s_mx[kSubsets][3], s_c[kSubsets];
-    __shared__ uint8_t s_part[kSubsets];
+// Forms the treelet of internal node p (the subtrees below it are final for this pass); all
+// lanes of the wave call it. The wave first loads p's subtree to depth 3 with the subtree
+// costs, one level per round, so that most treelets form from LDS. Lane 0 then refreshes p's
+// cost and opens the internal treelet leaf of largest area (the first on ties) until there
+// are 7 leaves, loading children from memory only below the prefetched levels.
+__device__ void treelet_form(const lgi_bvh_node* nodes, float* cost, int p, TreeletForm& F, TreeletGrow& G) {
     const int lane = threadIdx.x;
-    if (lane == 0) {
-        // the subtrees below were restructured already: refresh this node's cost first
-        const lgi_bvh_node nd = nodes[p];
-        cost[p] = half_area(nd.bb_min, nd.bb_max) * kSahNode + cost[nd.left] + cost[nd.right];
-        int nl = 2, ni = 1;
-        s_leaf[0] = nd.left;
-        s_leaf[1] = nd.right;
-        s_int[0] = p;
-        while (nl < kTreeletLeaves) {
-            int best = -1;
-            float ba = -1.0f;
-            for (int i = 0; i < nl; ++i) {
-                const lgi_bvh_node c = nodes[s_leaf[i]];
-                if (c.tri_count > 0) continue;
-                const float a = half_area(c.bb_min, c.bb_max);
-                if (a > ba) { ba = a; best = i; }
-            }
-            if (best < 0) break;
-            const int x = s_leaf[best];
-            s_int[ni++] = x;
-            s_leaf[best] = nodes[x].left;
-            s_leaf[nl++] = nodes[x].right;
-        }
-        s_n = nl;
-    }
+    if (lane == 0) G.node[1] = nodes[p];
     __syncthreads();
-    const int nl = s_n;
-    if (nl < 3) return;  // two leaves have one tree (uniform: s_n is shared)
-    const int ns = 1 << nl;
-    if (lane < nl) {
-        const lgi_bvh_node c = nodes[s_leaf[lane]];
-        const int s1 = 1 << lane;
-        for (int k = 0; k < 3; ++k) {
-            s_mn[s1][k] = c.bb_min[k];
-            s_mx[s1][k] = c.bb_max[k];
-        }
-        s_c[s1] = cost[s_leaf[lane]];
-    }
-    __syncthreads();
-    for (int sub = lane; sub < ns; sub += 64) {  // boxes of every subset of two or more leaves
-        if (__popc(sub) < 2) continue;
-        float mn[3], mx[3];
-        const int first = __ffs(sub) - 1;
-        for (int k = 0; k < 3; ++k) { mn[k] = s_mn[1 << first][k]; mx[k] = s_mx[1 << first][k]; }
-        for (int b = sub & (sub - 1); b; b &= b - 1) {
-            const int i = __ffs(b) - 1;
-            for (int k = 0; k < 3; ++k) {
-                mn[k] = fminf(mn[k], s_mn[1 << i][k]);
-                mx[k] = fmaxf(mx[k], s_mx[1 << i][k]);
+    for (int lv = 1; lv <= 3; ++lv) {
+        const int h = (1 << lv) + lane, up = h >> 1;
+        if (lane < (1 << lv)) {
+            if (G.node[up].tri_count == 0) {
+                const int x = (h & 1) ? G.node[up].right : G.node[up].left;
+                G.node[h] = nodes[x];
+                G.cost[h] = cost[x];
+            } else {
+                G.node[h].tri_count = 1;  // below a leaf: no node (read as a leaf)
             }
-        }
-        for (int k = 0; k < 3; ++k) { s_mn[sub][k] = mn[k]; s_mx[sub][k] = mx[k]; }
-    }
-    __syncthreads();
-    for (int size = 2; size <= nl; ++size) {
-        for (int sub = lane; sub < ns; sub += 64) {
-            if (__popc(sub) != size) continue;
-            const int low = sub & -sub;
-            float best = 3.4e38f;
-            int bp = low;
-            for (int q = (sub - 1) & sub; q > 0; q = (q - 1) & sub) {  // splits with `low` on the left
-                if (!(q & low)) continue;
-                const float v = s_c[q] + s_c[sub ^ q];
-                if (v < best) { best = v; bp = q; }
-            }
-            s_c[sub] = half_area(s_mn[sub], s_mx[sub]) * kSahNode + best;
-            s_part[sub] = (uint8_t)bp;
         }
         __syncthreads();
     }
-    if (lane != 0 || !(s_c[ns - 1] < cost[p] * (1.0f - 1e-6f))) return;
-    // rewire: internal slot s_int[0] = p takes the full set, the others take the inner splits
-    int st_sub[kTreeletLeaves], st_node[kTreeletLeaves], sp = 0, next = 1;
-    st_sub[sp] = ns - 1;
-    st_node[sp++] = p;
-    while (sp > 0) {
-        --sp;
-        const int sub = st_sub[sp], idx = st_node[sp];
-        const int l = s_part[sub], r = sub ^ l;
-        int ch[2];
-        const int parts[2] = {l, r};
-        for (int k = 0; k < 2; ++k) {
-            if (__popc(parts[k]) == 1) {
-                ch[k] = s_leaf[__ffs(parts[k]) - 1];
-            } else {
-                ch[k] = s_int[next++];
-                st_sub[sp] = parts[k];
-                st_node[sp++] = ch[k];
+    if (lane != 0) return;
+    const lgi_bvh_node& nd = G.node[1];
+    F.root_cost = half_area(nd.bb_min, nd.bb_max) * kSahNode + G.cost[2] + G.cost[3];
+    cost[p] = F.root_cost;
+    int nl = 2, ni = 1;
+    treelet_leaf(F, G, 0, nd.left, G.node[2], G.cost[2], 2);
+    treelet_leaf(F, G, 1, nd.right, G.node[3], G.cost[3], 3);
+    F.in[0] = p;
+    while (nl < kTreeletLeaves) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int i = 0; i < nl; ++i)
+            if (G.open[i] && G.area[i] > ba) { ba = G.area[i]; best = i; }
+        if (best < 0) break;
+        const int x = F.leaf[best], l = G.kid[best][0], r = G.kid[best][1], h = G.heap[best];
+        F.in[ni++] = x;
+        if (h > 0 && 2 * h + 1 < kHeapSlots) {
+            const lgi_bvh_node cl = G.node[2 * h], cr = G.node[2 * h + 1];
+            const float kl = G.cost[2 * h], kr = G.cost[2 * h + 1];
+            treelet_leaf(F, G, best, l, cl, kl, 2 * h);
+            treelet_leaf(F, G, nl++, r, cr, kr, 2 * h + 1);
+        } else {
+            const lgi_bvh_node cl = nodes[l], cr = nodes[r];
+            const float kl = cost[l], kr = cost[r];
+            treelet_leaf(F, G, best, l, cl, kl, 0);
+            treelet_leaf(F, G, nl++, r, cr, kr, 0);
+        }
+    }
+    F.n = nl;
+}
+
+// All 64 lanes find the least-cost topology of the formed treelet F and, when it is cheaper
+// than the current one, lane 0 rewires F's internal slots to it.
+__device__ void treelet_optimize(lgi_bvh_node* nodes, float* cost, const TreeletForm& F, TreeletDp& S,
+                                 const SubsetsBySize& by_size) {
+    const int lane = threadIdx.x;
+    const int nl = F.n, ns = 1 << nl;
+    if (lane < nl) S.c[1 << lane] = F.c[lane];
+    {
+        // boxes and areas of every subset of two or more leaves, from the leaf boxes held in
+        // registers; the union runs from the lowest leaf up
+        float lmn[kTreeletLeaves][3], lmx[kTreeletLeaves][3];
+        for (int i = 0; i < kTreeletLeaves; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lmn[i][k] = F.mn[i][k];
+                lmx[i][k] = F.mx[i][k];
+            }
+        for (int sub = lane; sub < ns; sub += 64) {
+            if (__popc(sub) < 2) continue;
+            float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
+            const int first = __ffs(sub) - 1;
+#pragma unroll
+            for (int i = 0; i < kTreeletLeaves; ++i) {
+                if (!((sub >> i) & 1)) continue;
+                for (int k = 0; k < 3; ++k) {
+                    mn[k] = i == first ? lmn[i][k] : fminf(mn[k], lmn[i][k]);
+                    mx[k] = i == first ? lmx[i][k] : fmaxf(mx[k], lmx[i][k]);
+                }
+            }
+            for (int k = 0; k < 3; ++k) { S.mn[sub][k] = mn[k]; S.mx[sub][k] = mx[k]; }
+            S.a[sub] = half_area(mn, mx);
+        }
+    }
+    __syncthreads();
+    // Least cost of every subset, by size. A subset's candidate splits put its lowest leaf on
+    // the left; larger subsets spread their splits over a group of lanes (1, 2, 8 or 64) and
+    // reduce. The winner is the least sum below 3.4e38, the largest left set on ties, else the
+    // lowest leaf alone; which lane evaluated a split does not change it.
+    for (int size = 2; size <= nl; ++size) {
+        const int la = size <= 4 ? 0 : size == 5 ? 1 : size == 6 ? 3 : 6;  // log2 of the lanes per subset
+        const int g = 1 << la, which = lane >> la, part = lane & (g - 1);
+        const int k0 = by_size.start[size], count = by_size.start[size + 1] - k0;
+        const int sub = which < count ? by_size.sub[k0 + which] : 0;
+        const bool mine = which < count && sub < ns;  // all of its leaves exist
+        const int low = sub & -sub, rest = sub ^ low;
+        // left = low + any subset of rest but rest itself: the la lowest leaves of rest are
+        // fixed by the lane (the bits of part), the others are enumerated
+        int lo = 0, base = 0;
+        for (int t = 0, b = rest; t < la; ++t, b &= b - 1) {
+            lo |= b & -b;
+            if ((part >> t) & 1) base |= b & -b;
+        }
+        const int hi = rest ^ lo, trips = 1 << (size - 1 - la);
+        float best = 3.4e38f;
+        int bp = low;
+        if (mine) {
+            for (int it = 0, r = 0; it < trips; it += 2) {  // two splits per round
+                const int r1 = (r - hi) & hi;
+                const int q0 = low | base | r, q1 = low | base | r1;
+                const float v0 = S.c[q0] + S.c[sub ^ q0];
+                const float v1 = it + 1 < trips ? S.c[q1] + S.c[sub ^ q1] : 3.4e38f;
+                if (q0 != sub && (v0 < best || (v0 == best && v0 < 3.4e38f && q0 > bp))) { best = v0; bp = q0; }
+                if (it + 1 < trips && q1 != sub && (v1 < best || (v1 == best && v1 < 3.4e38f && q1 > bp))) {
+                    best = v1;
+                    bp = q1;
+                }
+                r = (r1 - hi) & hi;
             }
         }
-        lgi_bvh_node& nd = nodes[idx];
-        nd.left = ch[0];
-        nd.right = ch[1];
-        for (int k = 0; k < 3; ++k) {
-            nd.bb_min[k] = s_mn[sub][k];
-            nd.bb_max[k] = s_mx[sub][k];
+        for (int off = 1; off < g; off <<= 1) {
+            const float ov = __shfl_xor(best, off, 64);
+            const int oq = __shfl_xor(bp, off, 64);
+            if (ov < best || (ov == best && ov < 3.4e38f && oq > bp)) { best = ov; bp = oq; }
         }
-        cost[idx] = s_c[sub];
+        if (mine && part == 0) {
+            S.c[sub] = S.a[sub] * kSahNode + best;
+            S.part[sub] = (uint8_t)bp;
+        }
+        __syncthreads();
     }
+    if (lane == 0 && S.c[ns - 1] < F.root_cost * (1.0f - 1e-6f)) {
+        // rewire: internal slot in[0] (the root) takes the full set, the others the inner splits
+        int st_sub[kTreeletLeaves], st_node[kTreeletLeaves], sp = 0, next = 1;
+        st_sub[sp] = ns - 1;
+        st_node[sp++] = F.in[0];
+        while (sp > 0) {
+            --sp;
+            const int sub = st_sub[sp], idx = st_node[sp];
+            const int l = S.part[sub], r = sub ^ l;
+            int ch[2];
+            const int parts[2] = {l, r};
+            for (int k = 0; k < 2; ++k) {
+                if (__popc(parts[k]) == 1) {
+                    ch[k] = F.leaf[__ffs(parts[k]) - 1];
+                } else {
+                    ch[k] = F.in[next++];
+                    st_sub[sp] = parts[k];
+                    st_node[sp++] = ch[k];
+                }
+            }
+            lgi_bvh_node& nd = nodes[idx];
+            nd.left = ch[0];
+            nd.right = ch[1];
+            for (int k = 0; k < 3; ++k) {
+                nd.bb_min[k] = S.mn[sub][k];
+                nd.bb_max[k] = S.mx[sub][k];
+            }
+            cost[idx] = S.c[sub];
+        }
+    }
+    __syncthreads();  // the next treelet reuses S
+}
+
+// Cost of every leaf (area x triangles); internal nodes get theirs when they are processed.
+__global__ __launch_bounds__(kBlock) void k_leaf_cost(const lgi_bvh_node* nodes, int m, float* cost) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= m) return;
+    const lgi_bvh_node nd = nodes[p];
+    if (nd.tri_count > 0) cost[p] = half_area(nd.bb_min, nd.bb_max) * kSahTri * (float)nd.tri_count;
+}
+
+// Restructures the treelet rooted at order[p0 + blockIdx.x] (p0 + blockIdx.x when order is
+// null), one wave per treelet. The treelets of one level are disjoint.
+__global__ __launch_bounds__(64) void k_treelet_level(lgi_bvh_node* nodes, const int32_t* order, int p0,
+                                                      float* cost) {
+    __shared__ TreeletForm F;
+    __shared__ union {
+        TreeletGrow g;
+        TreeletDp dp;
+    } U;
+    __shared__ SubsetsBySize by_size;
+    const int i = p0 + blockIdx.x;
+    const int p = order ? order[i] : i;
+    if (nodes[p].tri_count > 0) return;
+    // the subset table goes to LDS while the treelet forms: the DP indexes it per lane, which
+    // from constant memory would be a vector memory round trip per subset size
+    for (int k = threadIdx.x; k < (int)sizeof(SubsetsBySize); k += 64)
+        reinterpret_cast<uint8_t*>(&by_size)[k] = reinterpret_cast<const uint8_t*>(&kBySize)[k];
+    treelet_form(nodes, cost, p, F, U.g);
+    __syncthreads();
+    if (F.n >= 3) treelet_optimize(nodes, cost, F, U.dp, by_size);  // two leaves have one tree
 }
 
 // Levels of the current tree, in parallel: every slot finds its depth by walking its parent
@@ -881,9 +1033,7 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
             return lgi_set_err(ctx, LGI_ERR_HIP, "scratch allocation failed");
         const int32_t* order = nullptr;
         LGI_HIP(ctx, hipMemcpyAsync(onodes, d.nodes, (size_t)m * sizeof(lgi_bvh_node), hipMemcpyDeviceToDevice, st));
-        for (int lv = max_depth; lv >= 0; --lv)
-            hipLaunchKernelGGL(k_sah_level, dim3(ceil_div(hls[lv + 1] - hls[lv], kBlock)), dim3(kBlock), 0, st, onodes,
-                               hls[lv], hls[lv + 1], cost);
+        hipLaunchKernelGGL(k_leaf_cost, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, onodes, (int)m, cost);
         // Pass 1 takes its treelet roots by the reference tree's BFS levels (the array order). A
         // restructured tree no longer follows them (a root of one level may now lie below
         // another), so each later pass first takes the levels of the current tree.
@@ -914,8 +1064,8 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
             }
             if (pass == passes) break;
             for (int lv = n_lev - 2; lv >= 0; --lv)
-                hipLaunchKernelGGL(k_treelet, dim3(lv_host[lv + 1] - lv_host[lv]), dim3(64), 0, st, onodes, lv_order,
-                                   lv_host[lv], cost);
+                hipLaunchKernelGGL(k_treelet_level, dim3(lv_host[lv + 1] - lv_host[lv]), dim3(64), 0, st, onodes,
+                                   lv_order, lv_host[lv], cost);
         }
         if (tree_ok && n_lev - 1 <= kMaxTraversalDepth) {
             hipLaunchKernelGGL(k_bfs_flags, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, onodes, order, (int)m,

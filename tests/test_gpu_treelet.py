@@ -69,6 +69,11 @@ def test_traversal_tree_is_the_reference_tree_restructured(gpu, kind, w, h):
     print(f"  SAH cost {s_ref:.4g} -> {s_trav:.4g}, depth {len(levels(ref)) - 1} -> {len(lv) - 1}, "
           f"build {ctx.last_build_ms():.2f} ms", flush=True)
     assert s_trav < 0.8 * s_ref
+    # the bottom-up passes run treelets in whatever order the waves reach them: the tree must
+    # not depend on it
+    for _ in range(2):
+        ctx.build_bvh()
+        assert ctx.bvh_traversal().tobytes() == trav.tobytes(), "restructured tree differs between builds"
     ctx.set_treelet_passes(0)
     ctx.build_bvh()
     ref0, _, _ = ctx.bvh()
