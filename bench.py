@@ -206,6 +206,48 @@ def unet_workload(ckpt, reps=20):
     return out
 
 
+def train_workload(steps, warmup, batch=16, crop=256, n_frames=8):
+    """Training steps of the denoiser at batch 16 x 256x256 crops (Adam, lr 1e-4, L1 on log1p
+    radiance, dynamic loss scaling, activation recompute in the backward), on synthetic pairs of
+    1728x864 frames resident on the GPU. A step is crop sampling, forward, backward, optimizer
+    update. FLOP per step: the forward's 3x3 convolutions at the crop size, times 3 for forward,
+    data gradient and weight gradient (the recomputed forward counted apart)."""
+    import torch
+    from lgi import train as T
+    from lgi.unet import UNet
+    g = torch.Generator("cuda").manual_seed(0)
+    clean = torch.log1p(torch.rand(n_frames, 864, 1728, 3, device="cuda", generator=g) * 4.0)
+    noisy = clean + 0.1 * torch.randn(n_frames, 864, 1728, 3, device="cuda", generator=g)
+    torch.manual_seed(0)
+    model = UNet(seed=0, recompute=True).cuda()
+    params = list(model.parameters())
+    opt = torch.optim.Adam(params, lr=1e-4)
+    scaler = T.LossScaler()
+    gen = torch.Generator().manual_seed(0)
+
+    def step():
+        xb, yb = T.random_crops(noisy, clean, batch, crop, gen)
+        loss = (model(xb) - yb).abs().mean()
+        (loss * scaler.scale).backward()
+        scaler.step(params, opt)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / steps
+    fwd = UNET_GFLOP * batch * crop * crop / (1728 * 864)
+    return {"workload": f"denoiser training step, batch {batch} x {crop}x{crop} crops of synthetic 1728x864 "
+                        "log1p pairs, Adam lr 1e-4, loss scaling, recompute in the backward",
+            "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 3),
+            "gflop_forward": round(fwd, 1), "gflop_per_step": round(3 * fwd, 1),
+            "gflop_per_step_with_recompute": round(4 * fwd, 1),
+            "tflops": round(3 * fwd / ms, 1), "frac_of_bf16_peak": round(3 * fwd / ms / BF16_PEAK_TFLOPS, 4)}
+
+
 def _summary(r):
     """The figures of one time-to-40-dB evaluation that the bench line keeps."""
     keep = ("plain_spp_40db", "plain_ms_40db", "unet_reaches_40db", "unet_ms", "unet_variant_40db",
@@ -267,9 +309,10 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-probe", action="store_true", help="skip the L2/MALL/HBM read-bandwidth probe")
     ap.add_argument("--ckpt", default=str(DENOISER_CKPT), help="denoiser checkpoint for the 40 dB measurement")
-    ap.add_argument("--workload", choices=["frame", "poses", "unet"], default="frame",
+    ap.add_argument("--workload", choices=["frame", "poses", "unet", "train"], default="frame",
                     help="frame: the 64-spp frame of the 1.05M-triangle scene (default); poses: only time the "
-                         "50-pose multi-pose workload; unet: only time the denoiser on one 1728x864 frame")
+                         "50-pose multi-pose workload; unet: only time the denoiser on one 1728x864 frame; "
+                         "train: only time denoiser training steps (batch 16 x 256x256)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -293,6 +336,9 @@ def main():
         return
     if args.workload == "unet":
         print(json.dumps(unet_workload(args.ckpt)), flush=True)
+        return
+    if args.workload == "train":
+        print(json.dumps(train_workload(args.steps, args.warmup)), flush=True)
         return
 
     import numpy as np

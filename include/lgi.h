@@ -335,7 +335,10 @@ LGI_API lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, 
                                              int32_t cin, int32_t cin_split, int32_t n, int32_t h, int32_t w_,
                                              void* dx0, void* dx1, void* wt_scratch, void* stream);
 /* dw = sum_pixels (dy * (ymask > 0)) (x) patches(cat(src0, src1)) in FP32; db likewise (nullable).
- * scratch: FP32 workspace of scratch_floats (>= cout*9*cin). Deterministic (fixed-order sums). */
+ * scratch: FP32 workspace of scratch_floats (>= cout*9*cin, and >= cout when db is given);
+ * lgi_unet_wgrad_scratch_floats gives the size at which the pixel splits fill the GPU, less
+ * means fewer splits. Deterministic (fixed-order sums for a given scratch size). */
+LGI_API int64_t lgi_unet_wgrad_scratch_floats(int32_t cout, int32_t cin, int32_t n, int32_t h, int32_t w_);
 LGI_API lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_t cout, const void* src0,
                                                int32_t c0, const void* src1, int32_t c1, int32_t n, int32_t h,
                                                int32_t w_, float* dw, float* db, float* scratch,
@@ -373,6 +376,7 @@ LGI_API lgi_status lgi_unet_tail_fwd(const void* feat, const float* x, const flo
  * the denoiser's output conversion of tools/denoise.py (radiance out) inside the tail. */
 LGI_API lgi_status lgi_unet_tail_radiance_fwd(const void* feat, const float* hdr, const float* w, const float* b,
                                               int64_t npx, float* out, void* stream);
+/* scratch: 99 * min(1024, ceil(npx / 256)) floats at least (per-block partial sums). */
 LGI_API lgi_status lgi_unet_tail_bwd(const void* feat, const float* dout, const float* w, int64_t npx, void* dfeat,
                                      float* dw, float* db, float* scratch, int64_t scratch_floats, void* stream);
 

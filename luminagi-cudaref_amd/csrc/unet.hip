@@ -954,37 +954,86 @@ __global__ __launch_bounds__(kWgBlock) void k_conv_wgrad(const bf16* dy, const b
     }
 }
 
-// out[i] = sum over s of partial[s][i] (fixed order: deterministic).
-__global__ __launch_bounds__(kBlock) void k_reduce_splits(const float* partial, int64_t size, int splits, float* out,
-                                                          int accumulate) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= size) return;
-    float s = 0.f;
-    for (int k = 0; k < splits; ++k) s += partial[(int64_t)k * size + i];
-    out[i] = accumulate ? out[i] + s : s;
+// Column sums of rows of `size` floats: block (x, c) adds rows [c * per_chunk, (c + 1) *
+// per_chunk) of `rows` (row k at partial + k * row_stride * size) over 64 columns and writes
+// out + c * out_stride. The four waves take every fourth row, each lane into 8 running sums
+// (8 loads in flight) added in order at the end, and the wave sums are added in wave order:
+// every sum has a fixed order, so results are deterministic.
+__global__ __launch_bounds__(kBlock) void k_reduce_rows(const float* partial, int64_t size, int rows,
+                                                        int64_t row_stride, int per_chunk, float* out,
+                                                        int64_t out_stride, int accumulate) {
+    __shared__ float s[kBlock];
+    constexpr int kWaves = kBlock / 64, kIlp = 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const int r0 = blockIdx.y * per_chunk, r1 = min(rows, r0 + per_chunk);
+    float sum = 0.f;
+    if (i < size) {
+        float acc[kIlp] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int k0 = r0 + wave; k0 < r1; k0 += kWaves * kIlp)
+#pragma unroll
+            for (int u = 0; u < kIlp; ++u) {
+                const int k = k0 + u * kWaves;
+                if (k < r1) acc[u] += partial[(int64_t)k * row_stride * size + i];
+            }
+        for (int u = 0; u < kIlp; ++u) sum += acc[u];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();  // every row of the chunk is read before its first row is overwritten
+    if (wave == 0 && i < size) {
+        const float t = ((s[lane] + s[lane + 64]) + s[lane + 128]) + s[lane + 192];
+        float* o = out + blockIdx.y * out_stride;
+        o[i] = accumulate ? o[i] + t : t;
+    }
 }
 
-// db[co] = sum over pixels of dYm[p][co]; one block per 64 channels x pixel split.
-__global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* dy, const bf16* ymask, int64_t npx, int C,
-                                                      int px_per_split, float* partial) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int split = blockIdx.y;
-    __shared__ float s[kBlock];
-    float acc = 0.f;
-    if (c < C) {
-        const int64_t p0 = (int64_t)split * px_per_split;
-        const int64_t p1 = min(npx, p0 + px_per_split);
-        for (int64_t p = p0 + (threadIdx.x >> 6); p < p1; p += kBlock / 64) {
-            float v = (float)dy[p * C + c];
-            if (ymask && !((float)ymask[p * C + c] > 0.0f)) v = 0.f;
-            acc += v;
-        }
+// out[i] (+)= sum over s of partial[s][i], in a fixed order. Up to 64 splits in one pass;
+// more are summed in chunks of 64 first (in place, into each chunk's first row) so that the
+// reads of a small weight tensor with many splits spread over many blocks.
+void reduce_splits(hipStream_t st, float* partial, int64_t size, int splits, float* out, int accumulate) {
+    constexpr int kChunk = 64;
+    const int gx = cdiv(size, 64);
+    if (splits <= kChunk) {
+        hipLaunchKernelGGL(k_reduce_rows, dim3(gx, 1), dim3(kBlock), 0, st, partial, size, splits, (int64_t)1, splits,
+                           out, (int64_t)0, accumulate);
+        return;
     }
-    s[threadIdx.x] = acc;
+    const int chunks = cdiv(splits, kChunk);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(gx, chunks), dim3(kBlock), 0, st, partial, size, splits, (int64_t)1, kChunk,
+                       partial, (int64_t)kChunk * size, 0);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(gx, 1), dim3(kBlock), 0, st, partial, size, chunks, (int64_t)kChunk, chunks,
+                       out, (int64_t)0, accumulate);
+}
+
+// partial[split][c] = sum over the split's pixels of dYm[p][c] (the bias gradient before the
+// split sum). A thread reads 8 channels (16 bytes) of a pixel per step; the block's threads
+// cover kBlock / (C/8) pixels side by side, and their sums are added per channel in LDS in a
+// fixed order. C is a multiple of 8, at most 8 * kBlock.
+__global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* dy, const bf16* ymask, int64_t npx, int C,
+                                                      int64_t px_per_split, float* partial) {
+    __shared__ float s[kBlock * 8];
+    const int c8 = C / 8, rows = kBlock / c8, t = threadIdx.x, g = t % c8, r = t / c8;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (r < rows) {
+        const int64_t p0 = (int64_t)blockIdx.x * px_per_split, p1 = min(npx, p0 + px_per_split);
+        for (int64_t p = p0 + r; p < p1; p += rows) {
+            const uint4 v = *reinterpret_cast<const uint4*>(dy + p * C + g * 8);
+            const bf16x8 e = *reinterpret_cast<const bf16x8*>(&v);
+            if (ymask) {
+                const uint4 mv = *reinterpret_cast<const uint4*>(ymask + p * C + g * 8);
+                const bf16x8 m = *reinterpret_cast<const bf16x8*>(&mv);
+                for (int j = 0; j < 8; ++j) acc[j] += (float)m[j] > 0.0f ? (float)e[j] : 0.0f;
+            } else {
+                for (int j = 0; j < 8; ++j) acc[j] += (float)e[j];
+            }
+        }
+        for (int j = 0; j < 8; ++j) s[r * C + g * 8 + j] = acc[j];
+    }
     __syncthreads();
-    if (threadIdx.x < 64 && c < C) {
-        float t = s[threadIdx.x] + s[threadIdx.x + 64] + s[threadIdx.x + 128] + s[threadIdx.x + 192];
-        partial[(int64_t)split * C + c] = t;
+    for (int c = t; c < C; c += kBlock) {
+        float sum = 0.f;
+        for (int rr = 0; rr < rows; ++rr) sum += s[rr * C + c];
+        partial[(int64_t)blockIdx.x * C + c] = sum;
     }
 }
 
@@ -1144,14 +1193,18 @@ __global__ __launch_bounds__(kBlock) void k_tail_fwd(const bf16* feat, const flo
     }
 }
 
-// dfeat = W_t^T dout (BF16), per-block partial dW_t (3 x 32) and db_t (3).
+// dfeat = W_t^T dout (BF16), and per-block partial sums of dW_t (3 x 32) and db_t (3) as 99
+// values [o][33] (32 weights, then the bias). Threads stride over the pixels with the grid,
+// keep their 99 sums in registers, and reduce them once per block (waves, then LDS in wave
+// order): the order of every sum is fixed by the grid.
+constexpr int kTailBwdBlocks = 1024;
 __global__ __launch_bounds__(kBlock) void k_tail_bwd(const bf16* feat, const float* dout, const float* w, int64_t npx,
-                                                     bf16* dfeat, float* partial /* [blocks][99] */) {
-    __shared__ float s[kBlock];
-    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    float f[32], d[3] = {0.f, 0.f, 0.f};
-    const bool ok = p < npx;
-    if (ok) {
+                                                     bf16* dfeat, float* partial /* [gridDim.x][99] */) {
+    __shared__ float s[kBlock / 64][99];
+    float acc[99];
+    for (int k = 0; k < 99; ++k) acc[k] = 0.f;
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < npx; p += (int64_t)gridDim.x * kBlock) {
+        float f[32], d[3];
         const uint4* fp = reinterpret_cast<const uint4*>(feat + p * 32);
         for (int q = 0; q < 4; ++q) {
             const uint4 v = fp[q];
@@ -1159,35 +1212,46 @@ __global__ __launch_bounds__(kBlock) void k_tail_bwd(const bf16* feat, const flo
             for (int i = 0; i < 8; ++i) f[q * 8 + i] = (float)e[i];
         }
         for (int o = 0; o < 3; ++o) d[o] = dout[p * 3 + o];
-        for (int c = 0; c < 32; ++c) {
-            const float g = w[c] * d[0] + w[32 + c] * d[1] + w[64 + c] * d[2];
-            dfeat[p * 32 + c] = (bf16)g;
+        bf16x8 g8[4];
+        for (int c = 0; c < 32; ++c) g8[c / 8][c % 8] = (bf16)(w[c] * d[0] + w[32 + c] * d[1] + w[64 + c] * d[2]);
+        uint4* dp = reinterpret_cast<uint4*>(dfeat + p * 32);
+        for (int q = 0; q < 4; ++q) dp[q] = *reinterpret_cast<const uint4*>(&g8[q]);
+        for (int o = 0; o < 3; ++o) {
+            for (int c = 0; c < 32; ++c) acc[o * 33 + c] += d[o] * f[c];
+            acc[o * 33 + 32] += d[o];
         }
-    } else {
-        for (int c = 0; c < 32; ++c) f[c] = 0.f;
     }
-    // block reductions of d[o] * f[c] and d[o]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int k = 0; k < 99; ++k) {
-        const int o = k / 33, c = k % 33;
-        s[threadIdx.x] = c < 32 ? d[o] * f[c] : d[o];
-        __syncthreads();
-        for (int st = kBlock / 2; st > 0; st >>= 1) {
-            if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * 99 + k] = s[0];
-        __syncthreads();
+        float v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) s[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 99) {
+        float t = s[0][threadIdx.x];
+        for (int wv = 1; wv < kBlock / 64; ++wv) t += s[wv][threadIdx.x];
+        partial[(int64_t)blockIdx.x * 99 + threadIdx.x] = t;
     }
 }
 
-__global__ void k_tail_reduce(const float* partial, int blocks, float* dw, float* db) {
-    const int k = threadIdx.x;
-    if (k >= 99) return;
-    float s = 0.f;
-    for (int b = 0; b < blocks; ++b) s += partial[(int64_t)b * 99 + k];
-    const int o = k / 33, c = k % 33;
-    if (c < 32) dw[o * 32 + c] = s;
-    else db[o] = s;
+// dW_t and db_t from the blocks' partials: column k of [blocks][99] summed as in
+// k_reduce_rows (64 columns per block, four waves over every fourth row, wave order).
+__global__ __launch_bounds__(kBlock) void k_tail_reduce(const float* partial, int blocks, float* dw, float* db) {
+    __shared__ float s[kBlock];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane;
+    float sum = 0.f;
+    if (k < 99)
+        for (int b = wave; b < blocks; b += kBlock / 64) sum += partial[(int64_t)b * 99 + k];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    if (wave == 0 && k < 99) {
+        const float t = ((s[lane] + s[lane + 64]) + s[lane + 128]) + s[lane + 192];
+        const int o = k / 33, c = k % 33;
+        if (c < 32) dw[o * 32 + c] = t;
+        else db[o] = t;
+    }
 }
 
 // ------------------------------------------------------------------ tile-shape selection
@@ -1485,6 +1549,23 @@ lgi_status lgi_unet_conv3x3_bwd_data(const void* dy, const void* ymask, const vo
     return conv_launch(a, 9, st);
 }
 
+// Pixel splits of the weight gradient: enough 32 x 32-channel blocks to fill the chip (2048,
+// about 8 per CU), at most one per 8x16 tile. Small layers at full resolution get many splits;
+// the 512-channel layers few, which keeps their partial sums (splits x cout x 9 x cin floats)
+// small.
+static int64_t wgrad_splits(int cout, int cin, int64_t tiles) {
+    const int64_t per_split = (int64_t)(cout / 32) * (cin / 32);
+    return std::max<int64_t>(1, std::min<int64_t>(tiles, cdiv(2048, std::max<int64_t>(1, per_split))));
+}
+
+int64_t lgi_unet_wgrad_scratch_floats(int32_t cout, int32_t cin, int32_t n, int32_t h, int32_t wd) {
+    if (cout <= 0 || cin <= 0 || n <= 0 || h <= 0 || wd <= 0) return 0;
+    const int64_t tiles = (int64_t)cdiv(wd, kTW) * cdiv(h, kTH) * n;
+    const int64_t npx = (int64_t)n * h * wd;
+    return std::max<int64_t>(wgrad_splits(cout, cin, tiles) * cout * 9 * cin,
+                             std::min<int64_t>(1024, cdiv(npx, 64)) * cout);
+}
+
 lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_t cout, const void* src0, int32_t c0,
                                        const void* src1, int32_t c1, int32_t n, int32_t h, int32_t wd, float* dw,
                                        float* db, float* scratch, int64_t scratch_floats, void* stream) {
@@ -1494,23 +1575,22 @@ lgi_status lgi_unet_conv3x3_bwd_weight(const void* dy, const void* ymask, int32_
     const int tiles = cdiv(wd, kTW) * cdiv(h, kTH) * n;
     const int64_t wsize = (int64_t)cout * 9 * cin;
     // as many pixel splits as the scratch holds, at most one per tile, at most 64
-    int splits = (int)std::min<int64_t>(std::min<int64_t>(64, tiles), scratch_floats / wsize);
-    if (splits < 1) return LGI_ERR_INVALID_ARG;
+    int splits = (int)std::min<int64_t>(wgrad_splits(cout, cin, tiles), scratch_floats / wsize);
+    if (splits < 1 || (db && cout > 8 * kBlock)) return LGI_ERR_INVALID_ARG;
     const int per = cdiv(tiles, splits);
     splits = cdiv(tiles, per);
     hipLaunchKernelGGL(k_conv_wgrad, dim3(cout / 32, cin / 32, splits), dim3(kWgBlock), 0, st, (const bf16*)dy,
                        (const bf16*)ymask, cout, (const bf16*)src0, c0, (const bf16*)src1, src1 ? c1 : 0, n, h, wd, per,
                        scratch);
-    hipLaunchKernelGGL(k_reduce_splits, dim3(cdiv(wsize, kBlock)), dim3(kBlock), 0, st, scratch, wsize, splits, dw, 0);
+    reduce_splits(st, scratch, wsize, splits, dw, 0);
     if (db) {
         const int64_t npx = (int64_t)n * h * wd;
-        const int bsplits = (int)std::min<int64_t>(std::min<int64_t>(256, npx), scratch_floats / cout);
-        const int bper = cdiv(npx, bsplits);
+        const int bsplits = (int)std::min<int64_t>(std::min<int64_t>(1024, cdiv(npx, 64)), scratch_floats / cout);
+        const int64_t bper = cdiv(npx, bsplits);
         const int bs = cdiv(npx, bper);
-        hipLaunchKernelGGL(k_bias_grad, dim3(cdiv(cout, 64), bs), dim3(kBlock), 0, st, (const bf16*)dy,
-                           (const bf16*)ymask, npx, cout, bper, scratch);
-        hipLaunchKernelGGL(k_reduce_splits, dim3(cdiv(cout, kBlock)), dim3(kBlock), 0, st, scratch, (int64_t)cout, bs, db,
-                           0);
+        hipLaunchKernelGGL(k_bias_grad, dim3(bs), dim3(kBlock), 0, st, (const bf16*)dy, (const bf16*)ymask, npx, cout,
+                           bper, scratch);
+        reduce_splits(st, scratch, (int64_t)cout, bs, db, 0);
     }
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
@@ -1599,12 +1679,12 @@ lgi_status lgi_unet_tail_radiance_fwd(const void* feat, const float* hdr, const 
 lgi_status lgi_unet_tail_bwd(const void* feat, const float* dout, const float* w, int64_t npx, void* dfeat, float* dw,
                              float* db, float* scratch, int64_t scratch_floats, void* stream) {
     if (!feat || !dout || !w || !dfeat || !dw || !db || !scratch) return LGI_ERR_INVALID_ARG;
-    const int blocks = cdiv(npx, kBlock);
+    const int blocks = (int)std::min<int64_t>(kTailBwdBlocks, cdiv(npx, kBlock));
     if ((int64_t)blocks * 99 > scratch_floats) return LGI_ERR_INVALID_ARG;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_tail_bwd, dim3(blocks), dim3(kBlock), 0, st, (const bf16*)feat, dout, w, npx, (bf16*)dfeat,
                        scratch);
-    hipLaunchKernelGGL(k_tail_reduce, dim3(1), dim3(128), 0, st, scratch, blocks, dw, db);
+    hipLaunchKernelGGL(k_tail_reduce, dim3(2), dim3(kBlock), 0, st, scratch, blocks, dw, db);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 

@@ -140,6 +140,7 @@ SIGNATURES = {
     "lgi_unet_pack_input": (C.c_int, [_P, C.c_int64, _P, _P]),
     "lgi_unet_tail_fwd": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P]),
     "lgi_unet_tail_bwd": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P, _P, _P, C.c_int64, _P]),
+    "lgi_unet_wgrad_scratch_floats": (C.c_int64, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
 }
 
 _lib = None

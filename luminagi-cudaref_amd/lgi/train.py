@@ -33,14 +33,16 @@ class LossScaler:
 
     def step(self, params, opt):
         grads = [p.grad for p in params if p.grad is not None]
-        finite = all(bool(torch.isfinite(g).all()) for g in grads)
-        if not finite:
+        # one fused pass over all gradients: unscale in place (exact, the scale is a power of
+        # two) and flag any inf/nan; one host synchronization per step
+        found = torch.zeros(1, device=grads[0].device)
+        inv = torch.full((1,), 1.0 / self.scale, device=grads[0].device)
+        torch._amp_foreach_non_finite_check_and_unscale_(grads, found, inv)
+        if found.item():
             self.scale /= 2.0
             self.clean = 0
             opt.zero_grad(set_to_none=True)
             return False
-        for g in grads:
-            g.div_(self.scale)
         opt.step()
         opt.zero_grad(set_to_none=True)
         self.clean += 1

@@ -192,10 +192,9 @@ def conv3x3_bwd(dy, y_relu, w_bf16, src0, src1, need_dx=True):
               _p(wt), _stream())
     dw = torch.empty((cout, 9, cin), dtype=torch.float32, device=dy.device)
     db = torch.empty(cout, dtype=torch.float32, device=dy.device)
-    wsize = cout * 9 * cin
-    tiles = ((w + 15) // 16) * ((h + 7) // 8) * n
-    splits = max(1, min(64, tiles, WGRAD_SCRATCH_MAX // wsize))
-    scratch = torch.empty(max(splits * wsize, 256 * cout), dtype=torch.float32, device=dy.device)
+    # the library's preferred workspace (its pixel splits fill the GPU), capped at 256 MB
+    want = _abi.lib().lgi_unet_wgrad_scratch_floats(cout, cin, n, h, w)
+    scratch = torch.empty(max(min(want, WGRAD_SCRATCH_MAX), cout * 9 * cin), dtype=torch.float32, device=dy.device)
     _call("lgi_unet_conv3x3_bwd_weight", _p(dy), _p(y_relu), cout, _p(src0), c0, _p(src1), c1, n, h, w, _p(dw),
           _p(db), _p(scratch), scratch.numel(), _stream())
     return dx0, dx1, dw, db
@@ -284,7 +283,7 @@ class TailFn(torch.autograd.Function):
         dfeat = torch.empty_like(feat)
         dw = torch.empty((3, 32), dtype=torch.float32, device=feat.device)
         db = torch.empty(3, dtype=torch.float32, device=feat.device)
-        blocks = (npx + 255) // 256
+        blocks = min(1024, (npx + 255) // 256)  # the kernel's grid: per-block partial sums
         scratch = torch.empty(blocks * 99, dtype=torch.float32, device=feat.device)
         _call("lgi_unet_tail_bwd", _p(feat), _p(dout), _p(w.contiguous()), npx, _p(dfeat), _p(dw), _p(db), _p(scratch),
               scratch.numel(), _stream())
