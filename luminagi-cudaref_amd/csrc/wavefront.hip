@@ -92,34 +92,42 @@ __device__ __forceinline__ uint32_t ray_key(const WfParams& W, V3 o, V3 d) {
 }
 
 // ------------------------------------------------------------------ primary rays
-// Queue index i -> (sample s, 8x8 pixel tile, lane) so each wave starts on one tile.
-__global__ __launch_bounds__(kBlock) void k_wf_primary(WfParams W, int S, int tiles_x, int padded, QRay* q) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= (int64_t)S * padded) return;
-    const int s = (int)(i / padded), j = (int)(i % padded);
-    const int tile = W.tiles ? W.tiles[j >> 6] : j >> 6, lane = j & 63;
-    const int px = (tile % tiles_x) * 8 + (lane & 7), py = (tile / tiles_x) * 8 + (lane >> 3);
-    const DevCamera& c = W.P.cam;
+// The bounce-0 queue is never stored: entry i is a function of i, evaluated where it is read
+// (the closest-hit trace of bounce 0 and the shading of bounce 0). Queue index i -> (sample s,
+// 8x8 pixel tile, lane), so each wave starts on one tile.
+struct PrimarySrc {
+    DevCamera cam;
+    uint32_t seed;
+    int32_t sample0;      // first sample of this pass
+    int32_t npx;          // W*H
+    const int32_t* tiles; // 8x8 tiles to render, or null for the whole image
+    int32_t tiles_x, padded;
+};
+
+// Entry i's ray (o.w = path id, or -1 for a lane outside the image; d.w = tmax) and the path's
+// RNG after the two jitter draws.
+__device__ __forceinline__ QRay primary_entry(const PrimarySrc& ps, int64_t i, Pcg& g) {
+    const int s = (int)(i / ps.padded), j = (int)(i % ps.padded);
+    const int tile = ps.tiles ? ps.tiles[j >> 6] : j >> 6, lane = j & 63;
+    const int px = (tile % ps.tiles_x) * 8 + (lane & 7), py = (tile / ps.tiles_x) * 8 + (lane >> 3);
     QRay r;
-    if (px >= c.width || py >= c.height) {
+    if (px >= ps.cam.width || py >= ps.cam.height) {
         r.o = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         r.d = make_float4(0.f, 0.f, 1.f, 0.f);
-        q[i] = r;
-        return;
+        return r;
     }
-    const uint32_t pix = (uint32_t)py * c.width + px;
-    const int64_t p = (int64_t)s * W.npx + pix;
+    const uint32_t pix = (uint32_t)py * ps.cam.width + px;
+    const int64_t p = (int64_t)s * ps.npx + pix;
     Rng rng;
     rng.pcg = true;
-    pcg_for_sample(rng.g, W.P.seed, pix, (uint32_t)(W.P.sample0 + s));
+    pcg_for_sample(rng.g, ps.seed, pix, (uint32_t)(ps.sample0 + s));
     const float jx = rng.next();
     const float jy = rng.next();
-    Ray ray = primary_ray(c, px, py, jx, jy);
-    W.rng[p] = rng.g.state;
-    W.L[3 * p] = 0.f; W.L[3 * p + 1] = 0.f; W.L[3 * p + 2] = 0.f;  // throughput starts at 1: see k_wf_shade
+    const Ray ray = primary_ray(ps.cam, px, py, jx, jy);
+    g = rng.g;
     r.o = make_float4(ray.o.x, ray.o.y, ray.o.z, __int_as_float((int)p));
     r.d = make_float4(ray.d.x, ray.d.y, ray.d.z, 1e30f);
-    q[i] = r;
+    return r;
 }
 
 // ------------------------------------------------------------------ persistent traversal
@@ -153,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
                                                      const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev,
                                                      float4* hits, uint8_t* nee, uint32_t* cursors, int* spill,
-                                                     ShadowSrc shadow) {
+                                                     ShadowSrc shadow, PrimarySrc prim) {
     if (kMode == kTraceShadow) n = 0;
     if (kMode == kTraceClosest) n_sh = 0;
     if (n_dev) n = min(n, (int)*n_dev);
@@ -240,7 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                     active = true;
                 } else if (idx != 0xFFFFFFFFu) {
                     if (kMode == kTraceBoth) idx -= (uint32_t)n_sh;
-                    const QRay e = q[idx];
+                    Pcg unused;
+                    const QRay e = kMode == kTraceClosest ? primary_entry(prim, idx, unused) : q[idx];
                     qi = (int)idx;
                     slot = __float_as_int(e.o.w);
                     const float tmax = e.d.w;
@@ -371,7 +380,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 // kernels rank entries within their block, so no scan runs over the per-slot flags.
 __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n,
                                                      const uint32_t* n_dev, int bounce, QRay* nq, uint32_t* nkeys,
-                                                     uint8_t* ncont, uint32_t* blk_s, uint32_t* blk_n) {
+                                                     uint8_t* ncont, uint32_t* blk_s, uint32_t* blk_n,
+                                                     PrimarySrc prim) {
     if (n_dev) n = min(n, (int)*n_dev);
     if ((int)blockIdx.x * kBlock >= n) {  // whole block past the count: zero counts for the scans
         if (threadIdx.x == 0) {
@@ -384,9 +394,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     int p = -1;
     Hit h{-1, 0.f, 0.f, 0.f};
     QRay e;
+    Pcg g0{0u, 0u};  // bounce 0: the path's RNG after its jitter draws
     if (i < n) {
-        e = q[i];
+        e = bounce == 0 ? primary_entry(prim, i, g0) : q[i];
         p = __float_as_int(e.o.w);
+        // bounce 0 starts every path of the pass: its radiance sum begins at 0
+        if (bounce == 0 && p >= 0) {
+            W.L[3 * (int64_t)p] = 0.f; W.L[3 * (int64_t)p + 1] = 0.f; W.L[3 * (int64_t)p + 2] = 0.f;
+        }
         if (p >= 0) {
             const float4 hv = hits[i];
             h.tri = __float_as_int(hv.w);
@@ -442,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             const uint32_t pix = (uint32_t)(p % W.npx);
             const uint64_t stream = ((uint64_t)W.P.seed << 32) | pix;
             rng.g.inc = (stream << 1u) | 1u;
-            rng.g.state = W.rng[p];
+            rng.g.state = bounce == 0 ? g0.state : W.rng[p];
             if (bounce >= 1 && (W.P.flags & LGI_RENDER_NO_RR) == 0) {
                 float qv = fminf(0.95f, fmaxf(Tv.x, fmaxf(Tv.y, Tv.z)));
                 if (rng.next() > qv) cont = false;
@@ -620,11 +635,11 @@ int trace_grid(int n) {
 template <int kMode>
 void launch_trace(hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
                   const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev, float4* hits, uint8_t* nee,
-                  uint32_t* cursors, int* spill, const ShadowSrc& shadow) {
+                  uint32_t* cursors, int* spill, const ShadowSrc& shadow, const PrimarySrc& prim) {
     constexpr int kMin = kMode == kTraceShadow ? kTraceMinLanesAny : kTraceMinLanes;
     const int grid = trace_grid((kMode == kTraceShadow ? 0 : n) + (kMode == kTraceClosest ? 0 : n_sh));
     hipLaunchKernelGGL((k_wf_trace<kMode, kTraceLdsStack, kTraceTopNodes, kMin>), dim3(grid), dim3(kBlock), 0, st, tn,
-                       n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow);
+                       n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow, prim);
 }
 
 
@@ -731,19 +746,20 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         const int nb = cdiv(n_max, kBlock);
         hipLaunchKernelGGL(k_wf_fill, dim3(cdiv(kCounters * (P.bounces + 1), kBlock)), dim3(kBlock), 0, st, counters,
                            kCounters * (P.bounces + 1), 0u);
-        hipLaunchKernelGGL(k_wf_primary, dim3(cdiv(n_max, kBlock)), dim3(kBlock), 0, st, W, S, tiles_x, padded, qa);
+        // bounce 0's queue is computed where it is read (primary_entry), never stored
+        const PrimarySrc prim{W.P.cam, W.P.seed, W.P.sample0, W.npx, W.tiles, tiles_x, padded};
         QRay* cur = qa;
         QRay* other = qb;
         const uint32_t* n_dev = nullptr;  // live entries of this bounce's queue (bounce 0: all n_max)
         if ((ms = mark(true)) != LGI_OK) return ms;
         launch_trace<kTraceClosest>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, n_dev, 0, nullptr, hits,
-                                    nullptr, counters + kCursorStride, spill, shadow);
+                                    nullptr, counters + kCursorStride, spill, shadow, prim);
         if ((ms = mark(false)) != LGI_OK) return ms;
         for (int b = 0; b <= P.bounces; ++b) {
             uint32_t* cnt = counters + kCounters * b;
             const bool last = b == P.bounces;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
-                               sort ? keys_tmp : nullptr, ncont, blk_s, blk_n);
+                               sort ? keys_tmp : nullptr, ncont, blk_s, blk_n, prim);
             // block offsets of the shadow-ray candidates and of the continuations
             lgi_exclusive_scan(st, blk_s, nb, cnt + 1, scan_scratch);
             lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
@@ -770,10 +786,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             if ((ms = mark(true)) != LGI_OK) return ms;
             if (last)
                 launch_trace<kTraceShadow>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, nullptr, 0, nullptr, n_max * nl,
-                                           cnt + 1, nullptr, W.nee, cur_sh, spill, shadow);
+                                           cnt + 1, nullptr, W.nee, cur_sh, spill, shadow, prim);
             else
                 launch_trace<kTraceBoth>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2, n_max * nl,
-                                         cnt + 1, hits, W.nee, cur_sh, spill, shadow);
+                                         cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
             n_dev = cnt + 2;
