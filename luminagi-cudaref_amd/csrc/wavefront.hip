@@ -41,7 +41,7 @@ struct WfParams {
     float3 bb_inv;        // 1 / extent
     uint64_t* rng;        // PCG32 state per path
     float* T;             // throughput per path (3)
-    float* Tb;            // throughput before the bounce update (3)
+    float4* vT;           // per queue slot: throughput before the bounce update (xyz)
     float* L;             // radiance per path (3)
     float* contrib;       // per (queue slot, light): contribution if unoccluded (3)
     uint8_t* nee;         // per (queue slot, light): 0 none, 1 candidate, 2 occluded
@@ -410,6 +410,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     }
     const bool live = p >= 0 && h.tri >= 0;
     V3 N = v3(0.f, 0.f, 1.f), Pt = v3(0.f, 0.f, 0.f), albedo = v3(0.f, 0.f, 0.f);
+    V3 T0 = v3(1.f, 1.f, 1.f);  // throughput before this bounce: exactly 1 until the first bounce
     if (live) {
         const V3 d = v3(e.d.x, e.d.y, e.d.z);
         int mat;
@@ -418,13 +419,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
         if (dot(N, d) > 0.0f) N = mul(N, -1.0f);
         Pt = add(v3(e.o.x, e.o.y, e.o.z), mul(d, h.t));
         albedo = sample_albedo(W.P, mat, tu, tv);
-        // a path's throughput is exactly 1 until its first bounce, so bounce 0 does not read it
-        float* Tb = W.Tb + 3 * (int64_t)p;
-        if (bounce == 0) {
-            Tb[0] = 1.f; Tb[1] = 1.f; Tb[2] = 1.f;
-        } else {
+        if (bounce > 0) {
             const float* T = W.T + 3 * (int64_t)p;
-            Tb[0] = T[0]; Tb[1] = T[1]; Tb[2] = T[2];
+            T0 = v3(T[0], T[1], T[2]);
         }
     }
     // NEE candidates: flags and contributions in fixed slots i*nl + li (compacted afterwards);
@@ -442,16 +439,17 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
             ncand += cand ? 1u : 0u;
         }
-        if (ncand) {  // w: the path id, for k_wf_resolve
+        if (ncand) {  // w: the path id, for k_wf_resolve; the throughput it scales the light by
             W.vP[i] = make_float4(Pt.x, Pt.y, Pt.z, __int_as_float(p));
             W.vN[i] = make_float4(N.x, N.y, N.z, 0.f);
+            W.vT[i] = make_float4(T0.x, T0.y, T0.z, 0.f);
         }
         // continuation
         cont = live && bounce < W.P.bounces;
         Ray next;
         if (cont) {
             float* T = W.T + 3 * (int64_t)p;
-            V3 Tv = bounce == 0 ? v3(1.f, 1.f, 1.f) : v3(T[0], T[1], T[2]);
+            V3 Tv = T0;
             Rng rng;
             rng.pcg = true;
             const uint32_t pix = (uint32_t)(p % W.npx);
@@ -564,8 +562,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const 
     if (!any) return;
     const int p = __float_as_int(W.vP[i].w);
     float* L = W.L + 3 * (int64_t)p;
-    const float* Tb = W.Tb + 3 * (int64_t)p;
-    V3 Lv = add(v3(L[0], L[1], L[2]), had(v3(Tb[0], Tb[1], Tb[2]), color));
+    const float4 Tb = W.vT[i];  // the path's throughput before this bounce, stored by k_wf_shade
+    V3 Lv = add(v3(L[0], L[1], L[2]), had(v3(Tb.x, Tb.y, Tb.z), color));
     L[0] = Lv.x; L[1] = Lv.y; L[2] = Lv.z;
 }
 
@@ -714,7 +712,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     const int64_t paths = (int64_t)S_max * npx;
     W.rng = (uint64_t*)S_(58, paths * 8);
     W.T = (float*)S_(59, paths * 12);
-    W.Tb = (float*)S_(60, paths * 12);
+    W.vT = (float4*)S_(60, cap * 16);
     W.L = (float*)S_(61, paths * 12);
     W.contrib = (float*)S_(62, cap * nl * 12);
     W.nee = (uint8_t*)S_(63, cap * nl);
@@ -724,7 +722,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)blk_s, (void*)blk_n, (void*)ncont,
                     (void*)keys_tmp,
                     (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.rng, (void*)W.T,
-                    (void*)W.Tb, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
+                    (void*)W.vT, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
     const bool sort = !(P.flags & LGI_RENDER_NO_RAY_SORT);
