@@ -39,7 +39,6 @@ struct WfParams {
     int32_t nl;           // lights: sun + points
     float3 bb_min;        // scene box for the sort key
     float3 bb_inv;        // 1 / extent
-    uint64_t* rng;        // PCG32 state per path
     float* T;             // throughput per path (3)
     float4* vT;           // per queue slot: throughput before the bounce update (xyz)
     float* L;             // radiance per path (3)
@@ -355,6 +354,22 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
 // ------------------------------------------------------------------ shading
 // For each traced path: NEE candidates -> shadow queue, Russian roulette + cosine bounce ->
 // next queue with its sort key (src/pathtracer.cu:295-331, same order of RNG draws).
+// A path's RNG at bounce b >= 1, recomputed from its id rather than stored: its PCG32 stream
+// starts at the (pixel, sample) seed, and a path reaches bounce b only by continuing through
+// every earlier bounce, each of which drew a fixed number of values (2 for the camera jitter;
+// per earlier bounce j, 1 Russian-roulette draw when j >= 1 and roulette is on, then 2 for the
+// cosine-weighted direction). The state is advanced past those draws.
+__device__ __forceinline__ Pcg path_rng(const WfParams& W, int64_t p, int bounce) {
+    Pcg g;
+    const uint32_t pix = (uint32_t)(p % W.npx), s = (uint32_t)(p / W.npx);
+    pcg_for_sample(g, W.P.seed, pix, (uint32_t)W.P.sample0 + s);
+    const bool rr = (W.P.flags & LGI_RENDER_NO_RR) == 0;
+    int draws = 2;
+    for (int j = 0; j < bounce; ++j) draws += 2 + (j >= 1 && rr ? 1 : 0);
+    for (int k = 0; k < draws; ++k) pcg_next(g);
+    return g;
+}
+
 // Block-wide exclusive scan of one value per thread (256 threads; every thread must call).
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -452,10 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             V3 Tv = T0;
             Rng rng;
             rng.pcg = true;
-            const uint32_t pix = (uint32_t)(p % W.npx);
-            const uint64_t stream = ((uint64_t)W.P.seed << 32) | pix;
-            rng.g.inc = (stream << 1u) | 1u;
-            rng.g.state = bounce == 0 ? g0.state : W.rng[p];
+            rng.g = bounce == 0 ? g0 : path_rng(W, p, bounce);
             if (bounce >= 1 && (W.P.flags & LGI_RENDER_NO_RR) == 0) {
                 float qv = fminf(0.95f, fmaxf(Tv.x, fmaxf(Tv.y, Tv.z)));
                 if (rng.next() > qv) cont = false;
@@ -467,7 +479,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
                 Tv = had(Tv, albedo);
                 T[0] = Tv.x; T[1] = Tv.y; T[2] = Tv.z;
             }
-            W.rng[p] = rng.g.state;
         }
         if (cont) {
             QRay nr;
@@ -710,7 +721,6 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     W.bb_inv = make_float3(1.0f / std::max(bb_max.x - bb_min.x, 1e-6f), 1.0f / std::max(bb_max.y - bb_min.y, 1e-6f),
                            1.0f / std::max(bb_max.z - bb_min.z, 1e-6f));
     const int64_t paths = (int64_t)S_max * npx;
-    W.rng = (uint64_t*)S_(58, paths * 8);
     W.T = (float*)S_(59, paths * 12);
     W.vT = (float4*)S_(60, cap * 16);
     W.L = (float*)S_(61, paths * 12);
@@ -721,7 +731,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     ShadowSrc shadow{sids, W.vP, W.vN, nl, P};
     for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)blk_s, (void*)blk_n, (void*)ncont,
                     (void*)keys_tmp,
-                    (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.rng, (void*)W.T,
+                    (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.T,
                     (void*)W.vT, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
