@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
                                                      const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev,
                                                      float4* hits, uint8_t* nee, uint32_t* cursors, int* spill,
-                                                     ShadowSrc shadow, PrimarySrc prim) {
+                                                     ShadowSrc shadow, PrimarySrc prim, const int32_t* qord) {
     if (kMode == kTraceShadow) n = 0;
     if (kMode == kTraceClosest) n_sh = 0;
     if (n_dev) n = min(n, (int)*n_dev);
@@ -248,7 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 } else if (idx != 0xFFFFFFFFu) {
                     if (kMode == kTraceBoth) idx -= (uint32_t)n_sh;
                     Pcg unused;
-                    const QRay e = kMode == kTraceClosest ? primary_entry(prim, idx, unused) : q[idx];
+                    const QRay e = kMode == kTraceClosest ? primary_entry(prim, idx, unused)
+                                                          : q[qord ? qord[idx] : (int)idx];
                     qi = (int)idx;
                     slot = __float_as_int(e.o.w);
                     const float tmax = e.d.w;
@@ -396,7 +397,7 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, const float4* hits, int n,
                                                      const uint32_t* n_dev, int bounce, QRay* nq, uint32_t* nkeys,
                                                      uint8_t* ncont, uint32_t* blk_s, uint32_t* blk_n,
-                                                     PrimarySrc prim) {
+                                                     PrimarySrc prim, const int32_t* qord) {
     if (n_dev) n = min(n, (int)*n_dev);
     if ((int)blockIdx.x * kBlock >= n) {  // whole block past the count: zero counts for the scans
         if (threadIdx.x == 0) {
@@ -411,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     QRay e;
     Pcg g0{0u, 0u};  // bounce 0: the path's RNG after its jitter draws
     if (i < n) {
-        e = bounce == 0 ? primary_entry(prim, i, g0) : q[i];
+        e = bounce == 0 ? primary_entry(prim, i, g0) : q[qord ? qord[i] : i];
         p = __float_as_int(e.o.w);
         // bounce 0 starts every path of the pass: its radiance sum begins at 0
         if (bounce == 0 && p >= 0) {
@@ -578,13 +579,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const 
     L[0] = Lv.x; L[1] = Lv.y; L[2] = Lv.z;
 }
 
-__global__ __launch_bounds__(kBlock) void k_wf_gather(const QRay* src, const int32_t* idx, QRay* dst, int n,
-                                                      const uint32_t* n_dev) {
-    if (n_dev) n = min(n, (int)*n_dev);
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) dst[i] = src[idx[i]];
-}
-
 __global__ __launch_bounds__(kBlock) void k_wf_fill(uint32_t* p, int n, uint32_t v) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = v;
@@ -644,11 +638,12 @@ int trace_grid(int n) {
 template <int kMode>
 void launch_trace(hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
                   const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev, float4* hits, uint8_t* nee,
-                  uint32_t* cursors, int* spill, const ShadowSrc& shadow, const PrimarySrc& prim) {
+                  uint32_t* cursors, int* spill, const ShadowSrc& shadow, const PrimarySrc& prim,
+                  const int32_t* qord) {
     constexpr int kMin = kMode == kTraceShadow ? kTraceMinLanesAny : kTraceMinLanes;
     const int grid = trace_grid((kMode == kTraceShadow ? 0 : n) + (kMode == kTraceClosest ? 0 : n_sh));
     hipLaunchKernelGGL((k_wf_trace<kMode, kTraceLdsStack, kTraceTopNodes, kMin>), dim3(grid), dim3(kBlock), 0, st, tn,
-                       n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow, prim);
+                       n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow, prim, qord);
 }
 
 
@@ -758,34 +753,39 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         const PrimarySrc prim{W.P.cam, W.P.seed, W.P.sample0, W.npx, W.tiles, tiles_x, padded};
         QRay* cur = qa;
         QRay* other = qb;
+        // queue order of the bounce: entry i is cur[qord[i]] after a ray sort, cur[i] otherwise
+        const int32_t* qord = nullptr;
         const uint32_t* n_dev = nullptr;  // live entries of this bounce's queue (bounce 0: all n_max)
         if ((ms = mark(true)) != LGI_OK) return ms;
         launch_trace<kTraceClosest>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, n_dev, 0, nullptr, hits,
-                                    nullptr, counters + kCursorStride, spill, shadow, prim);
+                                    nullptr, counters + kCursorStride, spill, shadow, prim, nullptr);
         if ((ms = mark(false)) != LGI_OK) return ms;
         for (int b = 0; b <= P.bounces; ++b) {
             uint32_t* cnt = counters + kCounters * b;
             const bool last = b == P.bounces;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
-                               sort ? keys_tmp : nullptr, ncont, blk_s, blk_n, prim);
+                               sort ? keys_tmp : nullptr, ncont, blk_s, blk_n, prim, qord);
             // block offsets of the shadow-ray candidates and of the continuations
             lgi_exclusive_scan(st, blk_s, nb, cnt + 1, scan_scratch);
             lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
             hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n_max, n_dev,
                                (const uint32_t*)blk_s, sids);
             if (!last) {
-                // The next bounce's queue goes to cur (this bounce's queue is no longer read):
-                // with the sort, (key, slot) pairs of the continuations are compacted, sorted by
-                // key, and the rays gathered from shading's output in key order; without it, the
-                // rays are compacted.
+                // The next bounce's queue: with the sort, (key, slot) pairs of the continuations
+                // are compacted and sorted by key, and the next bounce reads shading's output
+                // through that slot order (qord); without it, the rays are compacted into cur
+                // (this bounce's queue is no longer read).
                 if (sort) {
                     hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, keys_tmp,
                                        (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n, keys, idx);
-                    const int32_t* sorted = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
-                    hipLaunchKernelGGL(k_wf_gather, dim3(nb), dim3(kBlock), 0, st, other, sorted, cur, n_max, cnt + 2);
+                    // the rays stay where shading wrote them: the next bounce reads them through
+                    // the sorted slot order (read before the next sort refills these buffers)
+                    qord = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
+                    std::swap(cur, other);
                 } else {
                     hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, (const uint8_t*)ncont,
                                        n_max, n_dev, (const uint32_t*)blk_n, cur);
+                    qord = nullptr;
                 }
             }
             // this bounce's shadow rays and, but for the last bounce, the next bounce's closest
@@ -794,10 +794,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             if ((ms = mark(true)) != LGI_OK) return ms;
             if (last)
                 launch_trace<kTraceShadow>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, nullptr, 0, nullptr, n_max * nl,
-                                           cnt + 1, nullptr, W.nee, cur_sh, spill, shadow, prim);
+                                           cnt + 1, nullptr, W.nee, cur_sh, spill, shadow, prim, nullptr);
             else
                 launch_trace<kTraceBoth>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2, n_max * nl,
-                                         cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim);
+                                         cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim, qord);
             if ((ms = mark(false)) != LGI_OK) return ms;
             hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
             n_dev = cnt + 2;
