@@ -103,14 +103,29 @@ struct PrimarySrc {
     int32_t tiles_x, padded;
 };
 
+// Sample s and pixel (px, py) of entry i; false for a lane outside the image.
+__device__ __forceinline__ bool primary_pixel(const PrimarySrc& ps, int64_t i, int& s, int& px, int& py) {
+    s = (int)(i / ps.padded);
+    const int j = (int)(i % ps.padded);
+    const int tile = ps.tiles ? ps.tiles[j >> 6] : j >> 6, lane = j & 63;
+    px = (tile % ps.tiles_x) * 8 + (lane & 7);
+    py = (tile / ps.tiles_x) * 8 + (lane >> 3);
+    return px < ps.cam.width && py < ps.cam.height;
+}
+
+// Path id of entry i (-1 for a lane outside the image).
+__device__ __forceinline__ int64_t primary_path(const PrimarySrc& ps, int64_t i) {
+    int s, px, py;
+    if (!primary_pixel(ps, i, s, px, py)) return -1;
+    return (int64_t)s * ps.npx + (uint32_t)py * ps.cam.width + px;
+}
+
 // Entry i's ray (o.w = path id, or -1 for a lane outside the image; d.w = tmax) and the path's
 // RNG after the two jitter draws.
 __device__ __forceinline__ QRay primary_entry(const PrimarySrc& ps, int64_t i, Pcg& g) {
-    const int s = (int)(i / ps.padded), j = (int)(i % ps.padded);
-    const int tile = ps.tiles ? ps.tiles[j >> 6] : j >> 6, lane = j & 63;
-    const int px = (tile % ps.tiles_x) * 8 + (lane & 7), py = (tile / ps.tiles_x) * 8 + (lane >> 3);
+    int s, px, py;
     QRay r;
-    if (px >= ps.cam.width || py >= ps.cam.height) {
+    if (!primary_pixel(ps, i, s, px, py)) {
         r.o = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         r.d = make_float4(0.f, 0.f, 1.f, 0.f);
         return r;
@@ -414,10 +429,6 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     if (i < n) {
         e = bounce == 0 ? primary_entry(prim, i, g0) : q[qord ? qord[i] : i];
         p = __float_as_int(e.o.w);
-        // bounce 0 starts every path of the pass: its radiance sum begins at 0
-        if (bounce == 0 && p >= 0) {
-            W.L[3 * (int64_t)p] = 0.f; W.L[3 * (int64_t)p + 1] = 0.f; W.L[3 * (int64_t)p + 2] = 0.f;
-        }
         if (p >= 0) {
             const float4 hv = hits[i];
             h.tri = __float_as_int(hv.w);
@@ -558,10 +569,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, i
 // without an unoccluded candidate would add Tb * 0 = +0 to a non-negative L, which changes no
 // bit, so it is skipped; the others find their path id in vP.w, written by the shading kernel
 // for exactly the slots that have candidates.
-__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const uint32_t* n_dev) {
+__global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const uint32_t* n_dev, int bounce,
+                                                       PrimarySrc prim) {
     if (n_dev) n = min(n, (int)*n_dev);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    // bounce 0's slots are every path of the pass: their radiance sums start here, at 0 + the
+    // unoccluded light (the same sum as adding to a zeroed L), or at 0
+    const int64_t p0 = bounce == 0 ? primary_path(prim, i) : -1;
     V3 color = v3(0.f, 0.f, 0.f);
     bool any = false;
     for (int li = 0; li < W.nl; ++li) {
@@ -571,11 +586,18 @@ __global__ __launch_bounds__(kBlock) void k_wf_resolve(WfParams W, int n, const 
             any = true;
         }
     }
-    if (!any) return;
+    if (!any) {
+        if (p0 >= 0) {
+            float* L = W.L + 3 * p0;
+            L[0] = 0.f; L[1] = 0.f; L[2] = 0.f;
+        }
+        return;
+    }
     const int p = __float_as_int(W.vP[i].w);
     float* L = W.L + 3 * (int64_t)p;
     const float4 Tb = W.vT[i];  // the path's throughput before this bounce, stored by k_wf_shade
-    V3 Lv = add(v3(L[0], L[1], L[2]), had(v3(Tb.x, Tb.y, Tb.z), color));
+    const V3 L0 = bounce == 0 ? v3(0.f, 0.f, 0.f) : v3(L[0], L[1], L[2]);
+    V3 Lv = add(L0, had(v3(Tb.x, Tb.y, Tb.z), color));
     L[0] = Lv.x; L[1] = Lv.y; L[2] = Lv.z;
 }
 
@@ -799,7 +821,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                 launch_trace<kTraceBoth>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2, n_max * nl,
                                          cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim, qord);
             if ((ms = mark(false)) != LGI_OK) return ms;
-            hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev);
+            hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev, b, prim);
             n_dev = cnt + 2;
         }
         const int n_acc = d_tiles ? n_tiles * 64 : npx;
