@@ -323,18 +323,24 @@ __global__ __launch_bounds__(kBlock) void k_ref_nodes(int n, const int32_t* pare
                                                       const uint8_t* splittable, uint32_t* out_keys,
                                                       int32_t* out_ids, uint32_t* counter) {
     const int x = blockIdx.x * kBlock + threadIdx.x;
-    if (x >= 2 * n - 1) return;
-    bool ref;
-    if (x == 0) ref = true;
-    else {
+    bool ref = x == 0;
+    if (x > 0 && x < 2 * n - 1) {
         const int p = parent[x];
         ref = (p >= 0) && splittable[p];
     }
+    // one atomic per wave for the wave's output slots: a single-address atomic from every node
+    // serialises in L2 (the output order does not matter: the BFS sort orders it)
+    const uint64_t mask = __ballot(ref);
+    if (mask == 0) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
     if (!ref) return;
     const int first = (x < n - 1) ? rfirst[x] : x - (n - 1);
     int depth = 0;
     for (int y = x; y != 0; y = parent[y]) ++depth;
-    const uint32_t slot = atomicAdd(counter, 1u);
+    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
     out_keys[slot] = ((uint32_t)depth << kDepthShift) | (uint32_t)first;
     out_ids[slot] = x;
 }
@@ -733,9 +739,8 @@ __global__ __launch_bounds__(kBlock) void k_tree_parents(const lgi_bvh_node* nod
 __global__ __launch_bounds__(kBlock) void k_tree_depth(int m, const int32_t* parent, const uint32_t* nparent,
                                                        uint32_t* depth, int32_t* slot, int32_t* info) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= m) return;
     int d = 0;
-    if (p > 0) {
+    if (p > 0 && p < m) {
         if (nparent[p] != 1u) info[1] = 1;
         int x = p;
         while (x > 0 && d < kMaxWalk) {  // depth <= kMaxWalk: level starts fit [0, kMaxWalk + 1]
@@ -747,9 +752,14 @@ __global__ __launch_bounds__(kBlock) void k_tree_depth(int m, const int32_t* par
             d = 0;
         }
     }
-    depth[p] = (uint32_t)d;
-    slot[p] = p;
-    atomicMax(&info[0], d);
+    if (p < m) {
+        depth[p] = (uint32_t)d;
+        slot[p] = p;
+    }
+    // one atomic per wave: a single-address atomic from every slot serialises in L2
+    int dm = d;
+    for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&info[0], dm);
 }
 
 __global__ __launch_bounds__(kBlock) void k_depth_starts(const uint32_t* depth_sorted, int m, int32_t* level_start) {

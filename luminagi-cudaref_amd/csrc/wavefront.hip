@@ -515,22 +515,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
     }
 }
 
-// Continuation rays of block b (flag ncont) to blk_off[b] + their rank in the block
-// (order-preserving): the next bounce's queue when the rays are not sorted.
-__global__ __launch_bounds__(kBlock) void k_wf_compact(const QRay* src, const uint8_t* flag, int n,
-                                                       const uint32_t* n_dev, const uint32_t* blk_off, QRay* dst) {
-    __shared__ uint32_t s_w[kBlock / 64];
-    if (n_dev) n = min(n, (int)*n_dev);
-    if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
-    const uint32_t r = block_excl_scan(f, s_w);
-    if (f) dst[blk_off[blockIdx.x] + r] = src[i];
-}
-
-// Sort input of the continuations of block b: their keys and source slots, to blk_off[b] + their
-// rank in the block. The rays themselves stay where shading wrote them; the gather after the
-// sort reads them from there, so each continuation ray is copied once.
+// Slots of the continuations of block b (flag ncont), and their sort keys when src_key is set,
+// to blk_off[b] + their rank in the block (order-preserving). The rays themselves stay where
+// shading wrote them: the next bounce reads them through these slots, sorted by key or, without
+// the ray sort, in slot order, so a continuation ray is never copied.
 __global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_key, const uint8_t* flag, int n,
                                                             const uint32_t* n_dev, const uint32_t* blk_off,
                                                             uint32_t* dst_key, int32_t* dst_slot) {
@@ -542,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_
     const uint32_t r = block_excl_scan(f, s_w);
     if (!f) return;
     const uint32_t o = blk_off[blockIdx.x] + r;
-    dst_key[o] = src_key[i];
+    if (src_key) dst_key[o] = src_key[i];
     dst_slot[o] = i;
 }
 
@@ -775,7 +763,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
         const PrimarySrc prim{W.P.cam, W.P.seed, W.P.sample0, W.npx, W.tiles, tiles_x, padded};
         QRay* cur = qa;
         QRay* other = qb;
-        // queue order of the bounce: entry i is cur[qord[i]] after a ray sort, cur[i] otherwise
+        // queue order of bounces after the first: entry i is cur[qord[i]]
         const int32_t* qord = nullptr;
         const uint32_t* n_dev = nullptr;  // live entries of this bounce's queue (bounce 0: all n_max)
         if ((ms = mark(true)) != LGI_OK) return ms;
@@ -793,22 +781,16 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n_max, n_dev,
                                (const uint32_t*)blk_s, sids);
             if (!last) {
-                // The next bounce's queue: with the sort, (key, slot) pairs of the continuations
-                // are compacted and sorted by key, and the next bounce reads shading's output
-                // through that slot order (qord); without it, the rays are compacted into cur
-                // (this bounce's queue is no longer read).
-                if (sort) {
-                    hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, keys_tmp,
-                                       (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n, keys, idx);
-                    // the rays stay where shading wrote them: the next bounce reads them through
-                    // the sorted slot order (read before the next sort refills these buffers)
-                    qord = lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2);
-                    std::swap(cur, other);
-                } else {
-                    hipLaunchKernelGGL(k_wf_compact, dim3(nb), dim3(kBlock), 0, st, other, (const uint8_t*)ncont,
-                                       n_max, n_dev, (const uint32_t*)blk_n, cur);
-                    qord = nullptr;
-                }
+                // The next bounce's queue is shading's output read through a slot order (qord):
+                // the continuations' slots compacted in slot order, and with the ray sort, sorted
+                // by their keys. The rays stay where shading wrote them (this bounce's queue is no
+                // longer read, so the next shading writes there); qord is read before the next
+                // compaction or sort refills its buffer.
+                hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, sort ? keys_tmp : nullptr,
+                                   (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n, sort ? keys : nullptr,
+                                   idx);
+                qord = sort ? lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2) : idx;
+                std::swap(cur, other);
             }
             // this bounce's shadow rays and, but for the last bounce, the next bounce's closest
             // hits in the same persistent launch
