@@ -28,6 +28,9 @@ HBM_PEAK_GBS = 8000.0
 # 3 x 12 B vertices per triangle test, 40 B (normals + material) per closest hit. The
 # traversal kernel's share is the node and triangle bytes; the hit bytes are read by shading.
 BYTES_NODE, BYTES_TRI, BYTES_HIT = 32, 36, 40
+# bench --mode -> renderer named in the bench line (all three give the same image bits)
+RENDERER_NAMES = {"wavefront": "wavefront renderer, no ray sort", "sorted": "wavefront renderer with the ray sort",
+                  "inline": "inline megakernel"}
 DENOISER_CKPT = ROOT / "luminagi-cudaref_amd" / "ml" / "weights" / "unet_log1p_c32.pt"
 
 
@@ -303,8 +306,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-row-step", type=int, default=2)
     ap.add_argument("--cpu-spp", type=int, default=64)
-    ap.add_argument("--mode", choices=["sorted", "inline"], default="sorted",
-                    help="sorted wavefront renderer (default) or the inline megakernel; same image bits")
+    ap.add_argument("--mode", choices=["wavefront", "sorted", "inline"], default="wavefront",
+                    help="wavefront renderer without the ray sort (default: the fastest on this frame), with "
+                         "the ray sort, or the inline megakernel; all three give the same image bits")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-40-dB measurement")
     ap.add_argument("--no-probe", action="store_true", help="skip the L2/MALL/HBM read-bandwidth probe")
@@ -365,7 +369,8 @@ def main():
     comp = torch.zeros(n_px * 3, device=dev)
     sumsq = torch.zeros(n_px, device=dev)
 
-    flags = lgi.RENDER_SORTED if args.mode == "sorted" else 0
+    flags = {"wavefront": lgi.RENDER_SORTED | lgi.RENDER_NO_RAY_SORT, "sorted": lgi.RENDER_SORTED,
+             "inline": 0}[args.mode]
     sample0 = 0
     for _ in range(args.warmup):
         ctx.accumulate(acc, comp, args.spp, args.bounces, flags=flags, sample0=sample0, d_sumsq=sumsq)
@@ -394,8 +399,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "plain path tracer (%s), 62-mesh procedural scene (%d triangles), %dx%d, %d spp "
-                               "per step, %d bounces, sun + 2 point lights, PCG32" % (args.mode, len(scene["tris"]),
-                                                                                      W, H, args.spp, args.bounces),
+                               "per step, %d bounces, sun + 2 point lights, PCG32" % (
+                                   RENDERER_NAMES[args.mode], len(scene["tris"]), W, H, args.spp, args.bounces),
                    "triangles": int(len(scene["tris"])), "width": W, "height": H, "spp_per_step": args.spp,
                    "bounces": args.bounces, "scene_seed": args.seed, "renderer": args.mode},
         "render_ms": round(avg_kernel_ms, 3),
@@ -431,13 +436,16 @@ def main():
                                 "write_factor": round(p["write_factor"], 4),
                                 "factors_from": "a 1 GiB device copy: FETCH_SIZE reads half the bytes moved, "
                                                 "WRITE_SIZE all of them"}
+                if "pipes" in p:
+                    # what does bound the traversal: VALU issue and the vector-memory data path
+                    counters_src["pipes_all_launches"] = p["pipes"]["per_launch_kind"]["all"]
         hbm_gbs = traffic / (avg_trace_ms * 1e-3) / 1e9 if traffic else None
         result["roofline"] = {"bound": "hbm", "achieved": round(hbm_gbs, 1) if hbm_gbs else None,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(hbm_gbs / HBM_PEAK_GBS, 4) if hbm_gbs else None, "traffic": traffic,
                               "achieved_from": "HBM-side bytes per frame (calibrated FETCH_SIZE + WRITE_SIZE of the "
                                                "traversal launches) over their live summed launch time",
-                              "kernel": "k_wf_trace (all traversal launches of one frame)" if args.mode == "sorted"
+                              "kernel": "k_wf_trace (all traversal launches of one frame)" if args.mode != "inline"
                               else "k_render",
                               "algorithmic": {"bytes_per_frame": int(algo_bytes), "achieved": round(algo_gbs, 1),
                                               "unit": "GB/s", "trace_bytes_per_sample": round(trace_bps, 1),
