@@ -134,47 +134,91 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int
     counts[(size_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: within a tile, element order is (round j, thread); ranks among equal
-// digits come from a 64-lane match built from 8 ballots plus per-wave digit counts.
+// Stable scatter of one kTile-key tile. Wave w owns the tile's keys [w * 1024, (w + 1) * 1024),
+// 64 at a time in index order, and ranks each key among the wave's earlier keys of its digit:
+// a 64-lane match from 8 ballots plus a per-wave running count per digit, with no block barrier
+// inside the loop. The tile is then staged in LDS in digit order (digit start in the tile + keys
+// of that digit in earlier waves + rank) and written out by consecutive threads to consecutive
+// positions of each digit's output run, so the global stores coalesce. The result is the stable
+// order: equal digits keep their index order.
+constexpr int kWaveKeys = kTile / (kBlock / 64);
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int32_t* vin,
                                                           uint32_t* kout, int32_t* vout, int n, const uint32_t* n_dev,
                                                           int shift, const uint32_t* offs, int nb) {
-    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_wc[kBlock / 64][256];  // per wave: keys of each digit so far, then its prefix
+    __shared__ uint32_t s_start[256];            // digit start in the tile
+    __shared__ uint32_t s_key[kTile];
+    __shared__ int32_t s_val[kTile];
     if (n_dev) n = min(n, (int)*n_dev);
-    if (blockIdx.x * kTile >= n) return;  // whole block past the device-side count
-    __shared__ uint32_t s_wc[kBlock / 64][256];
+    const int base = blockIdx.x * kTile;
+    if (base >= n) return;  // whole block past the device-side count
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    s_base[tid] = offs[(size_t)tid * nb + blockIdx.x];
     for (int w = 0; w < kBlock / 64; ++w) s_wc[w][tid] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kTile;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t key[kItems], rank[kItems];
+    int32_t val[kItems];
     for (int j = 0; j < kItems; ++j) {
-        const int i = base + j * kBlock + tid;
+        const int i = base + wave * kWaveKeys + j * 64 + lane;
         const bool valid = i < n;
-        const uint32_t key = valid ? kin[i] : 0xFFFFFFFFu;
-        const uint32_t d = (key >> shift) & 255u;
-        uint64_t peers = ~0ull;
+        key[j] = valid ? kin[i] : 0u;
+        val[j] = valid ? vin[i] : 0;
+        const uint32_t d = (key[j] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if ((__ffsll((long long)peers) - 1) == lane) s_wc[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int w = 0; w < wave; ++w) pre += s_wc[w][d];
-        if (valid) {
-            const uint32_t pos = s_base[d] + pre + rank;
-            kout[pos] = key;
-            vout[pos] = vin[i];
+        // the wave's earlier keys of digit d, then this round's lower lanes of digit d
+        const uint32_t before = s_wc[wave][d];
+        rank[j] = before + (uint32_t)__popcll(peers & lt_mask);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (__ffsll((long long)peers) - 1) == lane) s_wc[wave][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: total in the tile and each wave's prefix; digit starts by a block scan
+    uint32_t tot = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t c = s_wc[w][tid];
+        s_wc[w][tid] = tot;
+        tot += c;
+    }
+    {
+        uint32_t x = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
         }
+        s_start[tid] = x;  // inclusive within the wave for now
+    }
+    __syncthreads();
+    {
+        uint32_t pre = 0;
+        for (int w = 0; w < wave; ++w) pre += s_start[w * 64 + 63];
+        const uint32_t excl = pre + s_start[tid] - tot;
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < kBlock / 64; ++w) { add += s_wc[w][tid]; s_wc[w][tid] = 0; }
-        s_base[tid] += add;
-        __syncthreads();
+        s_start[tid] = excl;
+    }
+    __syncthreads();
+    for (int j = 0; j < kItems; ++j) {
+        const int i = base + wave * kWaveKeys + j * 64 + lane;
+        if (i < n) {
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint32_t L = s_start[d] + s_wc[wave][d] + rank[j];
+            s_key[L] = key[j];
+            s_val[L] = val[j];
+        }
+    }
+    __syncthreads();
+    const int n_tile = min(kTile, n - base);
+    for (int L = tid; L < n_tile; L += kBlock) {
+        const uint32_t k = s_key[L];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = offs[(size_t)d * nb + blockIdx.x] + ((uint32_t)L - s_start[d]);
+        kout[pos] = k;
+        vout[pos] = s_val[L];
     }
 }
 
