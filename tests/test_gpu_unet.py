@@ -10,7 +10,7 @@ from oracle import unet_ref
 
 pytestmark = pytest.mark.gpu
 
-BF16_TOL = 2e-2   # max |err| / max |ref| for BF16 outputs of FP32-accumulated sums
+BF16_TOL = 5e-3   # max |err| / max |ref| for BF16 outputs of FP32-accumulated sums: 2.5x the 2^-9 output rounding
 
 
 def rel(a, b):
