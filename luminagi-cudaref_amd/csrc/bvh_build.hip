@@ -63,23 +63,37 @@ __device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t&
     return wpre + x - v;
 }
 
-// In-place exclusive scan of one kTile chunk per block; block totals to sums.
+// In-place exclusive scan of one kTile chunk per block; block totals to sums. A thread's kItems
+// consecutive values move as 16-byte vectors (data is 16-byte aligned) unless they straddle n.
 __global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t* data, int n, uint32_t* sums) {
     __shared__ uint32_t s_wave[kBlock / 64];
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    const bool whole = base + kItems <= n;
     uint32_t v[kItems];
     uint32_t acc = 0;
-    for (int k = 0; k < kItems; ++k) {
-        int i = base + k;
-        v[k] = (i < n) ? data[i] : 0u;
-        acc += v[k];
+    if (whole) {
+        const uint4* src = reinterpret_cast<const uint4*>(data + base);
+        for (int q = 0; q < kItems / 4; ++q) {
+            const uint4 x = src[q];
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    } else {
+        for (int k = 0; k < kItems; ++k) v[k] = (base + k < n) ? data[base + k] : 0u;
     }
+    for (int k = 0; k < kItems; ++k) acc += v[k];
     uint32_t total;
     uint32_t pre = block_exclusive_scan(acc, s_wave, total);
     for (int k = 0; k < kItems; ++k) {
-        int i = base + k;
-        if (i < n) data[i] = pre;
-        pre += v[k];
+        const uint32_t x = v[k];
+        v[k] = pre;
+        pre += x;
+    }
+    if (whole) {
+        uint4* dst = reinterpret_cast<uint4*>(data + base);
+        for (int q = 0; q < kItems / 4; ++q) dst[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+        for (int k = 0; k < kItems; ++k)
+            if (base + k < n) data[base + k] = v[k];
     }
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
@@ -112,6 +126,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_single(uint32_t* data, int n, u
 __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* data, int n, const uint32_t* sums) {
     const uint32_t add = sums[blockIdx.x];
     const int base = blockIdx.x * kTile + threadIdx.x * kItems;
+    if (base + kItems <= n) {
+        uint4* d = reinterpret_cast<uint4*>(data + base);
+        for (int q = 0; q < kItems / 4; ++q) {
+            const uint4 x = d[q];
+            d[q] = make_uint4(x.x + add, x.y + add, x.z + add, x.w + add);
+        }
+        return;
+    }
     for (int k = 0; k < kItems; ++k) {
         int i = base + k;
         if (i < n) data[i] += add;
@@ -144,9 +166,11 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* keys, int
 constexpr int kWaveKeys = kTile / (kBlock / 64);
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, const int32_t* vin,
                                                           uint32_t* kout, int32_t* vout, int n, const uint32_t* n_dev,
-                                                          int shift, const uint32_t* offs, int nb) {
+                                                          int shift, const uint32_t* offs, const uint32_t* offs_tiles,
+                                                          int nb) {
     __shared__ uint32_t s_wc[kBlock / 64][256];  // per wave: keys of each digit so far, then its prefix
     __shared__ uint32_t s_start[256];            // digit start in the tile
+    __shared__ uint32_t s_out[256];              // digit's first output position
     __shared__ uint32_t s_key[kTile];
     __shared__ int32_t s_val[kTile];
     if (n_dev) n = min(n, (int)*n_dev);
@@ -154,6 +178,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, c
     if (base >= n) return;  // whole block past the device-side count
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int w = 0; w < kBlock / 64; ++w) s_wc[w][tid] = 0;
+    {
+        // offs holds each digit-major count's offset within its kTile chunk of counts, offs_tiles
+        // the chunks' exclusive prefix: their sum is the global output position
+        const size_t c = (size_t)tid * nb + blockIdx.x;
+        s_out[tid] = offs[c] + offs_tiles[c / kTile];
+    }
     __syncthreads();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t key[kItems], rank[kItems];
@@ -216,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* kin, c
     for (int L = tid; L < n_tile; L += kBlock) {
         const uint32_t k = s_key[L];
         const uint32_t d = (k >> shift) & 255u;
-        const uint32_t pos = offs[(size_t)d * nb + blockIdx.x] + ((uint32_t)L - s_start[d]);
+        const uint32_t pos = s_out[d] + ((uint32_t)L - s_start[d]);
         kout[pos] = k;
         vout[pos] = s_val[L];
     }
@@ -908,11 +938,12 @@ int radix_sort(hipStream_t st, SortBufs& b, int n, int bits, const uint32_t* n_d
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, st, b.k[cur], n, n_dev, shift, b.counts, nb);
+        // digit offsets: scanned within kTile chunks of the counts plus the chunks' scanned totals,
+        // added by the scatter itself
         hipLaunchKernelGGL(k_scan_tiles, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
         hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, b.sums, nsb, (uint32_t*)nullptr);
-        hipLaunchKernelGGL(k_scan_add, dim3(nsb), dim3(kBlock), 0, st, b.counts, ncount, b.sums);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, st, b.k[cur], b.v[cur], b.k[cur ^ 1],
-                           b.v[cur ^ 1], n, n_dev, shift, b.counts, nb);
+                           b.v[cur ^ 1], n, n_dev, shift, b.counts, b.sums, nb);
         cur ^= 1;
     }
     return cur;
@@ -928,10 +959,14 @@ lgi_status dev_alloc(lgi_ctx* ctx, T** p, size_t count) {
 
 }  // namespace
 
+// Scratch of lgi_sort_pairs: the second key and value buffers, the digit counts and their chunk
+// totals, each starting on a 256-byte boundary (the scans move the counts as 16-byte vectors).
+static size_t sort_part(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
 size_t lgi_sort_scratch_bytes(int64_t n) {
     const int nb = ceil_div(n, kTile);
     const size_t ncount = (size_t)256 * nb;
-    return 4 * (4 * (size_t)n + ncount + ceil_div(ncount, kTile) + 64);
+    return 2 * sort_part(4 * (size_t)n) + sort_part(4 * ncount) + sort_part(4 * ((size_t)ceil_div(ncount, kTile) + 64));
 }
 
 const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int n, int bits, void* scratch,
@@ -942,9 +977,9 @@ const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int
     const size_t ncount = (size_t)256 * nb;
     b.k[0] = keys;
     b.v[0] = vals;
-    b.k[1] = (uint32_t*)p; p += 4 * (size_t)n;
-    b.v[1] = (int32_t*)p; p += 4 * (size_t)n;
-    b.counts = (uint32_t*)p; p += 4 * ncount;
+    b.k[1] = (uint32_t*)p; p += sort_part(4 * (size_t)n);
+    b.v[1] = (int32_t*)p; p += sort_part(4 * (size_t)n);
+    b.counts = (uint32_t*)p; p += sort_part(4 * ncount);
     b.sums = (uint32_t*)p;
     return b.v[radix_sort(st, b, n, bits, n_dev)];
 }
