@@ -986,6 +986,13 @@ const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int
 
 size_t lgi_scan_scratch_bytes(int64_t n) { return 4 * ((size_t)ceil_div(n, kTile) + 64); }
 
+void lgi_exclusive_scan_chunks(hipStream_t st, uint32_t* data, int n, uint32_t* total, uint32_t* chunk_sums) {
+    static_assert(kTile == kScanChunk, "chunk of the two-part scan");
+    const int nb = ceil_div(n, kTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(std::max(1, nb)), dim3(kBlock), 0, st, data, n, chunk_sums);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, st, chunk_sums, nb, total);
+}
+
 void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, void* scratch) {
     const int nb = ceil_div(n, kTile);
     uint32_t* sums = (uint32_t*)scratch;

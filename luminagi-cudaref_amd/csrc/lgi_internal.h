@@ -138,6 +138,12 @@ const int32_t* lgi_sort_pairs(hipStream_t st, uint32_t* keys, int32_t* vals, int
 // In-place exclusive scan of n u32 values; *total (device) receives the sum.
 size_t lgi_scan_scratch_bytes(int64_t n);
 void lgi_exclusive_scan(hipStream_t st, uint32_t* data, int n, uint32_t* total, void* scratch);
+// The same scan in two parts that a consumer adds itself: data[i] becomes the exclusive prefix
+// within its kScanChunk-value chunk and chunk_sums[i / kScanChunk] the prefix of the chunks, so
+// the full prefix is data[i] + chunk_sums[i / kScanChunk] (chunk_sums: lgi_scan_scratch_bytes).
+// data is 16-byte aligned.
+constexpr int kScanChunk = 4096;
+void lgi_exclusive_scan_chunks(hipStream_t st, uint32_t* data, int n, uint32_t* total, uint32_t* chunk_sums);
 void lgi_free_dev_scene(DevScene& d);
 DevCamera lgi_make_dev_camera(const lgi_camera& c);
 lgi_status lgi_upload_lights(lgi_ctx* ctx, DevLights* out);

@@ -516,12 +516,14 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
 }
 
 // Slots of the continuations of block b (flag ncont), and their sort keys when src_key is set,
-// to blk_off[b] + their rank in the block (order-preserving). The rays themselves stay where
+// to the block's offset + their rank in the block (order-preserving). The block's offset is
+// blk_off[b] + blk_chunk[b / kScanChunk] (lgi_exclusive_scan_chunks). The rays themselves stay where
 // shading wrote them: the next bounce reads them through these slots, sorted by key or, without
 // the ray sort, in slot order, so a continuation ray is never copied.
 __global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_key, const uint8_t* flag, int n,
                                                             const uint32_t* n_dev, const uint32_t* blk_off,
-                                                            uint32_t* dst_key, int32_t* dst_slot) {
+                                                            const uint32_t* blk_chunk, uint32_t* dst_key,
+                                                            int32_t* dst_slot) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
     if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
@@ -529,15 +531,16 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_keys(const uint32_t* src_
     const uint32_t f = (i < n && flag[i]) ? 1u : 0u;
     const uint32_t r = block_excl_scan(f, s_w);
     if (!f) return;
-    const uint32_t o = blk_off[blockIdx.x] + r;
+    const uint32_t o = blk_off[blockIdx.x] + blk_chunk[blockIdx.x / kScanChunk] + r;
     if (src_key) dst_key[o] = src_key[i];
     dst_slot[o] = i;
 }
 
-// NEE candidate ids j = slot * nl + light of block b to blk_off[b] + their rank in the block
-// (slot-major, lights in order).
+// NEE candidate ids j = slot * nl + light of block b to the block's offset (as above) + their rank
+// in the block (slot-major, lights in order).
 __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, int nl, int n, const uint32_t* n_dev,
-                                                           const uint32_t* blk_off, uint32_t* ids) {
+                                                           const uint32_t* blk_off, const uint32_t* blk_chunk,
+                                                           uint32_t* ids) {
     __shared__ uint32_t s_w[kBlock / 64];
     if (n_dev) n = min(n, (int)*n_dev);
     if ((int)blockIdx.x * kBlock >= n) return;  // whole block past the count (block-uniform)
@@ -545,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_compact_ids(const uint8_t* nee, i
     uint32_t c = 0;
     if (i < n)
         for (int li = 0; li < nl; ++li) c += nee[(int64_t)i * nl + li] == 1 ? 1u : 0u;
-    uint32_t o = blk_off[blockIdx.x] + block_excl_scan(c, s_w);
+    uint32_t o = blk_off[blockIdx.x] + blk_chunk[blockIdx.x / kScanChunk] + block_excl_scan(c, s_w);
     if (!c) return;
     for (int li = 0; li < nl; ++li) {
         const int64_t j = (int64_t)i * nl + li;
@@ -709,7 +712,12 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     uint32_t* keys = (uint32_t*)S_(54, cap * 4);
     int32_t* idx = (int32_t*)S_(55, cap * 4);
     void* sort_scratch = S_(56, lgi_sort_scratch_bytes(cap));
-    void* scan_scratch = S_(68, lgi_scan_scratch_bytes(nblk));
+    // chunk sums of the two block-count scans (candidates, continuations), kept until the
+    // compaction kernels add them
+    const size_t scan_words = (lgi_scan_scratch_bytes(nblk) + 255) / 256 * 64;
+    uint32_t* scan_scratch = (uint32_t*)S_(68, 8 * scan_words);
+    uint32_t* sums_s = scan_scratch;
+    uint32_t* sums_n = scan_scratch + scan_words;
     // per bounce: [1] shadow rays, [2] continuations, then the closest-hit and the shadow-ray
     // traversal's segment cursors, kCursorStride apart
     constexpr int kCounters = kCursorStride * (1 + 2 * kSegments);
@@ -736,7 +744,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     ShadowSrc shadow{sids, W.vP, W.vN, nl, P};
     for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)blk_s, (void*)blk_n, (void*)ncont,
                     (void*)keys_tmp,
-                    (void*)keys, (void*)idx, sort_scratch, scan_scratch, (void*)counters, (void*)W.T,
+                    (void*)keys, (void*)idx, sort_scratch, (void*)scan_scratch, (void*)counters, (void*)W.T,
                     (void*)W.vT, (void*)W.L, (void*)W.contrib, (void*)W.nee, (void*)W.vP, (void*)W.vN})
         if (!p) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
     const int n_tn = (int)ctx->dev.n_tnodes;
@@ -775,11 +783,12 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             const bool last = b == P.bounces;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
                                sort ? keys_tmp : nullptr, ncont, blk_s, blk_n, prim, qord);
-            // block offsets of the shadow-ray candidates and of the continuations
-            lgi_exclusive_scan(st, blk_s, nb, cnt + 1, scan_scratch);
-            lgi_exclusive_scan(st, blk_n, nb, cnt + 2, scan_scratch);
+            // block offsets of the shadow-ray candidates and (but for the last bounce) of the
+            // continuations, in two parts that the compaction kernels add
+            lgi_exclusive_scan_chunks(st, blk_s, nb, cnt + 1, sums_s);
+            if (!last) lgi_exclusive_scan_chunks(st, blk_n, nb, cnt + 2, sums_n);
             hipLaunchKernelGGL(k_wf_compact_ids, dim3(nb), dim3(kBlock), 0, st, (const uint8_t*)W.nee, nl, n_max, n_dev,
-                               (const uint32_t*)blk_s, sids);
+                               (const uint32_t*)blk_s, (const uint32_t*)sums_s, sids);
             if (!last) {
                 // The next bounce's queue is shading's output read through a slot order (qord):
                 // the continuations' slots compacted in slot order, and with the ray sort, sorted
@@ -787,8 +796,8 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
                 // longer read, so the next shading writes there); qord is read before the next
                 // compaction or sort refills its buffer.
                 hipLaunchKernelGGL(k_wf_compact_keys, dim3(nb), dim3(kBlock), 0, st, sort ? keys_tmp : nullptr,
-                                   (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n, sort ? keys : nullptr,
-                                   idx);
+                                   (const uint8_t*)ncont, n_max, n_dev, (const uint32_t*)blk_n,
+                                   (const uint32_t*)sums_n, sort ? keys : nullptr, idx);
                 qord = sort ? lgi_sort_pairs(st, keys, idx, n_max, kKeySortBits, sort_scratch, cnt + 2) : idx;
                 std::swap(cur, other);
             }
