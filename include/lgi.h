@@ -155,7 +155,7 @@ LGI_API lgi_status lgi_bvh_node_count(const lgi_ctx* ctx, int64_t* n_nodes);
  * triangle now at position i; keys: sorted 30-bit Morton codes. Any output may be null. */
 LGI_API lgi_status lgi_bvh_get(lgi_ctx* ctx, lgi_bvh_node* nodes, int32_t* tri_perm, uint32_t* keys);
 /* Traversal tree (no reference counterpart). The tracers traverse a copy of the reference tree
- * whose topology above the leaves is restructured to lower its surface-area cost (7-leaf
+ * whose topology above the leaves is restructured to lower its surface-area cost (5-leaf
  * treelets, Karras & Aila 2013): same leaves and triangle order, every box the exact union of
  * its children's, so the closest hit equals the reference tree's except where visit order
  * decides it (exact distance ties, boxes grazed within rounding). passes 0..8 (default 2; 0 =

@@ -497,8 +497,8 @@ __global__ __launch_bounds__(kBlock) void k_level_starts(const uint32_t* bfs_key
 // rebuilt treelet by treelet to lower its surface-area cost, after Karras and Aila, "Fast
 // Parallel Construction of High-Quality Bounding Volume Hierarchies" (HPG 2013): a treelet
 // is a node plus descendants, grown by repeatedly opening the treelet leaf of largest surface
-// area until it has 7 leaves; dynamic programming over the 127 leaf subsets finds the binary
-// tree over those leaves of least cost, C(S) = area(S) * kSahNode + min over splits of
+// area until it has kTreeletLeaves leaves; dynamic programming over the leaf subsets finds the
+// binary tree over those leaves of least cost, C(S) = area(S) * kSahNode + min over splits of
 // C(P) + C(S \ P), and the treelet's internal node slots are rewired to it when it is cheaper.
 // Leaves (triangle ranges) never change, and every box stays the exact FP32 union of its
 // children's, so a ray enters a leaf's ancestors whenever it enters the leaf, and the closest
@@ -513,7 +513,10 @@ __global__ __launch_bounds__(kBlock) void k_level_starts(const uint32_t* bfs_key
 // ran 4x slower: every step up needs an agent-scope release and acquire, which write back and
 // invalidate the L2 of the XCD.)
 constexpr float kSahNode = 1.2f, kSahTri = 1.0f;
-constexpr int kTreeletLeaves = 7;
+// 5 leaves (31 subsets): on the 1.05M-triangle frame two passes traverse as fast as with 7-leaf
+// treelets (127 subsets; 19.24-19.27 vs 19.27-19.32 ms) and restructure in 1.90 instead of 2.49 ms.
+// 6 leaves: 19.18-19.24 ms, 2.21 ms; 4 leaves: 19.50 ms, 1.68 ms; one 5-leaf pass: 19.56 ms, 1.00 ms.
+constexpr int kTreeletLeaves = 5;
 constexpr int kSubsets = 1 << kTreeletLeaves;
 
 __device__ __forceinline__ float half_area(const float* mn, const float* mx) {
@@ -522,14 +525,15 @@ __device__ __forceinline__ float half_area(const float* mn, const float* mx) {
 }
 
 // A treelet as its root's lane formed it: the internal slots (in[0] = the root) and the
-// leaves, each leaf's box and subtree cost, and the root's refreshed cost. 65 words, so that
-// the 64 lanes' records start in different LDS banks.
+// leaves, each leaf's box and subtree cost, and the root's refreshed cost. An odd number of
+// words, so that the 64 lanes' records start in different LDS banks.
 struct TreeletForm {
     int leaf[kTreeletLeaves], in[kTreeletLeaves - 1], n;
     float root_cost;
     float mn[kTreeletLeaves][3], mx[kTreeletLeaves][3], c[kTreeletLeaves];
     int pad;
 };
+static_assert(sizeof(TreeletForm) / 4 % 2 == 1, "odd word count");
 // Working state while a treelet is formed: the root's subtree to depth 3, prefetched by the
 // wave in heap order (slot 1 = the root, slots 2h and 2h+1 = the children of slot h; valid only
 // below an internal node), and per treelet leaf its children, area, whether it is internal,
@@ -549,7 +553,7 @@ struct TreeletDp {
     uint8_t part[kSubsets];
 };
 
-// The subsets of 7 treelet leaves ordered by size: sub[start[s] + i] is the i-th of size s.
+// The subsets of the treelet leaves ordered by size: sub[start[s] + i] is the i-th of size s.
 struct SubsetsBySize {
     uint8_t sub[kSubsets];
     uint8_t start[kTreeletLeaves + 2];
@@ -590,7 +594,7 @@ __device__ __forceinline__ void treelet_leaf(TreeletForm& F, TreeletGrow& G, int
 // lanes of the wave call it. The wave first loads p's subtree to depth 3 with the subtree
 // costs, one level per round, so that most treelets form from LDS. Lane 0 then refreshes p's
 // cost and opens the internal treelet leaf of largest area (the first on ties) until there
-// are 7 leaves, loading children from memory only below the prefetched levels.
+// are kTreeletLeaves leaves, loading children from memory only below the prefetched levels.
 __device__ void treelet_form(const lgi_bvh_node* nodes, float* cost, int p, TreeletForm& F, TreeletGrow& G) {
     const int lane = threadIdx.x;
     if (lane == 0) G.node[1] = nodes[p];

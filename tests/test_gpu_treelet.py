@@ -1,4 +1,4 @@
-"""The traversal tree: the reference LBVH restructured above its leaves (7-leaf treelets, after
+"""The traversal tree: the reference LBVH restructured above its leaves (5-leaf treelets, after
 Karras and Aila 2013). Checks it against the reference node array (same leaves in the same
 slots, a proper tree, every box the exact union of its children's, depth within the traversal
 stacks), its surface-area cost against the reference tree's, closest hits through it (the
