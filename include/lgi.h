@@ -106,6 +106,11 @@ typedef struct lgi_batch_job {       /* BatchJob, src/main.cpp:63-69 */
 #define LGI_RENDER_NO_RAY_SORT 8u   /* with LGI_RENDER_SORTED: no key sort between bounces, the
                                        continuations keep their compaction order (same image bits;
                                        no reference counterpart) */
+#define LGI_RENDER_SET_SUMS   16u   /* lgi_render_accumulate(_noalloc): the call's sums replace the
+                                       contents of d_sum, d_comp and d_sumsq (as if they held
+                                       zeros) instead of adding to them, so that a frame needs no
+                                       clearing pass; same bits as zeroed buffers. The batch and
+                                       tile renders reject it (no reference counterpart) */
 
 /* ---- context ---- */
 LGI_API lgi_status  lgi_create(lgi_ctx** out, int device, void* hip_stream);
@@ -316,6 +321,25 @@ LGI_API lgi_status lgi_unet_conv3x3_fp8_fwd(const void* src0, int32_t c0, const 
                                             const void* w_e4m3, const float* w_scale, float act_scale,
                                             const float* bias, int32_t cout, int32_t n, int32_t h, int32_t w_,
                                             int32_t relu, void* out, void* pool_out, void* stream);
+/* Decoder convolution for inference on the upsample it follows (ml/train.py:62-64, 71-73:
+ * Upsample(2x, bilinear) + cat + Conv2d): out (n, 2h, 2w, cout) = relu?(conv3x3(cat(up2(lo), skip))
+ * + bias) without the upsampled tensor. lo (n, h, w, cu) is the upsample's input, skip
+ * (n, 2h, 2w, cs) (may be null with cs 0), w [cout][9][cu + cs] BF16 the layer's weights and
+ * w_phase [4][cout][9][cu] BF16 the weights lgi_unet_up_phase_weights derives from them: the
+ * convolution of the upsampled image equals, at each of the four output phases (row, column
+ * parity), a 3x3 convolution of lo with mixed weights. Pixels whose window reaches the image
+ * border are computed from the upsampled values themselves. Equals lgi_unet_upsample_fwd then
+ * lgi_unet_conv3x3_fwd up to BF16 rounding (of the mixed weights instead of the upsampled
+ * activations) and FP32 summation order. cu, cs, cout multiples of 32. */
+LGI_API lgi_status lgi_unet_conv3x3_up_fwd(const void* lo, int32_t cu, const void* skip, int32_t cs,
+                                           const void* w_phase, const void* w, const float* bias, int32_t cout,
+                                           int32_t n, int32_t h, int32_t w_, int32_t relu, void* out, void* stream);
+/* w_phase [4][cout][9][cu] BF16 of lgi_unet_conv3x3_up_fwd from the FP32 weights w [cout][9][cin]
+ * (their first cu input channels): for phase 2 py + px, tap 3 dy + dx,
+ * w_phase = sum_{ky,kx} M_py[dy][ky] M_px[dx][kx] w[3 ky + kx], M_0 = [[.75,.25,0],[.25,.75,.75],
+ * [0,0,.25]], M_1 = [[.25,0,0],[.75,.75,.25],[0,.25,.75]], summed in FP32. */
+LGI_API lgi_status lgi_unet_up_phase_weights(const float* w, int32_t cout, int32_t cin, int32_t cu, void* w_phase,
+                                             void* stream);
 /* First layer for inference: y = relu?(conv3x3(x) + bias) with x the FP32 3-channel NHWC
  * input (read directly, no padded copy), w [32][9][cin_w] BF16 of which channels 0..2 are used,
  * y (n, h, w, 32) BF16. Equals lgi_unet_conv3x3_fwd on lgi_unet_pack_input(x) up to FP32

@@ -708,6 +708,8 @@ lgi_status lgi_render_batch(lgi_ctx* ctx, const lgi_pose_job* jobs, int32_t n, i
     if (st != LGI_OK) return st;
     if (flags & (LGI_RENDER_SORTED | LGI_RNG_XORSHIFT_REF))
         return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "the batch queue renders the PCG32 inline path");
+    if (flags & LGI_RENDER_SET_SUMS)
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "the batch queue adds its items' samples to the sums");
     const int tiles = ((cam0.width + 7) / 8) * ((cam0.height + 7) / 8);
     std::vector<DevCamera> cams(n);
     std::vector<lgi_work_item> items;
@@ -734,6 +736,8 @@ lgi_status lgi_render_tiles(lgi_ctx* ctx, const lgi_camera* cam_in, const int32_
     const lgi_camera& cam = cam_in ? *cam_in : ctx->scene.camera;
     lgi_status st = check_render_ready(ctx, cam);
     if (st != LGI_OK) return st;
+    if (flags & LGI_RENDER_SET_SUMS)
+        return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "tile renders add samples to the sums");
     if (flags & LGI_RNG_XORSHIFT_REF)
         return lgi_set_err(ctx, LGI_ERR_UNSUPPORTED, "tile renders use PCG32 per-sample streams");
     const int ntiles = ((cam.width + 7) / 8) * ((cam.height + 7) / 8);

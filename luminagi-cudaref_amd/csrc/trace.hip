@@ -144,9 +144,11 @@ __device__ __forceinline__ void tile_pixel(int bx, int by, int tid, int& px, int
 __device__ void render_pixel(const TraceParams& P, const DevCamera& cam, int px, int py, int spp, int sample0,
                              float* sum, float* comp, float* sumsq, const TravStack& stk) {
     const uint32_t pix = (uint32_t)py * (uint32_t)cam.width + (uint32_t)px;
-    float s0 = sum[3 * pix], s1 = sum[3 * pix + 1], s2 = sum[3 * pix + 2];
-    float c0 = comp[3 * pix], c1 = comp[3 * pix + 1], c2 = comp[3 * pix + 2];
-    float sq = sumsq ? sumsq[pix] : 0.0f;
+    // LGI_RENDER_SET_SUMS: start from zero instead of the buffers' contents
+    const bool add = (P.flags & LGI_RENDER_SET_SUMS) == 0;
+    float s0 = add ? sum[3 * pix] : 0.f, s1 = add ? sum[3 * pix + 1] : 0.f, s2 = add ? sum[3 * pix + 2] : 0.f;
+    float c0 = add ? comp[3 * pix] : 0.f, c1 = add ? comp[3 * pix + 1] : 0.f, c2 = add ? comp[3 * pix + 2] : 0.f;
+    float sq = (add && sumsq) ? sumsq[pix] : 0.0f;
     const bool xs = (P.flags & LGI_RNG_XORSHIFT_REF) != 0;
     Rng rng;
     rng.pcg = !xs;

@@ -18,6 +18,7 @@
 // the gather of the data-gradient pass.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <atomic>
 #include <cstdint>
@@ -1254,6 +1255,412 @@ __global__ __launch_bounds__(kBlock) void k_tail_reduce(const float* partial, in
     }
 }
 
+// ------------------------------------------------------------------ upsample + concat + conv
+// Inference form of a decoder level's first convolution, relu(conv3x3(cat(up2(lo), skip)) + b),
+// that never writes the upsampled tensor. Bilinear 2x upsampling (align_corners = False, the
+// source index clamped into the image) gives, along each axis, up[2i] = .25 x[i-1] + .75 x[i]
+// and up[2i+1] = .75 x[i] + .25 x[i+1], with x replicated past its edges. So at an output pixel
+// of phase (py, px) = (Y & 1, X & 1) the 3x3 convolution of up equals a 3x3 convolution of x
+// around (Y >> 1, X >> 1) whose weights mix the original taps:
+//   w_p[dy][dx] = sum_{ky,kx} M_py[dy][ky] M_px[dx][kx] w[ky][kx],
+//   M_0 = [[.75, .25, 0], [.25, .75, .75], [0, 0, .25]], M_1 = [[.25, 0, 0], [.75, .75, .25], [0, .25, .75]]
+// (row: low-resolution tap -1, 0, +1; column: full-resolution tap -1, 0, +1). The identity holds
+// wherever the 3x3 window stays inside the upsampled image. On the outermost ring of output
+// pixels the window reaches the convolution's zero padding, which the phase form would read as
+// replicated values, so the ring is computed from explicitly upsampled neighbourhoods.
+//
+// Tiles: 8 low-resolution rows x 32 columns = 16 x 64 output pixels x 32 output channels. Each
+// of the kUpNW waves owns kUpRW low-resolution rows and the four phases of their output rows
+// (4 kUpRW 32x32 accumulators; an MFMA covers 32 pixels of one phase along a row). Per 16-channel chunk of lo
+// the block stages the 10 x 34 low-resolution halo (replicated at the edges) and the four phase
+// weight slices; per chunk of skip the 18 x 66 full-resolution halo (zero padded), even and odd
+// columns in separate planes so that the 32 same-phase pixels of an A fragment (every other
+// column) are consecutive in LDS, and the plain weight slice. A wave reads every distinct
+// fragment of a chunk once: lo, per kernel column, kUpRW + 2 halo rows and 12 phase-weight
+// fragments; skip, the 9 weight fragments and 4 (2 kUpRW + 2) halo fragments (output row + kernel
+// row and output column parity + kernel column coincide across phases) for 36 kUpRW MFMAs.
+// The grid is persistent, one block per CU: a block walks its tiles as one stream of chunks in
+// two LDS buffers, loaded by direct global-to-LDS loads one chunk ahead of the MFMAs, also
+// across tile boundaries. A tile's epilogue stages its output through the buffer of its last
+// chunk in two halves and stores 16 bytes per thread; pixels on the ring are not stored.
+// Ring work: each wave computes a strip of 32 ring pixels (part of the top or bottom row, or of
+// the left or right column without its corners) as one 32x32 MFMA tile over an explicitly
+// upsampled 3 x 34 neighbourhood (k_up_fwd's arithmetic, BF16, zero outside the image) with the
+// original weights. Groups of 4 strips go to the blocks with the fewest tiles, before their
+// tiles.
+// LDS slots are 16 bytes (8 channels); a pixel of a 16-channel chunk has two, XOR-swizzled by bit
+// 3 of the pixel index, and a weight row (phase, output channel) 18, its two parts per tap
+// swizzled by bit 3 of the row, so that the 32-lane fragment reads hit distinct banks.
+constexpr int kUpNW = 8, kUpT = kUpNW * 64;           // waves per block, threads
+constexpr int kUpLR = 8, kUpLC = 32;                  // low-resolution rows and columns per tile
+constexpr int kUpRW = kUpLR / kUpNW;                  // low-resolution rows per wave
+constexpr int kUpUH = kUpLR + 2, kUpUW = kUpLC + 2;   // low-resolution halo: 10 x 34
+constexpr int kUpSH = 2 * kUpLR + 2, kUpSW = 2 * kUpLC + 2, kUpSP = kUpSW / 2;  // 18 x 66, planes of 33
+// slots of a chunk: halo (padded to whole waves, so that each load instruction of a wave reads
+// one array) + weights; lo 704 + 2304, skip 2432 + 576
+constexpr int kUpUHalo = (kUpUH * kUpUW * 2 + 63) / 64 * 64, kUpUWt = 4 * 32 * 18;
+constexpr int kUpSHalo = (kUpSH * kUpSW * 2 + 63) / 64 * 64, kUpSWt = 32 * 18;
+constexpr int kUpUsed = kUpUHalo + kUpUWt > kUpSHalo + kUpSWt ? kUpUHalo + kUpUWt : kUpSHalo + kUpSWt;
+constexpr int kUpBuf = (kUpUsed + kUpT - 1) / kUpT * kUpT;        // slots per buffer (3072)
+constexpr int kUpLoads = kUpBuf / kUpT;              // buffer loads per thread per chunk
+constexpr int kUpRing = 3 * kUpUW * 2;                // slots of one ring strip's neighbourhood
+constexpr int kUpMaxCout = 1024;
+static_assert(kUpNW * kUpRing + kUpSWt <= kUpBuf, "ring neighbourhoods and weights fit one buffer");
+static_assert(8 * 64 * 32 * 2 <= kUpBuf * 16, "half of the output tile fits one buffer");
+
+struct UpConvArgs {
+    const bf16* lo;       // [N,h,w,Cu]: input of the upsample
+    const bf16* skip;     // [N,2h,2w,Cs] or null (Cs 0)
+    const bf16* wp;       // [4][Cout][9][Cu]: phase weights, phase = 2 py + px
+    const bf16* wk;       // [Cout][9][Cu + Cs]: original weights (skip part, ring pixels)
+    const float* bias;    // [Cout] or null
+    bf16* out;            // [N,2h,2w,Cout]
+    int Cu, Cs, Cout, N, h, w, relu;
+    int tiles;            // tiles x output-channel groups (work items, tile-major)
+    int ring_groups;      // groups of kUpNW ring strips x output-channel groups
+};
+
+__device__ __forceinline__ int up_px_slot(int p, int k) { return p * 2 + (k ^ ((p >> 3) & 1)); }
+__device__ __forceinline__ int up_w_slot(int row, int tap, int k) { return row * 18 + tap * 2 + (k ^ ((row >> 3) & 1)); }
+
+// 8 channels of up2(lo) at full-resolution (Y, X), rounded to BF16 as k_up_fwd stores them.
+__device__ __forceinline__ uint4 up_value(const bf16* lo, int n, int h, int w, int C, int c, int Y, int X) {
+    int y0, y1, x0, x1;
+    float ly, lx;
+    lin_src(Y, h, y0, y1, ly);
+    lin_src(X, w, x0, x1, lx);
+    const bf16* r0 = lo + ((int64_t)n * h + y0) * w * C + c;
+    const bf16* r1 = lo + ((int64_t)n * h + y1) * w * C + c;
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(r0 + (int64_t)x0 * C);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(r0 + (int64_t)x1 * C);
+    const bf16x8 cc = *reinterpret_cast<const bf16x8*>(r1 + (int64_t)x0 * C);
+    const bf16x8 d = *reinterpret_cast<const bf16x8*>(r1 + (int64_t)x1 * C);
+    bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+        v[e] = (bf16)((1.f - ly) * ((1.f - lx) * (float)a[e] + lx * (float)b[e]) +
+                      ly * ((1.f - lx) * (float)cc[e] + lx * (float)d[e]));
+    return *reinterpret_cast<const uint4*>(&v);
+}
+
+// Ring work item `item` (group of kUpNW strips x output-channel group) into LDS buffer s.
+__device__ __forceinline__ void up_ring(const UpConvArgs a, int item, uint4* s) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, kg = lane >> 5;
+    const int cog = a.Cout / 32, co0 = (item % cog) * 32, group = item / cog;
+    const int H = 2 * a.h, W = 2 * a.w, Cin = a.Cu + a.Cs;
+    const int rs = (W + 31) / 32, cs = (H - 2 + 31) / 32, per = 2 * rs + 2 * cs;
+    // this wave's strip: n, kind (0 top, 1 bottom, 2 left, 3 right), start along the strip
+    const int strip = group * kUpNW + wave;
+    const bool live = strip < a.N * per;
+    int n = 0, kind = 0, s0 = 0, len = 0;
+    if (live) {
+        n = strip / per;
+        int r = strip - n * per;
+        if (r < 2 * rs) { kind = r / rs; s0 = (r % rs) * 32; len = W; }
+        else { r -= 2 * rs; kind = 2 + r / cs; s0 = 1 + (r % cs) * 32; len = H - 1; }
+    }
+    const bool row = kind < 2;
+    const int fixed = kind == 0 ? 0 : kind == 1 ? H - 1 : kind == 2 ? 0 : W - 1;
+    uint4* nb = s + wave * kUpRing;       // [3][34] pixels, across the strip x along it
+    uint4* sw = s + kUpNW * kUpRing;      // 32 weight rows
+    f32x16 acc;
+    for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+    for (int ci0 = 0; ci0 < Cin; ci0 += 16) {
+        const bool up = ci0 < a.Cu;
+        __syncthreads();
+        if (live)
+            for (int t = lane; t < kUpRing; t += 64) {
+                const int p = t >> 1, k = t & 1, across = p / kUpUW, along = p % kUpUW;
+                const int Y = row ? fixed + across - 1 : s0 + along - 1;
+                const int X = row ? s0 + along - 1 : fixed + across - 1;
+                uint4 v = zero16();
+                if (Y >= 0 && Y < H && X >= 0 && X < W) {
+                    if (up) v = up_value(a.lo, n, a.h, a.w, a.Cu, ci0 + k * 8, Y, X);
+                    else v = *reinterpret_cast<const uint4*>(a.skip + (((int64_t)n * H + Y) * W + X) * a.Cs +
+                                                             (ci0 - a.Cu) + k * 8);
+                }
+                nb[up_px_slot(p, k)] = v;
+            }
+        for (int t = tid; t < kUpSWt; t += kUpT) {
+            const int co = t / 18, r = t % 18, tap = r >> 1, k = r & 1;
+            sw[up_w_slot(co, tap, k)] =
+                *reinterpret_cast<const uint4*>(a.wk + ((int64_t)(co0 + co) * 9 + tap) * Cin + ci0 + k * 8);
+        }
+        __syncthreads();
+        if (!live) continue;
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap % 3;
+            const int p = (row ? ky : kx) * kUpUW + l32 + (row ? kx : ky);
+            const uint4 av = nb[up_px_slot(p, kg)];
+            const uint4 bv = sw[up_w_slot(l32, tap, kg)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(&av),
+                                                          *reinterpret_cast<const bf16x8*>(&bv), acc, 0, 0, 0);
+        }
+    }
+    __syncthreads();  // the buffer is free for the caller's next use
+    if (!live) return;
+    const float b = a.bias ? a.bias[co0 + l32] : 0.f;
+    for (int j = 0; j < 16; ++j) {
+        const int m = 8 * (j >> 2) + 4 * kg + (j & 3), at = s0 + m;
+        if (at >= len) continue;
+        const int Y = row ? fixed : at, X = row ? at : fixed;
+        float v = acc[j] + b;
+        if (a.relu) v = fmaxf(v, 0.f);
+        a.out[(((int64_t)n * H + Y) * W + X) * a.Cout + co0 + l32] = (bf16)v;
+    }
+}
+
+// Per-block constants of k_conv_up's tile stream, for the force-inlined steps below (no lambdas:
+// their by-reference closures kept such values in scratch memory, and so does a buffer resource
+// inside a struct; the four resources travel as separate arguments).
+struct UpK {
+    bf16* out;
+    int h, w, Cu, Cout, H, W, Cin, cog, tiles_x, tiles_y, nu, nch, P, b, tid, wave, l32, kg, relu;
+};
+struct UpTile { int n, i0, j0, co0; };
+
+__device__ __forceinline__ UpTile up_tile(const UpK& k, int t) {
+    const int item = k.b + t * k.P, q = item / k.cog;
+    const int n = q / (k.tiles_x * k.tiles_y), ty = (q / k.tiles_x) % k.tiles_y, tx = q % k.tiles_x;
+    return UpTile{n, ty * kUpLR, tx * kUpLC, (item % k.cog) * 32};
+}
+
+// Issues chunk s (tile s / nch, channel chunk c = s % nch: c < nu lo channels [16c, 16c + 16),
+// else skip channels [16(c - nu), ...)) into buffer base. Slot t of the buffer is filled by
+// thread t % kUpT in step t / kUpT with the 16 bytes the layout puts there (offset past the
+// range: zeros).
+__device__ __forceinline__ void up_issue(const UpK& k, __amdgpu_buffer_rsrc_t rl, __amdgpu_buffer_rsrc_t rs,
+                                         __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rw, int s, uint4* base) {
+    const UpTile T = up_tile(k, s / k.nch);
+    const int c = s % k.nch;
+    const bool up = c < k.nu;
+    const unsigned ch0 = up ? c * 16 : (c - k.nu) * 16;
+    const int Y0 = 2 * T.i0, X0 = 2 * T.j0;
+#pragma unroll
+    for (int i = 0; i < kUpLoads; ++i) {
+        int t = i * kUpT + k.tid;
+        // opaque to the optimizer: the slot decode below is recomputed at each call instead of
+        // being hoisted out of the chunk loop for all kUpLoads slots (registers)
+        asm volatile("" : "+v"(t));
+        lds_void_t* dst = (lds_void_t*)(base + i * kUpT + k.wave * 64);
+        // halo or weights: the same for the whole wave (the halo regions are whole waves); taken
+        // from lane 0 so that the branch is uniform and each load has one scalar buffer resource
+        const int tw = __builtin_amdgcn_readfirstlane(t);
+        const bool halo = tw < (up ? kUpUHalo : kUpSHalo);
+        unsigned off = kDlOut;
+        if (up && halo) {
+            const int p = t >> 1, q = (t & 1) ^ ((p >> 3) & 1), r = p / kUpUW, cc = p % kUpUW;
+            const int yy = min(max(T.i0 + r - 1, 0), k.h - 1), xx = min(max(T.j0 + cc - 1, 0), k.w - 1);
+            const unsigned o = ((((unsigned)T.n * k.h + yy) * k.w + xx) * k.Cu + ch0 + q * 8) * 2u;
+            off = r < kUpUH ? o : kDlOut;
+        } else if (up) {
+            const int u = t - kUpUHalo, row = u / 18, r = u % 18, tap = r >> 1;
+            const int q = (r & 1) ^ ((row >> 3) & 1);
+            const unsigned o =
+                (((((unsigned)(row >> 5) * k.Cout + T.co0 + (row & 31)) * 9 + tap) * k.Cu) + ch0 + q * 8) * 2u;
+            off = u < kUpUWt ? o : kDlOut;
+        } else if (halo) {
+            const int p = t >> 1, q = (t & 1) ^ ((p >> 3) & 1), r = p / kUpSW, v = p % kUpSW;
+            const int cc = 2 * (v % kUpSP) + v / kUpSP;
+            const int Y = Y0 + r - 1, X = X0 + cc - 1;
+            const unsigned o = ((((unsigned)T.n * k.H + Y) * k.W + X) * (k.Cin - k.Cu) + ch0 + q * 8) * 2u;
+            off = (r < kUpSH && Y >= 0 && Y < k.H && X >= 0 && X < k.W) ? o : kDlOut;
+        } else {
+            const int u = t - kUpSHalo, co = u / 18, r = u % 18, tap = r >> 1;
+            const int q = (r & 1) ^ ((co >> 3) & 1);
+            const unsigned o = ((((unsigned)(T.co0 + co) * 9 + tap) * k.Cin) + k.Cu + ch0 + q * 8) * 2u;
+            off = u < kUpSWt ? o : kDlOut;
+        }
+        const __amdgpu_buffer_rsrc_t rsrc = up ? (halo ? rl : rp) : (halo ? rs : rw);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, dst, 16, off, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void up_mfma(f32x16& acc, const uint4& av, const uint4& bv) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8*>(&av),
+                                                  *reinterpret_cast<const bf16x8*>(&bv), acc, 0, 0, 0);
+}
+
+// The MFMAs of channel chunk c from buffer sb; acc [low-resolution row of the wave][phase].
+__device__ __forceinline__ void up_compute(const UpK& k, int c, const uint4* sb, f32x16 (&acc)[kUpRW][4]) {
+    int wave = k.wave, l32 = k.l32, kg = k.kg;
+    // opaque lane indices: the fragment addresses are computed here, not hoisted out of the
+    // chunk loop for every read of both chunk kinds (hundreds of registers, spills)
+    asm volatile("" : "+v"(wave), "+v"(l32), "+v"(kg));
+    if (c < k.nu) {
+        const uint4* sw = sb + kUpUHalo;
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            // keeps the scheduler from hoisting the next column's LDS reads (registers)
+            __builtin_amdgcn_sched_barrier(0);
+            uint4 bv[3][4];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int ph = 0; ph < 4; ++ph) bv[dy][ph] = sw[up_w_slot(ph * 32 + l32, dy * 3 + dx, kg)];
+#pragma unroll
+            for (int R = 0; R < kUpRW + 2; ++R) {   // halo row RW wave + R = row r of the wave + tap row dy
+                const uint4 av = sb[up_px_slot((kUpRW * wave + R) * kUpUW + l32 + dx, kg)];
+#pragma unroll
+                for (int r = 0; r < kUpRW; ++r) {
+                    const int dy = R - r;
+                    if (dy < 0 || dy > 2) continue;
+#pragma unroll
+                    for (int ph = 0; ph < 4; ++ph) up_mfma(acc[r][ph], av, bv[dy][ph]);
+                }
+            }
+        }
+    } else {
+        const uint4* sw = sb + kUpSHalo;
+        uint4 bv[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) bv[tap] = sw[up_w_slot(l32, tap, kg)];
+#pragma unroll
+        for (int R = 0; R < 2 * kUpRW + 2; ++R) {   // halo row 2 RW wave + R = output row o + kernel row ky
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int C = 0; C < 4; ++C) {   // halo column 2 l32 + C = output parity px + kernel column kx
+                const int cc = 2 * l32 + C;
+                const uint4 av = sb[up_px_slot((2 * kUpRW * wave + R) * kUpSW + (C & 1) * kUpSP + (cc >> 1), kg)];
+#pragma unroll
+                for (int o = 0; o < 2 * kUpRW; ++o) {
+                    const int ky = R - o;
+                    if (ky < 0 || ky > 2) continue;
+#pragma unroll
+                    for (int px = 0; px < 2; ++px) {
+                        const int kx = C - px;
+                        if (kx < 0 || kx > 2) continue;
+                        up_mfma(acc[o >> 1][(o & 1) * 2 + px], av, bv[ky * 3 + kx]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Epilogue of the block's tile t through buffer sb: bias, ReLU, output rows 0..7 (the first half
+// of the waves) then 8..15 as [row][column][channel] BF16, 16-byte stores of the pixels off the
+// ring.
+__device__ __forceinline__ void up_epilogue(const UpK& k, int t, uint4* sb, const float* s_bias,
+                                            const f32x16 (&acc)[kUpRW][4]) {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) u32x4 global_u32x4;
+    const UpTile T = up_tile(k, t);
+    const int Y0 = 2 * T.i0, X0 = 2 * T.j0;
+    int wave = k.wave, l32 = k.l32, kg = k.kg;
+    asm volatile("" : "+v"(wave), "+v"(l32), "+v"(kg));   // as in up_compute
+    const float bias = s_bias[T.co0 + l32];
+    bf16* so = reinterpret_cast<bf16*>(sb);
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        __syncthreads();
+        if (wave / (kUpNW / 2) == half) {
+#pragma unroll
+            for (int r = 0; r < kUpRW; ++r)
+#pragma unroll
+                for (int ph = 0; ph < 4; ++ph)
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int m = 8 * (j >> 2) + 4 * kg + (j & 3);
+                        float v = acc[r][ph][j] + bias;
+                        if (k.relu) v = fmaxf(v, 0.f);
+                        const int orow = 2 * kUpRW * (wave % (kUpNW / 2)) + 2 * r + (ph >> 1);
+                        so[(orow * 64 + 2 * m + (ph & 1)) * 32 + l32] = (bf16)v;
+                    }
+        }
+        __syncthreads();
+        for (int q = k.tid; q < 8 * 64 * 4; q += kUpT) {
+            const int px = q >> 2, part = q & 3, ry = px >> 6, rx = px & 63;
+            const int Y = Y0 + 8 * half + ry, X = X0 + rx;
+            if (Y <= 0 || Y >= k.H - 1 || X <= 0 || X >= k.W - 1) continue;
+            // a global (not flat) store: it joins the in-order vector memory count the chunk
+            // stream's waits rely on
+            *(global_u32x4*)(k.out + (((int64_t)T.n * k.H + Y) * k.W + X) * k.Cout + T.co0 + part * 8) =
+                *reinterpret_cast<const u32x4*>(so + (ry * 64 + rx) * 32 + part * 8);
+        }
+    }
+}
+
+// Step s of the chunk stream: chunk s is in buffer cur, and chunk s + 1 goes to buffer next,
+// which chunk s - 1 used. The wait (all of this thread's loads: chunk s, and the previous tile's
+// stores) and the barrier make chunk s visible to every thread and free buffer next.
+__device__ __forceinline__ void up_step(const UpK& k, __amdgpu_buffer_rsrc_t rl, __amdgpu_buffer_rsrc_t rs,
+                                        __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rw, int s, int S,
+                                        const uint4* cur, uint4* next, const float* s_bias,
+                                        f32x16 (&acc)[kUpRW][4]) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (s + 1 < S) up_issue(k, rl, rs, rp, rw, s + 1, next);
+    const int c = s % k.nch;
+    if (c == 0) {
+#pragma unroll
+        for (int r = 0; r < kUpRW; ++r)
+#pragma unroll
+            for (int ph = 0; ph < 4; ++ph)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[r][ph][j] = 0.f;
+    }
+    up_compute(k, c, cur, acc);
+    if (c == k.nch - 1) up_epilogue(k, s / k.nch, const_cast<uint4*>(cur), s_bias, acc);
+}
+
+__global__ __launch_bounds__(kUpT, 1) void k_conv_up(UpConvArgs a) {
+    // two buffers as two LDS objects (see k_conv_dl)
+    __shared__ __attribute__((aligned(16))) uint4 s_b0[kUpBuf];
+    __shared__ __attribute__((aligned(16))) uint4 s_b1[kUpBuf];
+    __shared__ float s_bias[kUpMaxCout];   // read from LDS so that no global load joins the vmcnt stream
+    for (int t = threadIdx.x; t < a.Cout; t += kUpT) s_bias[t] = a.bias ? a.bias[t] : 0.f;
+    __syncthreads();
+    const int P = gridDim.x, b = blockIdx.x;
+    // ring groups to the blocks with the fewest tiles (the last ones), before their tiles
+    for (int r = P - 1 - b; r < a.ring_groups; r += P) up_ring(a, r, s_b0);
+    UpK k;
+    k.h = a.h; k.w = a.w; k.Cu = a.Cu; k.Cout = a.Cout; k.H = 2 * a.h; k.W = 2 * a.w; k.Cin = a.Cu + a.Cs;
+    k.cog = a.Cout / 32; k.tiles_x = (a.w + kUpLC - 1) / kUpLC; k.tiles_y = (a.h + kUpLR - 1) / kUpLR;
+    k.nu = a.Cu / 16; k.nch = k.nu + a.Cs / 16; k.P = P; k.b = b; k.relu = a.relu; k.out = a.out;
+    k.tid = threadIdx.x; k.wave = k.tid >> 6; k.l32 = k.tid & 31; k.kg = (k.tid >> 5) & 1;
+    // buffer resources (byte sizes < 2^31 checked by the launcher)
+    const __amdgpu_buffer_rsrc_t rl =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.lo, 0, (int)((int64_t)a.N * a.h * a.w * a.Cu * 2), kDlRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.skip ? a.skip : a.lo), 0, (int)((int64_t)a.N * k.H * k.W * a.Cs * 2), kDlRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rp =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.wp, 0, (int)((int64_t)4 * a.Cout * 9 * a.Cu * 2), kDlRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rw =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.wk, 0, (int)((int64_t)a.Cout * 9 * k.Cin * 2), kDlRsrcWord3);
+    const int my_tiles = b < a.tiles ? (a.tiles - 1 - b) / P + 1 : 0;
+    const int S = my_tiles * k.nch;   // chunks this block streams
+    f32x16 acc[kUpRW][4];
+    if (S > 0) up_issue(k, rl, rs, rp, rw, 0, s_b0);
+    for (int s = 0; s < S; s += 2) {
+        up_step(k, rl, rs, rp, rw, s, S, s_b0, s_b1, s_bias, acc);
+        if (s + 1 >= S) break;
+        up_step(k, rl, rs, rp, rw, s + 1, S, s_b1, s_b0, s_bias, acc);
+    }
+}
+
+// Phase weights of k_conv_up from the FP32 weights [Cout][9][Cin] (first Cu input channels):
+// wp[ph][co][dy * 3 + dx][ci] = sum_{ky,kx} M_py[dy][ky] M_px[dx][kx] w[co][ky * 3 + kx][ci],
+// summed in FP32 in (ky, kx) order and rounded to BF16.
+__global__ __launch_bounds__(kBlock) void k_up_phase_weights(const float* w, int Cout, int Cin, int Cu, bf16* wp) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t total = (int64_t)4 * Cout * 9 * Cu;
+    if (i >= total) return;
+    const int ci = (int)(i % Cu);
+    const int tap = (int)((i / Cu) % 9);
+    const int co = (int)((i / ((int64_t)Cu * 9)) % Cout);
+    const int ph = (int)(i / ((int64_t)Cu * 9 * Cout));
+    const float M[2][3][3] = {{{.75f, .25f, 0.f}, {.25f, .75f, .75f}, {0.f, 0.f, .25f}},
+                              {{.25f, 0.f, 0.f}, {.75f, .75f, .25f}, {0.f, .25f, .75f}}};
+    const int py = ph >> 1, px = ph & 1, dy = tap / 3, dx = tap % 3;
+    float s = 0.f;
+    for (int ky = 0; ky < 3; ++ky)
+        for (int kx = 0; kx < 3; ++kx)
+            s += M[py][dy][ky] * M[px][dx][kx] * w[((int64_t)co * 9 + ky * 3 + kx) * Cin + ci];
+    wp[i] = (bf16)s;
+}
+
 // ------------------------------------------------------------------ tile-shape selection
 // Convolution variants: 0 = 8x16-pixel tile x 32/64 channels (k_conv), 1..4 = register-blocked
 // k_conv_rb<RW, NB>, 5..7 = k_conv32 on 32x32x16 MFMAs. Which one is fastest depends on the
@@ -1506,6 +1913,46 @@ lgi_status lgi_unet_conv3x3_fp8_fwd(const void* src0, int32_t c0, const void* sr
     if (!src0 || !w_e4m3 || !w_scale || !out || !(act_scale > 0.0f) || (pool_out && (h % 2 || wd % 2)))
         return LGI_ERR_INVALID_ARG;
     return conv_launch(a, 9, (hipStream_t)stream);
+}
+
+lgi_status lgi_unet_up_phase_weights(const float* w, int32_t cout, int32_t cin, int32_t cu, void* w_phase,
+                                     void* stream) {
+    if (!w || !w_phase || cout <= 0 || cu <= 0 || cu > cin) return LGI_ERR_INVALID_ARG;
+    const int64_t total = (int64_t)4 * cout * 9 * cu;
+    hipLaunchKernelGGL(k_up_phase_weights, dim3(cdiv(total, kBlock)), dim3(kBlock), 0, (hipStream_t)stream, w, cout,
+                       cin, cu, (bf16*)w_phase);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
+}
+
+lgi_status lgi_unet_conv3x3_up_fwd(const void* lo, int32_t cu, const void* skip, int32_t cs, const void* w_phase,
+                                   const void* w, const float* bias, int32_t cout, int32_t n, int32_t h, int32_t wd,
+                                   int32_t relu, void* out, void* stream) {
+    if (!skip) cs = 0;
+    if (!lo || !w_phase || !w || !out || n <= 0 || h <= 0 || wd <= 0 || cu <= 0 || cu % 32 || cs < 0 || cs % 32 ||
+        cout <= 0 || cout % 32)
+        return LGI_ERR_INVALID_ARG;
+    if (cout > kUpMaxCout) return LGI_ERR_UNSUPPORTED;
+    const int64_t H = 2 * (int64_t)h, W = 2 * (int64_t)wd, cog = cout / 32;
+    const int64_t strips = (int64_t)n * (2 * ((W + 31) / 32) + 2 * ((H - 2 + 31) / 32));
+    const int64_t ring = (strips + kUpNW - 1) / kUpNW * cog;
+    const int64_t tiles = (int64_t)n * ((h + kUpLR - 1) / kUpLR) * ((wd + kUpLC - 1) / kUpLC) * cog;
+    // the kernel addresses lo, skip and both weight arrays through 32-bit buffer offsets
+    const int64_t lim = int64_t(1) << 31;
+    if (ring + tiles > INT32_MAX || (int64_t)n * h * wd * cu * 2 >= lim || (int64_t)n * H * W * cs * 2 >= lim ||
+        (int64_t)4 * cout * 9 * cu * 2 >= lim || (int64_t)cout * 9 * (cu + cs) * 2 >= lim)
+        return LGI_ERR_UNSUPPORTED;
+    // persistent grid: one block per CU (the kernel's 100 KB of LDS allows no more)
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return LGI_ERR_HIP;
+    UpConvArgs a{};
+    a.lo = (const bf16*)lo; a.skip = (const bf16*)skip; a.wp = (const bf16*)w_phase; a.wk = (const bf16*)w;
+    a.bias = bias; a.out = (bf16*)out; a.Cu = cu; a.Cs = cs; a.Cout = cout; a.N = n; a.h = h; a.w = wd;
+    a.relu = relu; a.tiles = (int)tiles; a.ring_groups = (int)ring;
+    const int P = (int)std::min<int64_t>(cus, tiles + ring);
+    hipLaunchKernelGGL(k_conv_up, dim3(P), dim3(kUpT), 0, (hipStream_t)stream, a);
+    return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }
 
 static lgi_status first_fwd(bool log_in, const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,

@@ -47,6 +47,7 @@ struct WfParams {
     float4* vP;           // per queue slot: shading point (xyz)
     float4* vN;           // per queue slot: facing shading normal (xyz)
     const int32_t* tiles; // 8x8 tiles to render (row-major tile ids), or null for the whole image
+    int32_t set_sums;     // k_wf_accum starts from zero (LGI_RENDER_SET_SUMS, first pass)
 };
 
 // Shadow rays are not stored: the any-hit kernel reads the id j = slot * nl + light of each
@@ -613,8 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_accum(WfParams W, int S, int n_ti
     }
     float* sum = W.P.sum + 3 * (int64_t)pix;
     float* comp = W.P.comp + 3 * (int64_t)pix;
-    float s0 = sum[0], s1 = sum[1], s2 = sum[2], c0 = comp[0], c1 = comp[1], c2 = comp[2];
-    float sq = W.P.sumsq ? W.P.sumsq[pix] : 0.f;
+    const bool add = !W.set_sums;
+    float s0 = add ? sum[0] : 0.f, s1 = add ? sum[1] : 0.f, s2 = add ? sum[2] : 0.f;
+    float c0 = add ? comp[0] : 0.f, c1 = add ? comp[1] : 0.f, c2 = add ? comp[2] : 0.f;
+    float sq = (add && W.P.sumsq) ? W.P.sumsq[pix] : 0.f;
     for (int s = 0; s < S; ++s) {
         const float* L = W.L + 3 * ((int64_t)s * W.npx + pix);
         kahan_add(s0, c0, L[0]);
@@ -763,6 +766,7 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     for (int s0 = 0; s0 < P.spp; s0 += S_max) {
         const int S = std::min(S_max, P.spp - s0);
         W.P.sample0 = P.sample0 + s0;
+        W.set_sums = (P.flags & LGI_RENDER_SET_SUMS) != 0 && s0 == 0;
         const int n_max = S * padded;
         const int nb = cdiv(n_max, kBlock);
         hipLaunchKernelGGL(k_wf_fill, dim3(cdiv(kCounters * (P.bounces + 1), kBlock)), dim3(kBlock), 0, st, counters,

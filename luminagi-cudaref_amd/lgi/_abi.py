@@ -23,6 +23,7 @@ RNG_XORSHIFT_REF = 1
 RENDER_SORTED = 2
 RENDER_NO_RR = 4
 RENDER_NO_RAY_SORT = 8
+RENDER_SET_SUMS = 16
 
 
 class Camera(C.Structure):
@@ -126,6 +127,9 @@ SIGNATURES = {
     "lgi_unet_tail_radiance_fwd": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P]),
     "lgi_unet_conv3x3_first_fwd": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P,
                                              _P]),
+    "lgi_unet_conv3x3_up_fwd": (C.c_int, [_P, C.c_int32, _P, C.c_int32, _P, _P, _P, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_int32, _P, _P]),
+    "lgi_unet_up_phase_weights": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, _P, _P]),
     "lgi_unet_conv3x3_bwd_data": (C.c_int, [_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                             C.c_int32, _P, _P, _P, _P]),
     "lgi_unet_conv3x3_bwd_weight": (C.c_int, [_P, _P, C.c_int32, _P, C.c_int32, _P, C.c_int32, C.c_int32, C.c_int32,

@@ -117,10 +117,12 @@ class DenoiseGraph:
 
 
 class FrameGraph:
-    """One frame of the 8-spp + UNet path captured in a single HIP graph: clear the
-    accumulators, render `spp` samples with the sorted renderer (from `sample0`), mean radiance,
-    UNet, tonemap. A replay re-renders the same sample indices, so it reproduces the frame bit
-    for bit; the outputs (`hdr`, `rgb`) are static tensors overwritten by each replay.
+    """One frame of the 8-spp + UNet path captured in a single HIP graph: render `spp` samples
+    (from `sample0`) with the renderer `flags` names, its sums written over the accumulators
+    (LGI_RENDER_SET_SUMS: no clearing pass, and no squared luminance, which only the variance
+    gate reads), mean radiance, UNet, tonemap. A replay re-renders the same sample indices, so it
+    reproduces the frame bit for bit; the outputs (`hdr`, `rgb`) are static tensors overwritten
+    by each replay.
 
     The render needs its device buffers allocated and the UNet its convolution tiles chosen
     before capture, so the constructor runs the frame once eagerly first. The context is moved
@@ -148,10 +150,8 @@ class FrameGraph:
 
     def _frame(self, ctx, model, h, w, spp, bounces, sample0, flags):
         f = self.frame
-        f.sum.zero_()
-        f.comp.zero_()
-        f.sumsq.zero_()
-        ctx.accumulate(f.sum, f.comp, spp, bounces, flags=flags, sample0=sample0, d_sumsq=f.sumsq, camera=f.cam)
+        ctx.accumulate(f.sum, f.comp, spp, bounces, flags=flags | _abi.RENDER_SET_SUMS, sample0=sample0,
+                       camera=f.cam)
         hdr = denoise(model, f.sum.view(h, w, 3) / float(spp))
         return hdr, tonemap(ctx, hdr)
 

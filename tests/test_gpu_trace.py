@@ -223,6 +223,40 @@ def test_wavefront_multi_pass_equals_inline(gpu, flags):
         assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
 
 
+@pytest.mark.parametrize("flags,spp,noalloc", [(0, 8, False), (2, 8, False), (10, 8, False), (0, 8, True),
+                                               (10, WAVEFRONT_PATHS // (64 * 64) + 40, False)],
+                         ids=["inline", "sorted", "no_ray_sort", "noalloc", "wavefront_multi_pass"])
+def test_set_sums_equals_zeroed_buffers(gpu, flags, spp, noalloc):
+    """LGI_RENDER_SET_SUMS writes the sums over whatever the buffers held: bit for bit what an
+    accumulation into zeroed buffers gives (the multi-pass case adds its later passes)."""
+    import torch
+    import lgi
+    ctx, _, s = make_pair(seed=4)
+    cam = small_camera(s["camera"], 64, 64)
+    ref = gpu_accumulate(ctx, cam, spp, 2, flags=flags, sample0=3)
+    bufs = [torch.full_like(t, float("nan")) for t in ref]
+    bufs[1].fill_(123.0)
+    call = ctx.accumulate_noalloc if noalloc else ctx.accumulate
+    call(bufs[0], bufs[1], spp, 2, flags=flags | lgi.RENDER_SET_SUMS, sample0=3, d_sumsq=bufs[2], camera=cam)
+    torch.cuda.synchronize()
+    for a, b in zip(ref, bufs):
+        assert torch.equal(a, b)
+
+
+def test_set_sums_rejected_by_batch_and_tile_renders(gpu):
+    import torch
+    import lgi
+    ctx, _, s = make_pair(seed=4)
+    cam = small_camera(s["camera"], 32, 32)
+    n = cam.width * cam.height
+    bufs = [torch.zeros(n * 3, device="cuda"), torch.zeros(n * 3, device="cuda"), torch.zeros(n, device="cuda")]
+    with pytest.raises(lgi.LgiError):
+        ctx.render_tiles(np.array([0, 1], np.int32), 4, bufs[0], bufs[1], 2, d_sumsq=bufs[2], camera=cam,
+                         flags=lgi.RENDER_SET_SUMS)
+    with pytest.raises(lgi.LgiError):
+        ctx.render_batch([(cam, 4, 0)], bufs[0], bufs[1], 2, d_sumsq=bufs[2], flags=lgi.RENDER_SET_SUMS)
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (7, 3), (9, 17), (65, 1)])
 def test_tiny_and_ragged_images(gpu, w, h):
     """Image sizes that leave partial 8x8 tiles (and a single pixel): the inline renderer, the

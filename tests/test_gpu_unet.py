@@ -133,6 +133,71 @@ def test_upsample_forward_backward(gpu):
     assert rel(xc.grad, xd.grad) < 1e-2
 
 
+UP_CASES = [  # n, h, w (upsample input), cu, cs, cout, relu
+    (1, 8, 32, 64, 32, 32, True),      # one tile
+    (2, 11, 45, 32, 32, 64, True),     # ragged tiles, two images, two output-channel blocks
+    (1, 3, 5, 64, 64, 32, False),      # image smaller than one tile
+    (1, 1, 1, 32, 32, 32, True),       # 2x2 output: every pixel on the border ring
+    (1, 20, 40, 128, 0, 32, True),     # no skip tensor
+]
+
+
+@pytest.mark.parametrize("case", UP_CASES, ids=lambda c: "x".join(map(str, c)))
+def test_conv3x3_up_matches_upsample_then_conv(gpu, case):
+    """The fused decoder convolution (phase weights inside the image, explicitly upsampled
+    neighbourhoods on the border ring) against the FP64 upsample + convolution, and against the
+    two-kernel BF16 path; the ring pixels separately, since they take their own code path."""
+    from lgi import unet
+    n, h, w, cu, cs, cout, relu = case
+    lo = bf16_rand(n, h, w, cu, seed=21)
+    skip = bf16_rand(n, 2 * h, 2 * w, cs, seed=22) if cs else None
+    wt = bf16_rand(cout, 9, cu + cs, scale=(2.0 / (9 * (cu + cs))) ** 0.5, seed=23)
+    b = torch.randn(cout, generator=torch.Generator().manual_seed(24)) * 0.1
+    wp = unet.up_phase_weights(wt.float().cuda(), cu)
+    y = unet.conv3x3_up_fwd(lo.cuda(), None if skip is None else skip.cuda(), wp, wt.cuda(), b.cuda(), relu)
+    up = unet_ref.up(lo.double().permute(0, 3, 1, 2)).permute(0, 2, 3, 1)
+    ref = ref_conv_nhwc(up, skip, wt, b, relu)
+    two = unet.conv3x3_fwd(unet.UpsampleFn.apply(lo.cuda()), None if skip is None else skip.cuda(), wt.cuda(),
+                           b.cuda(), relu)
+    ring = torch.zeros(2 * h, 2 * w, dtype=torch.bool)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    yc, tc = y.cpu(), two.cpu()
+    errs = {"fp64": rel(yc, ref), "two_kernel": rel(yc, tc), "ring_fp64": rel(yc[:, ring], ref[:, ring]),
+            "ring_two_kernel": rel(yc[:, ring], tc[:, ring])}
+    print(f"conv up {case}: {errs}", flush=True)
+    assert y.shape == ref.shape
+    # each path is within BF16_TOL of the FP64 result, so they differ by up to twice that
+    assert errs["fp64"] < BF16_TOL and errs["ring_fp64"] < BF16_TOL, errs
+    assert errs["two_kernel"] < 2 * BF16_TOL and errs["ring_two_kernel"] < 2 * BF16_TOL, errs
+
+
+def test_conv3x3_up_many_tiles_per_block(gpu):
+    """More tiles than the persistent grid has blocks (one per CU), two output-channel groups:
+    each block streams several tiles' chunks through its three buffers, across tile boundaries."""
+    from lgi import unet
+    n, h, w, cu, cs, cout = 2, 72, 272, 64, 32, 64   # 2 x 9 x 9 tiles x 2 channel groups = 324 items
+    g = torch.Generator(device="cuda").manual_seed(31)
+    lo = torch.randn(n, h, w, cu, device="cuda", generator=g).to(torch.bfloat16)
+    skip = torch.randn(n, 2 * h, 2 * w, cs, device="cuda", generator=g).to(torch.bfloat16)
+    wt = (torch.randn(cout, 9, cu + cs, device="cuda", generator=g) * (2.0 / (9 * (cu + cs))) ** 0.5).to(torch.bfloat16)
+    b = torch.randn(cout, device="cuda", generator=g) * 0.1
+    y = unet.conv3x3_up_fwd(lo, skip, unet.up_phase_weights(wt.float(), cu), wt, b, True)
+    two = unet.conv3x3_fwd(unet.UpsampleFn.apply(lo), skip, wt, b, True)
+    e = rel(y, two)
+    print(f"conv up, many tiles per block: rel diff to the two-kernel path {e:.2e}", flush=True)
+    assert e < 2 * BF16_TOL
+
+
+def test_conv3x3_up_rejects_bad_shapes(gpu):
+    from lgi import unet
+    from lgi._abi import LgiError
+    lo = torch.zeros((1, 4, 4, 48), dtype=torch.bfloat16, device="cuda")   # 48 channels: not a multiple of 32
+    wt = torch.zeros((32, 9, 48), dtype=torch.bfloat16, device="cuda")
+    wp = torch.zeros((4, 32, 9, 48), dtype=torch.bfloat16, device="cuda")
+    with pytest.raises(LgiError):
+        unet.conv3x3_up_fwd(lo, None, wp, wt, None, True)
+
+
 def test_tail_forward_backward(gpu):
     from lgi import unet
     feat = bf16_rand(1, 8, 16, 32, seed=14)
