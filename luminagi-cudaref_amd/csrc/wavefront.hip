@@ -171,6 +171,12 @@ constexpr uint32_t kChunk = 64;
 // kernel's tail, the time its last rays take after the queue has run dry.
 enum TraceMode { kTraceClosest = 0, kTraceShadow = 1, kTraceBoth = 2 };
 
+// Lane of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int nth_lane(uint64_t m, int k) {
+    for (int j = 0; j < k; ++j) m &= m - 1;
+    return __ffsll((unsigned long long)m) - 1;
+}
+
 template <int kMode, int kLds, int kTop, int kMinLanes>
 __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
@@ -292,6 +298,42 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 cend = nend;
             } else {
                 cbase += from_old;
+            }
+        }
+        if (kMode != kTraceClosest && exhausted) {
+            // Work sharing once the queue is dry: an idle lane takes the next pending subtree of
+            // a lane whose any-hit (shadow) ray still has one (its stack top, with the ray), so
+            // the long rays that end a launch are traversed by several lanes. A hit in any
+            // subtree marks the ray occluded (nee[slot] = 2), which no order of visits changes.
+            // Closest-hit rays are not shared: their tie order at equal distances would follow
+            // the schedule.
+            const bool can_give = active && any && sp > 0;
+            const uint64_t idle = __ballot(!active), donors = __ballot(can_give);
+            if (idle != 0 && donors != 0) {
+                const int npair = min(__popcll(idle), __popcll(donors));
+                const int drank = __popcll(donors & lt), irank = __popcll(idle & lt);
+                int give = 0;
+                if (can_give && drank < npair) give = stk.pop(--sp);
+                const bool take = !active && irank < npair;
+                const int src = take ? nth_lane(donors, irank) : lane;
+                const int g = __shfl(give, src, 64), sl = __shfl(slot, src, 64);
+                const float bt = __shfl(best_t, src, 64);
+                const V3 o = v3(__shfl(r.o.x, src, 64), __shfl(r.o.y, src, 64), __shfl(r.o.z, src, 64));
+                const V3 d = v3(__shfl(r.d.x, src, 64), __shfl(r.d.y, src, 64), __shfl(r.d.z, src, 64));
+                const V3 iv = v3(__shfl(r.inv.x, src, 64), __shfl(r.inv.y, src, 64), __shfl(r.inv.z, src, 64));
+                if (take) {
+                    any = true;
+                    r.o = o;
+                    r.d = d;
+                    r.inv = iv;
+                    best_t = bt;
+                    slot = sl;
+                    node = g;
+                    leaf = 0;
+                    sp = 0;
+                    best_tri = -1;
+                    active = true;
+                }
             }
         }
         if (__ballot(active) == 0) {
