@@ -392,29 +392,45 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* keys, int n, 
     if (i == 0) parent[0] = -1;
 }
 
-// Reference nodes: the root and every child of a splittable node. Emits (BFS key, id).
-__global__ __launch_bounds__(kBlock) void k_ref_nodes(int n, const int32_t* parent, const int32_t* rfirst,
-                                                      const uint8_t* splittable, uint32_t* out_keys,
-                                                      int32_t* out_ids, uint32_t* counter) {
+// Reference nodes: the root and every child of a splittable node.
+__device__ inline bool is_ref_node(int x, int n, const int32_t* parent, const uint8_t* splittable) {
+    if (x == 0) return true;
+    if (x >= 2 * n - 1) return false;
+    const int p = parent[x];
+    return p >= 0 && splittable[p];
+}
+
+// Reference-node count of each block of k_ref_nodes (their exclusive scan gives the blocks' output
+// offsets: an atomic slot counter shared by all waves serialises in L2).
+__global__ __launch_bounds__(kBlock) void k_ref_count(int n, const int32_t* parent, const uint8_t* splittable,
+                                                      uint32_t* block_counts) {
+    __shared__ uint32_t s_wave[kBlock / 64];
     const int x = blockIdx.x * kBlock + threadIdx.x;
-    bool ref = x == 0;
-    if (x > 0 && x < 2 * n - 1) {
-        const int p = parent[x];
-        ref = (p >= 0) && splittable[p];
+    const uint64_t mask = __ballot(is_ref_node(x, n, parent, splittable));
+    if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += s_wave[w];
+        block_counts[blockIdx.x] = t;
     }
-    // one atomic per wave for the wave's output slots: a single-address atomic from every node
-    // serialises in L2 (the output order does not matter: the BFS sort orders it)
-    const uint64_t mask = __ballot(ref);
-    if (mask == 0) return;
-    const int lane = threadIdx.x & 63, leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
+}
+
+// Emits (BFS key, id) of every reference node at its block's offset plus its rank in the block
+// (node order; the BFS sort orders the keys, which are unique: depth and first primitive).
+__global__ __launch_bounds__(kBlock) void k_ref_nodes(int n, const int32_t* parent, const int32_t* rfirst,
+                                                      const uint8_t* splittable, const uint32_t* block_base,
+                                                      uint32_t* out_keys, int32_t* out_ids) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    const bool ref = is_ref_node(x, n, parent, splittable);
+    uint32_t total;
+    const uint32_t rank = block_exclusive_scan(ref ? 1u : 0u, s_wave, total);
     if (!ref) return;
     const int first = (x < n - 1) ? rfirst[x] : x - (n - 1);
     int depth = 0;
     for (int y = x; y != 0; y = parent[y]) ++depth;
-    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    const uint32_t slot = block_base[blockIdx.x] + rank;
     out_keys[slot] = ((uint32_t)depth << kDepthShift) | (uint32_t)first;
     out_ids[slot] = x;
 }
@@ -787,7 +803,7 @@ __global__ __launch_bounds__(64) void k_treelet_level(lgi_bvh_node* nodes, const
 
 // Levels of the current tree, in parallel: every slot finds its depth by walking its parent
 // chain, then one stable 8-bit radix pass orders the slots by depth (by slot index within a
-// level). info[0] receives the deepest depth and info[1] an error flag: set when a child ref is
+// level); the level-start pass writes the deepest depth to info[0]. info[1] is an error flag: set when a child ref is
 // out of range, a slot other than the root has no parent or two, or a walk does not reach the
 // root within kMaxWalk steps (a cycle). Only a tree of m slots passes.
 constexpr int kMaxWalk = 63;
@@ -834,18 +850,19 @@ __global__ __launch_bounds__(kBlock) void k_tree_depth(int m, const int32_t* par
         depth[p] = (uint32_t)d;
         slot[p] = p;
     }
-    // one atomic per wave: a single-address atomic from every slot serialises in L2
-    int dm = d;
-    for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&info[0], dm);
 }
 
-__global__ __launch_bounds__(kBlock) void k_depth_starts(const uint32_t* depth_sorted, int m, int32_t* level_start) {
+// Level starts of the depth-sorted slots; the last slot holds the maximum depth (to max_depth).
+__global__ __launch_bounds__(kBlock) void k_depth_starts(const uint32_t* depth_sorted, int m, int32_t* level_start,
+                                                         int32_t* max_depth) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
     const int d = (int)depth_sorted[p];
     if (p == 0 || (int)depth_sorted[p - 1] != d) level_start[d] = p;
-    if (p == m - 1) level_start[d + 1] = m;
+    if (p == m - 1) {
+        level_start[d + 1] = m;
+        *max_depth = d;
+    }
 }
 
 // Traversal numbering of the restructured tree: its internal nodes level by level (order[] from
@@ -1080,9 +1097,13 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
     } else {
         hipLaunchKernelGGL(k_karras, dim3(ceil_div(n - 1, kBlock)), dim3(kBlock), 0, st, d.keys, n, child, parent,
                            rfirst, rlast, splittable);
-        LGI_HIP(ctx, hipMemsetAsync(counter, 0, 4, st));
-        hipLaunchKernelGGL(k_ref_nodes, dim3(ceil_div(2 * n - 1, kBlock)), dim3(kBlock), 0, st, n, parent, rfirst,
-                           splittable, sb.k[0], sb.v[0], counter);
+        // per-block counts and their offsets in tflags (free until the traversal layout), the
+        // node count to counter
+        const int nb_ref = ceil_div(2 * n - 1, kBlock);
+        hipLaunchKernelGGL(k_ref_count, dim3(nb_ref), dim3(kBlock), 0, st, n, parent, splittable, tflags);
+        lgi_exclusive_scan(st, tflags, nb_ref, counter, tsums);
+        hipLaunchKernelGGL(k_ref_nodes, dim3(nb_ref), dim3(kBlock), 0, st, n, parent, rfirst, splittable, tflags,
+                           sb.k[0], sb.v[0]);
         uint32_t hm = 0;
         LGI_HIP(ctx, hipMemcpyAsync(&hm, counter, 4, hipMemcpyDeviceToHost, st));
         LGI_HIP(ctx, hipStreamSynchronize(st));
@@ -1155,7 +1176,8 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
                 hipLaunchKernelGGL(k_tree_depth, dim3(gm), dim3(kBlock), 0, st, (int)m, parent, rank, sb.k[0], sb.v[0],
                                    info);
                 const int cur = radix_sort(st, sb, (int)m, 8);
-                hipLaunchKernelGGL(k_depth_starts, dim3(gm), dim3(kBlock), 0, st, sb.k[cur], (int)m, lstart);
+                hipLaunchKernelGGL(k_depth_starts, dim3(gm), dim3(kBlock), 0, st, sb.k[cur], (int)m, lstart,
+                                   info);
                 LGI_HIP(ctx, hipMemcpyAsync(lv_host, lstart, sizeof(lv_host), hipMemcpyDeviceToHost, st));
                 LGI_HIP(ctx, hipStreamSynchronize(st));
                 order = lv_order = sb.v[cur];
