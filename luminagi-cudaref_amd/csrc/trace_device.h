@@ -115,11 +115,6 @@ LGI_D Ray primary_ray(const DevCamera& c, int px, int py, float jx, float jy) {
     return make_ray(v3(c.pos[0], c.pos[1], c.pos[2]), dw);
 }
 
-// One NEE candidate of shade_direct (src/pathtracer.cu:230-272): light index 0 is the sun,
-// 1 + i the i-th point light. Returns false when the light cannot contribute; otherwise
-// the shadow ray, its tmax and the contribution albedo * (color * I) to add if unoccluded.
-// A point light at distance >= radius has attenuation exactly 0, so it is reported as not
-// contributing (its shadow ray would only ever add +0).
 // Diffuse texture lookup with the semantics of the reference's CUDA texture object
 // (src/pathtracer.cu:274-282, 563-576): normalised coordinates (u, 1 - v), wrap addressing,
 // RGBA8 texels converted from sRGB to linear before filtering, bilinear filtering with the
@@ -182,23 +177,33 @@ LGI_D void shadow_ray(int li, V3 Pt, V3 N, V3 toLight, float dist, Ray& shadow, 
     tmax = li == 0 ? 1e30f : dist - kShadowBias;
 }
 
+// Whether light li is a NEE candidate at (Pt, N) in shade_direct (src/pathtracer.cu:230-272):
+// light index 0 is the sun, 1 + i the i-th point light. Also returns the direction and distance
+// to the light and the clamped cosine. A point light at distance >= radius has attenuation
+// exactly 0, so it is not a candidate (its shadow ray would only ever add +0).
+LGI_D bool nee_light(const TraceParams& P, int li, V3 Pt, V3 N, V3& toLight, float& dist, float& ndotl) {
+    light_dir(P, li, Pt, toLight, dist);
+    ndotl = fmaxf(0.0f, dot(N, toLight));
+    if (li == 0) return ndotl > 0.0f;
+    const float r = P.points[li - 1].radius;
+    if (ndotl <= 0.0f || dist >= r * 3.0f) return false;
+    return !(dist >= r);  // atten == 0 exactly
+}
+
+// One NEE candidate: false when the light cannot contribute; otherwise the shadow ray, its tmax
+// and the contribution albedo * (color * I) to add if unoccluded.
 LGI_D bool nee_candidate(const TraceParams& P, int li, V3 Pt, V3 N, V3 albedo, Ray& shadow, float& tmax,
                          V3& contrib) {
     V3 toLight;
-    float dist;
-    light_dir(P, li, Pt, toLight, dist);
-    float ndotl = fmaxf(0.0f, dot(N, toLight));
+    float dist, ndotl;
+    if (!nee_light(P, li, Pt, N, toLight, dist, ndotl)) return false;
+    shadow_ray(li, Pt, N, toLight, dist, shadow, tmax);
     if (li == 0) {
-        if (!(ndotl > 0.0f)) return false;
-        shadow_ray(li, Pt, N, toLight, dist, shadow, tmax);
         float I = P.sun_intensity * ndotl;
         contrib = had(albedo, mul(v3(P.sun_color[0], P.sun_color[1], P.sun_color[2]), I));
         return true;
     }
     const lgi_point_light L = P.points[li - 1];
-    if (ndotl <= 0.0f || dist >= L.radius * 3.0f) return false;
-    if (dist >= L.radius) return false;  // atten == 0 exactly
-    shadow_ray(li, Pt, N, toLight, dist, shadow, tmax);
     float r = L.radius;
     float atten = fmaxf(0.0f, 1.0f - lgi_div(dist, r));
     atten = atten * atten;

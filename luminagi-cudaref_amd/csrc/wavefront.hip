@@ -46,6 +46,9 @@ struct WfParams {
     uint8_t* nee;         // per (queue slot, light): 0 none, 1 candidate, 2 occluded
     float4* vP;           // per queue slot: shading point (xyz)
     float4* vN;           // per queue slot: facing shading normal (xyz)
+    uint8_t* occl;        // per (queue slot, light) of the last bounce when its shadow rays are
+                          // traced in the launch before (kFold): for its candidates, 2 occluded,
+                          // else 0; or null
     const int32_t* tiles; // 8x8 tiles to render (row-major tile ids), or null for the whole image
     int32_t set_sums;     // k_wf_accum starts from zero (LGI_RENDER_SET_SUMS, first pass)
 };
@@ -59,6 +62,7 @@ struct ShadowSrc {
     const float4* vN;
     int nl;
     TraceParams P;        // light parameters
+    uint8_t* occl;        // kFold: occlusion of the last bounce's candidates (2 = occluded)
 };
 
 __device__ __forceinline__ uint32_t spread3(uint32_t v) {
@@ -169,6 +173,11 @@ constexpr uint32_t kChunk = 64;
 // (queue positions [0, n_sh) are shadow rays, [n_sh, n_sh + n) closest-hit rays). Both depend
 // only on the shading of the bounce before, and one launch instead of two saves a persistent
 // kernel's tail, the time its last rays take after the queue has run dry.
+// kFold (kTraceBoth whose closest-hit rays are the path's last bounce): a lane that finds such a
+// ray's closest hit computes its shading point and normal as the shading kernel does, then
+// traces that hit's NEE shadow rays itself, one light after another, and marks occluded ones in
+// shadow.occl. The last bounce's shading then reads those marks instead of a shadow-ray launch
+// of its own: one persistent launch, with its tail, fewer per pass.
 enum TraceMode { kTraceClosest = 0, kTraceShadow = 1, kTraceBoth = 2 };
 
 // Lane of the k-th set bit of m (k < popcount(m)).
@@ -177,8 +186,8 @@ __device__ __forceinline__ int nth_lane(uint64_t m, int k) {
     return __ffsll((unsigned long long)m) - 1;
 }
 
-template <int kMode, int kLds, int kTop, int kMinLanes>
-__global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
+template <int kMode, int kLds, int kTop, int kMinLanes, bool kFold>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kFold ? 7 : 1))) void k_wf_trace(const TNode* __restrict__ tnodes, int n_tnodes,
                                                      const TriV* __restrict__ triv, const QRay* __restrict__ q, int n,
                                                      const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev,
                                                      float4* hits, uint8_t* nee, uint32_t* cursors, int* spill,
@@ -190,6 +199,9 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
     const int n_all = n_sh + n;
     __shared__ int s_stack[kLds * kBlock];
     __shared__ TNode s_top[kTop > 0 ? kTop : 1];
+    // kFold: the shading point and normal of the lane's last-bounce hit, [component][thread]
+    __shared__ float s_fold[kFold ? 6 * kBlock : 1];
+    static_assert(!kFold || kMode == kTraceBoth, "the fold traces closest-hit and shadow rays");
     if (kTop > 0) {
         const int cnt = min(kTop, n_tnodes);
         for (int i = threadIdx.x; i < cnt * 4; i += kBlock)
@@ -211,10 +223,44 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
     // node: >= 0 internal node to visit, < 0 leaf (~first triangle), kNone when the stack is
     // exhausted; leaf: a parked leaf (0 = none)
     constexpr int kNone = INT_MIN;
+    // kFold: qi = -1 while the lane traces a shadow ray of its own last-bounce hit (-2 when it
+    // helps another lane with one); while inactive, slot = ~j marks candidate j = qi * nl + light
+    // of that hit as the next one to test
     int qi = 0, node = 0, sp = 0, leaf = 0, best_tri = -1, slot = 0;
     float best_t = 0.f, best_u = 0.f, best_v = 0.f;
     Ray r;
     for (;;) {
+        if (kFold && !active && slot < 0) {
+            // the next light of this lane's last-bounce hit that is a NEE candidate, if any
+            const int nl = shadow.nl, j0 = ~slot, i = j0 / nl;
+            const float* f = s_fold + threadIdx.x;
+            const V3 Pt = v3(f[0], f[kBlock], f[2 * kBlock]), N = v3(f[3 * kBlock], f[4 * kBlock], f[5 * kBlock]);
+            V3 toLight = v3(0.f, 0.f, 0.f);
+            float dist = 0.f, ndotl = 0.f;
+            int li = j0 - i * nl;
+            while (li < nl && !nee_light(shadow.P, li, Pt, N, toLight, dist, ndotl)) ++li;
+            slot = 0;
+            if (li < nl) {
+                Ray sr;
+                float tmax;
+                shadow_ray(li, Pt, N, toLight, dist, sr, tmax);
+                r.o = sr.o;
+                r.d = sr.d;
+                r.inv = v3(lgi_div(1.0f, r.d.x), lgi_div(1.0f, r.d.y), lgi_div(1.0f, r.d.z));
+                best_t = tmax;
+                best_tri = -1;
+                node = 0;
+                leaf = 0;
+                sp = 0;
+                any = true;
+                qi = -1;
+                slot = i * nl + li;
+                active = true;
+                // written before any lane can help with this ray (helpers take subtrees of
+                // a ray in a later iteration of this wave), so a helper's 2 lands after it
+                shadow.occl[slot] = 0;
+            }
+        }
         const uint64_t need = __ballot(!active);
         if (need && !exhausted) {
             const uint32_t cnt = (uint32_t)__popcll(need);
@@ -287,6 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                         active = true;
                     } else {
                         hits[idx] = make_float4(-1.f, 0.f, 0.f, __int_as_float(-1));
+                        if (kFold) slot = 0;  // no pending candidate
                     }
                 }
             }
@@ -300,6 +347,27 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 cbase += from_old;
             }
         }
+        if (kFold && exhausted) {
+            // Once the queue is dry, a lane that has just started a light of its own hit hands
+            // the lights after it to an idle lane (with the hit's shading point and normal), so
+            // a hit's shadow rays run side by side in the launch's tail instead of one after
+            // another. The taker tests them from the next iteration on, as its own.
+            const bool can_hand = active && qi == -1 && (slot + 1) % shadow.nl != 0;
+            const uint64_t idle = __ballot(!active), handers = __ballot(can_hand);
+            if (idle != 0 && handers != 0) {
+                const int npair = min(__popcll(idle), __popcll(handers));
+                const int hrank = __popcll(handers & lt), irank = __popcll(idle & lt);
+                const bool take = !active && irank < npair;
+                const int src = take ? nth_lane(handers, irank) : lane;
+                const int sl = __shfl(slot, src, 64);
+                if (can_hand && hrank < npair) qi = -2;  // its later lights are the taker's
+                if (take) {
+                    const int from = (int)threadIdx.x - lane + src;
+                    for (int k = 0; k < 6; ++k) s_fold[k * kBlock + threadIdx.x] = s_fold[k * kBlock + from];
+                    slot = ~(sl + 1);
+                }
+            }
+        }
         if (kMode != kTraceClosest && exhausted) {
             // Work sharing once the queue is dry: an idle lane takes the next pending subtree of
             // a lane whose any-hit (shadow) ray still has one (its stack top, with the ray), so
@@ -308,21 +376,24 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
             // Closest-hit rays are not shared: their tie order at equal distances would follow
             // the schedule.
             const bool can_give = active && any && sp > 0;
-            const uint64_t idle = __ballot(!active), donors = __ballot(can_give);
+            // (a lane with a pending light of its own hit is not idle)
+            const uint64_t idle = __ballot(!active && !(kFold && slot < 0)), donors = __ballot(can_give);
             if (idle != 0 && donors != 0) {
                 const int npair = min(__popcll(idle), __popcll(donors));
                 const int drank = __popcll(donors & lt), irank = __popcll(idle & lt);
                 int give = 0;
                 if (can_give && drank < npair) give = stk.pop(--sp);
-                const bool take = !active && irank < npair;
+                const bool take = !active && !(kFold && slot < 0) && irank < npair;
                 const int src = take ? nth_lane(donors, irank) : lane;
                 const int g = __shfl(give, src, 64), sl = __shfl(slot, src, 64);
+                const int dq = kFold ? __shfl(qi, src, 64) : 0;
                 const float bt = __shfl(best_t, src, 64);
                 const V3 o = v3(__shfl(r.o.x, src, 64), __shfl(r.o.y, src, 64), __shfl(r.o.z, src, 64));
                 const V3 d = v3(__shfl(r.d.x, src, 64), __shfl(r.d.y, src, 64), __shfl(r.d.z, src, 64));
                 const V3 iv = v3(__shfl(r.inv.x, src, 64), __shfl(r.inv.y, src, 64), __shfl(r.inv.z, src, 64));
                 if (take) {
                     any = true;
+                    if (kFold) qi = dq < 0 ? -2 : 0;  // a helper marks the array its donor marks
                     r.o = o;
                     r.d = d;
                     r.inv = iv;
@@ -336,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
                 }
             }
         }
-        if (__ballot(active) == 0) {
+        if (__ballot(active || (kFold && slot < 0)) == 0) {
             if (exhausted) break;
             continue;
         }
@@ -401,9 +472,31 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(const TNode* __restrict__ t
         }
         if (node == kNone && leaf == 0) {
             if (any) {
-                if (best_tri >= 0) nee[slot] = 2;
+                if (kFold && qi < 0) {
+                    if (best_tri >= 0) shadow.occl[slot] = 2;
+                    if (qi == -1) {  // the lane's own hit: its next light, if any
+                        const int nx = slot + 1;
+                        slot = nx % shadow.nl != 0 ? ~nx : 0;
+                    }
+                } else if (best_tri >= 0) {
+                    nee[slot] = 2;
+                }
             } else {
                 hits[qi] = make_float4(best_t, best_u, best_v, __int_as_float(best_tri));
+                if (kFold && best_tri >= 0) {
+                    // the last bounce's shading point and facing normal, with the shading
+                    // kernel's expressions (k_wf_shade), and its first light as pending
+                    int mat;
+                    float tu, tv;
+                    const Hit h{best_tri, best_t, best_u, best_v};
+                    V3 N = hit_normal(shadow.P.tris, h, mat, tu, tv);
+                    if (dot(N, r.d) > 0.0f) N = mul(N, -1.0f);
+                    const V3 Pt = add(r.o, mul(r.d, best_t));
+                    float* f = s_fold + threadIdx.x;
+                    f[0] = Pt.x; f[kBlock] = Pt.y; f[2 * kBlock] = Pt.z;
+                    f[3 * kBlock] = N.x; f[4 * kBlock] = N.y; f[5 * kBlock] = N.z;
+                    slot = ~(qi * shadow.nl);
+                }
             }
             active = false;
         }
@@ -505,7 +598,10 @@ __global__ __launch_bounds__(kBlock) void k_wf_shade(WfParams W, const QRay* q, 
             V3 c = v3(0.f, 0.f, 0.f);
             const bool cand = live && nee_candidate(W.P, li, Pt, N, albedo, shadow, tmax, c);
             const int64_t j = (int64_t)i * W.nl + li;
-            W.nee[j] = cand ? 1 : 0;
+            // the last bounce's shadow rays were traced in the launch before (kFold): their
+            // occlusion marks (written for every candidate)
+            const bool occluded = W.occl && bounce == W.P.bounces && cand && W.occl[j] == 2;
+            W.nee[j] = occluded ? 2 : cand ? 1 : 0;
             if (cand) { W.contrib[3 * j] = c.x; W.contrib[3 * j + 1] = c.y; W.contrib[3 * j + 2] = c.z; }
             ncand += cand ? 1u : 0u;
         }
@@ -693,14 +789,14 @@ int trace_grid(int n) {
     return std::max(1, std::min(cdiv(n, kBlock), kTraceGridMax));
 }
 
-template <int kMode>
+template <int kMode, bool kFold = false>
 void launch_trace(hipStream_t st, const TNode* tn, int n_tn, const TriV* tv, const QRay* q, int n,
                   const uint32_t* n_dev, int n_sh, const uint32_t* n_sh_dev, float4* hits, uint8_t* nee,
                   uint32_t* cursors, int* spill, const ShadowSrc& shadow, const PrimarySrc& prim,
                   const int32_t* qord) {
     constexpr int kMin = kMode == kTraceShadow ? kTraceMinLanesAny : kTraceMinLanes;
     const int grid = trace_grid((kMode == kTraceShadow ? 0 : n) + (kMode == kTraceClosest ? 0 : n_sh));
-    hipLaunchKernelGGL((k_wf_trace<kMode, kTraceLdsStack, kTraceTopNodes, kMin>), dim3(grid), dim3(kBlock), 0, st, tn,
+    hipLaunchKernelGGL((k_wf_trace<kMode, kTraceLdsStack, kTraceTopNodes, kMin, kFold>), dim3(grid), dim3(kBlock), 0, st, tn,
                        n_tn, tv, q, n, n_dev, n_sh, n_sh_dev, hits, nee, cursors, spill, shadow, prim, qord);
 }
 
@@ -786,7 +882,12 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
     W.nee = (uint8_t*)S_(63, cap * nl);
     W.vP = (float4*)S_(64, cap * 16);
     W.vN = (float4*)S_(70, cap * 16);
-    ShadowSrc shadow{sids, W.vP, W.vN, nl, P};
+    // With a bounce after the first hit, the last bounce's shadow rays are traced in the launch
+    // that finds its closest hits (kFold).
+    const bool fold = P.bounces >= 1;
+    W.occl = fold ? (uint8_t*)S_(75, cap * nl) : nullptr;
+    if (fold && !W.occl) return lgi_set_err(ctx, LGI_ERR_HIP, "wavefront buffer allocation failed");
+    ShadowSrc shadow{sids, W.vP, W.vN, nl, P, W.occl};
     for (void* p : {(void*)qa, (void*)qb, (void*)hits, (void*)sids, (void*)blk_s, (void*)blk_n, (void*)ncont,
                     (void*)keys_tmp,
                     (void*)keys, (void*)idx, sort_scratch, (void*)scan_scratch, (void*)counters, (void*)W.T,
@@ -829,6 +930,10 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             const bool last = b == P.bounces;
             hipLaunchKernelGGL(k_wf_shade, dim3(nb), dim3(kBlock), 0, st, W, cur, hits, n_max, n_dev, b, other,
                                sort ? keys_tmp : nullptr, ncont, blk_s, blk_n, prim, qord);
+            if (last && fold) {  // its shadow rays are traced: add the unoccluded light
+                hipLaunchKernelGGL(k_wf_resolve, dim3(nb), dim3(kBlock), 0, st, W, n_max, n_dev, b, prim);
+                break;
+            }
             // block offsets of the shadow-ray candidates and (but for the last bounce) of the
             // continuations, in two parts that the compaction kernels add
             lgi_exclusive_scan_chunks(st, blk_s, nb, cnt + 1, sums_s);
@@ -854,6 +959,9 @@ lgi_status lgi_render_wavefront(lgi_ctx* ctx, const TraceParams& P, float3 bb_mi
             if (last)
                 launch_trace<kTraceShadow>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, nullptr, 0, nullptr, n_max * nl,
                                            cnt + 1, nullptr, W.nee, cur_sh, spill, shadow, prim, nullptr);
+            else if (fold && b + 1 == P.bounces)
+                launch_trace<kTraceBoth, true>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2,
+                                               n_max * nl, cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim, qord);
             else
                 launch_trace<kTraceBoth>(st, ctx->dev.tnodes, n_tn, ctx->dev.triv, cur, n_max, cnt + 2, n_max * nl,
                                          cnt + 1, hits, W.nee, cur_sh, spill, shadow, prim, qord);

@@ -92,6 +92,20 @@ def test_sorted_wavefront_matches_inline_and_oracle(gpu, flags):
     assert_close_images(wave[0].cpu().numpy(), o["sum"], f"sorted flags={flags}")
 
 
+@pytest.mark.parametrize("bounces", [0, 1, 3])
+@pytest.mark.parametrize("flags", [2, 2 | 8], ids=["sorted", "no_ray_sort"])
+def test_wavefront_bounce_counts_match_inline(gpu, flags, bounces):
+    """Other bounce counts: the last bounce's shadow rays are traced in the launch that finds its
+    closest hits once there is a bounce (with 3, after a launch that does not), or in a launch of
+    their own without one; the sums equal the inline kernel's bit for bit either way."""
+    ctx, _, s = make_pair(seed=6)
+    cam = small_camera(s["camera"], 200, 100)
+    inline = gpu_accumulate(ctx, cam, 8, bounces)
+    wave = gpu_accumulate(ctx, cam, 8, bounces, flags=flags)
+    for a, b in zip(inline, wave):
+        assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+
+
 def test_converged_means_agree_across_rng_policies(gpu):
     """4096-spp means of the PCG32 sorted renderer and of the reference's xorshift32 seeding
     (src/pathtracer.cu:345-364) agree within 3 standard errors of their difference: the RNG
