@@ -534,6 +534,11 @@ constexpr float kSahNode = 1.2f, kSahTri = 1.0f;
 // 6 leaves: 19.18-19.24 ms, 2.21 ms; 4 leaves: 19.50 ms, 1.68 ms; one 5-leaf pass: 19.56 ms, 1.00 ms.
 constexpr int kTreeletLeaves = 5;
 constexpr int kSubsets = 1 << kTreeletLeaves;
+// A wave works on 4 treelets at once, 16 lanes each: forming one takes up to 8 lanes (the
+// prefetch's deepest level) and its dynamic program at most 10 (the 10 subsets of 2 or of 3
+// leaves, one lane each; the full set of 5 takes 2). The wide levels (up to ~85 k treelets) are
+// bound by one treelet's chain of dependent loads, so four chains per wave.
+constexpr int kTreeletLanes = 16, kTreeletsPerWave = 64 / kTreeletLanes;
 
 __device__ __forceinline__ float half_area(const float* mn, const float* mx) {
     const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
@@ -606,13 +611,15 @@ __device__ __forceinline__ void treelet_leaf(TreeletForm& F, TreeletGrow& G, int
     G.heap[at] = h;
 }
 
-// Forms the treelet of internal node p (the subtrees below it are final for this pass); all
-// lanes of the wave call it. The wave first loads p's subtree to depth 3 with the subtree
-// costs, one level per round, so that most treelets form from LDS. Lane 0 then refreshes p's
-// cost and opens the internal treelet leaf of largest area (the first on ties) until there
-// are kTreeletLeaves leaves, loading children from memory only below the prefetched levels.
-__device__ void treelet_form(const lgi_bvh_node* nodes, float* cost, int p, TreeletForm& F, TreeletGrow& G) {
-    const int lane = threadIdx.x;
+// Forms the treelet of internal node p (the subtrees below it are final for this pass); the
+// kTreeletLanes lanes of its group call it (lane: 0..15 within the group). The group first
+// loads p's subtree to depth 3 with the subtree costs, one level per round, so that most
+// treelets form from LDS. Lane 0 then refreshes p's cost and opens the internal treelet leaf
+// of largest area (the first on ties) until there are kTreeletLeaves leaves, loading children
+// from memory only below the prefetched levels.
+__device__ void treelet_form(const lgi_bvh_node* nodes, float* cost, int p, TreeletForm& F, TreeletGrow& G,
+                             int lane) {
+    static_assert(kTreeletLanes >= 8, "the prefetch's deepest level takes 8 lanes");
     if (lane == 0) G.node[1] = nodes[p];
     __syncthreads();
     for (int lv = 1; lv <= 3; ++lv) {
@@ -659,11 +666,10 @@ __device__ void treelet_form(const lgi_bvh_node* nodes, float* cost, int p, Tree
     F.n = nl;
 }
 
-// All 64 lanes find the least-cost topology of the formed treelet F and, when it is cheaper
-// than the current one, lane 0 rewires F's internal slots to it.
+// The kTreeletLanes lanes of a group find the least-cost topology of the formed treelet F and,
+// when it is cheaper than the current one, lane 0 rewires F's internal slots to it.
 __device__ void treelet_optimize(lgi_bvh_node* nodes, float* cost, const TreeletForm& F, TreeletDp& S,
-                                 const SubsetsBySize& by_size) {
-    const int lane = threadIdx.x;
+                                 const SubsetsBySize& by_size, int lane) {
     const int nl = F.n, ns = 1 << nl;
     if (lane < nl) S.c[1 << lane] = F.c[lane];
     {
@@ -675,7 +681,7 @@ __device__ void treelet_optimize(lgi_bvh_node* nodes, float* cost, const Treelet
                 lmn[i][k] = F.mn[i][k];
                 lmx[i][k] = F.mx[i][k];
             }
-        for (int sub = lane; sub < ns; sub += 64) {
+        for (int sub = lane; sub < ns; sub += kTreeletLanes) {
             if (__popc(sub) < 2) continue;
             float mn[3] = {0.f, 0.f, 0.f}, mx[3] = {0.f, 0.f, 0.f};
             const int first = __ffs(sub) - 1;
@@ -698,6 +704,7 @@ __device__ void treelet_optimize(lgi_bvh_node* nodes, float* cost, const Treelet
     // lowest leaf alone; which lane evaluated a split does not change it.
     for (int size = 2; size <= nl; ++size) {
         const int la = size <= 4 ? 0 : size == 5 ? 1 : size == 6 ? 3 : 6;  // log2 of the lanes per subset
+        static_assert(kTreeletLeaves <= 5 && kTreeletLanes >= 10, "a group's lanes cover every subset of a size");
         const int g = 1 << la, which = lane >> la, part = lane & (g - 1);
         const int k0 = by_size.start[size], count = by_size.start[size + 1] - k0;
         const int sub = which < count ? by_size.sub[k0 + which] : 0;
@@ -779,26 +786,28 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cost(const lgi_bvh_node* nodes,
     if (nd.tri_count > 0) cost[p] = half_area(nd.bb_min, nd.bb_max) * kSahTri * (float)nd.tri_count;
 }
 
-// Restructures the treelet rooted at order[p0 + blockIdx.x] (p0 + blockIdx.x when order is
-// null), one wave per treelet. The treelets of one level are disjoint.
-__global__ __launch_bounds__(64) void k_treelet_level(lgi_bvh_node* nodes, const int32_t* order, int p0,
+// Restructures the treelets rooted at order[i] (i when order is null) for i in [p0, p1), four
+// per wave (one per group of kTreeletLanes lanes). The treelets of one level are disjoint.
+__global__ __launch_bounds__(64) void k_treelet_level(lgi_bvh_node* nodes, const int32_t* order, int p0, int p1,
                                                       float* cost) {
-    __shared__ TreeletForm F;
+    __shared__ TreeletForm F[kTreeletsPerWave];
     __shared__ union {
         TreeletGrow g;
         TreeletDp dp;
-    } U;
+    } U[kTreeletsPerWave];
     __shared__ SubsetsBySize by_size;
-    const int i = p0 + blockIdx.x;
-    const int p = order ? order[i] : i;
-    if (nodes[p].tri_count > 0) return;
-    // the subset table goes to LDS while the treelet forms: the DP indexes it per lane, which
+    // the subset table goes to LDS while the treelets form: the DP indexes it per lane, which
     // from constant memory would be a vector memory round trip per subset size
     for (int k = threadIdx.x; k < (int)sizeof(SubsetsBySize); k += 64)
         reinterpret_cast<uint8_t*>(&by_size)[k] = reinterpret_cast<const uint8_t*>(&kBySize)[k];
-    treelet_form(nodes, cost, p, F, U.g);
+    const int grp = threadIdx.x / kTreeletLanes, lane = threadIdx.x % kTreeletLanes;
+    const int i = p0 + blockIdx.x * kTreeletsPerWave + grp;
+    if (i >= p1) return;
+    const int p = order ? order[i] : i;
+    if (nodes[p].tri_count > 0) return;
+    treelet_form(nodes, cost, p, F[grp], U[grp].g, lane);
     __syncthreads();
-    if (F.n >= 3) treelet_optimize(nodes, cost, F, U.dp, by_size);  // two leaves have one tree
+    if (F[grp].n >= 3) treelet_optimize(nodes, cost, F[grp], U[grp].dp, by_size, lane);  // two leaves: one tree
 }
 
 // Levels of the current tree, in parallel: every slot finds its depth by walking its parent
@@ -1186,8 +1195,8 @@ lgi_status lgi_gpu_build_bvh(lgi_ctx* ctx) {
             }
             if (pass == passes) break;
             for (int lv = n_lev - 2; lv >= 0; --lv)
-                hipLaunchKernelGGL(k_treelet_level, dim3(lv_host[lv + 1] - lv_host[lv]), dim3(64), 0, st, onodes,
-                                   lv_order, lv_host[lv], cost);
+                hipLaunchKernelGGL(k_treelet_level, dim3(ceil_div(lv_host[lv + 1] - lv_host[lv], kTreeletsPerWave)),
+                                   dim3(64), 0, st, onodes, lv_order, lv_host[lv], lv_host[lv + 1], cost);
         }
         if (tree_ok && n_lev - 1 <= kMaxTraversalDepth) {
             hipLaunchKernelGGL(k_bfs_flags, dim3(ceil_div(m, kBlock)), dim3(kBlock), 0, st, onodes, order, (int)m,
