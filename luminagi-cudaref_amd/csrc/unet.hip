@@ -784,74 +784,92 @@ __global__ __launch_bounds__(kBlock, RW * NB >= 32 ? 1 : 2) void k_conv_rb(ConvA
 // instead of the 9 taps x 32 (mostly zero) channels of the packed-input path. Output tile
 // 8 x 16 pixels x 32 channels; weights [32][9][cin_w] BF16 (channels 0..2 used).
 // kLog: x holds radiance and the layer reads log1p(max(x, 0)).
+// Persistent blocks (a few per CU) stage the weights once and walk the tiles; each thread's
+// halo pixel of the next tile is loaded into registers before the current tile is computed,
+// so the loads' latency overlaps the MFMAs and the stores.
+__device__ __forceinline__ void first_halo_load(const float* __restrict__ x, int tile, int tiles_x, int tiles_y,
+                                                int H, int W, float& v0, float& v1, float& v2) {
+    const int n = tile / (tiles_x * tiles_y);
+    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+    const int t = threadIdx.x, hy = t / kHW, hx = t % kHW;
+    const int gy = ty * kTH + hy - 1, gx = tx * kTW + hx - 1;
+    v0 = 0.f; v1 = 0.f; v2 = 0.f;
+    if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+        const float* px = x + (((int64_t)n * H + gy) * W + gx) * 3;
+        v0 = px[0]; v1 = px[1]; v2 = px[2];
+    }
+}
+
 template <bool kLog>
 __global__ __launch_bounds__(kBlock) void k_conv_first(const float* __restrict__ x, const bf16* __restrict__ wk,
                                                        int cin_w, const float* __restrict__ bias, int N, int H, int W,
                                                        int relu, bf16* __restrict__ out) {
+    static_assert(kHH * kHW <= kBlock, "one halo pixel per thread");
     __shared__ bf16 s_in[kHH * kHW * 4];  // halo pixels x 4 (3 channels + pad)
     __shared__ __attribute__((aligned(16))) bf16 s_w[32 * 40];  // [co][k], k = tap * 3 + c, 40 = padded row
+    __shared__ __attribute__((aligned(16))) bf16 s_out[kTH * kTW * 32];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tiles_x = (W + kTW - 1) / kTW, tiles_y = (H + kTH - 1) / kTH;
-    const int tile = blockIdx.x;
-    const int n = tile / (tiles_x * tiles_y);
-    const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
-    const int y0 = ty * kTH, x0 = tx * kTW;
-    for (int t = tid; t < kHH * kHW; t += kBlock) {
-        const int hy = t / kHW, hx = t % kHW;
-        const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
-            const float* px = x + (((int64_t)n * H + gy) * W + gx) * 3;
-            v0 = px[0]; v1 = px[1]; v2 = px[2];
-            if (kLog) {
-                v0 = to_log(v0); v1 = to_log(v1); v2 = to_log(v2);
-            }
-        }
-        s_in[t * 4 + 0] = (bf16)v0; s_in[t * 4 + 1] = (bf16)v1; s_in[t * 4 + 2] = (bf16)v2;
-        s_in[t * 4 + 3] = (bf16)0.f;
-    }
+    const int tiles_x = (W + kTW - 1) / kTW, tiles_y = (H + kTH - 1) / kTH, ntiles = tiles_x * tiles_y * N;
+    const bool halo = tid < kHH * kHW;
     for (int t = tid; t < 32 * 32; t += kBlock) {
         const int co = t >> 5, k = t & 31;
         s_w[co * 40 + k] = k < 27 ? wk[((int64_t)co * 9 + k / 3) * cin_w + k % 3] : (bf16)0.f;
     }
-    __syncthreads();
     const int g = lane >> 4, r16 = lane & 15;
-    f32x4 acc[2][2];
-    for (int mi = 0; mi < 2; ++mi) {
-        bf16x8 A;
-        for (int i = 0; i < 8; ++i) {
-            const int k = 8 * g + i;
-            bf16 v = (bf16)0.f;
-            if (k < 27) {
-                const int tap = k / 3, c = k % 3, ky = tap / 3, kx = tap % 3;
-                v = s_in[((wave * 2 + mi + ky) * kHW + r16 + kx) * 4 + c];
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;
+    if (halo && (int)blockIdx.x < ntiles) first_halo_load(x, blockIdx.x, tiles_x, tiles_y, H, W, c0, c1, c2);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        __syncthreads();  // the previous tile's readers of s_in and s_out are done
+        if (halo) {
+            float v0 = c0, v1 = c1, v2 = c2;
+            if (kLog) {
+                v0 = to_log(v0); v1 = to_log(v1); v2 = to_log(v2);
             }
-            A[i] = v;
+            s_in[tid * 4 + 0] = (bf16)v0; s_in[tid * 4 + 1] = (bf16)v1; s_in[tid * 4 + 2] = (bf16)v2;
+            s_in[tid * 4 + 3] = (bf16)0.f;
+            const int next = tile + (int)gridDim.x;
+            if (next < ntiles) first_halo_load(x, next, tiles_x, tiles_y, H, W, c0, c1, c2);
         }
-        for (int ni = 0; ni < 2; ++ni) {
-            const bf16x8 B = *reinterpret_cast<const bf16x8*>(&s_w[(ni * 16 + r16) * 40 + g * 8]);
-            acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-        }
-    }
-    // epilogue through LDS: the 8 x 16 x 32 tile as [row][column][channel], then 16-byte stores
-    __shared__ __attribute__((aligned(16))) bf16 s_out[kTH * kTW * 32];
-    for (int mi = 0; mi < 2; ++mi)
-        for (int ni = 0; ni < 2; ++ni) {
-            const int co = ni * 16 + r16;
-            const float b = bias ? bias[co] : 0.0f;
-            for (int j = 0; j < 4; ++j) {
-                float v = acc[mi][ni][j] + b;
-                if (relu) v = fmaxf(v, 0.0f);
-                s_out[((wave * 2 + mi) * kTW + 4 * g + j) * 32 + co] = (bf16)v;
+        __syncthreads();
+        const int n = tile / (tiles_x * tiles_y);
+        const int ty = (tile / tiles_x) % tiles_y, tx = tile % tiles_x;
+        const int y0 = ty * kTH, x0 = tx * kTW;
+        f32x4 acc[2][2];
+        for (int mi = 0; mi < 2; ++mi) {
+            bf16x8 A;
+            for (int i = 0; i < 8; ++i) {
+                const int k = 8 * g + i;
+                bf16 v = (bf16)0.f;
+                if (k < 27) {
+                    const int tap = k / 3, c = k % 3, ky = tap / 3, kx = tap % 3;
+                    v = s_in[((wave * 2 + mi + ky) * kHW + r16 + kx) * 4 + c];
+                }
+                A[i] = v;
+            }
+            for (int ni = 0; ni < 2; ++ni) {
+                const bf16x8 B = *reinterpret_cast<const bf16x8*>(&s_w[(ni * 16 + r16) * 40 + g * 8]);
+                acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
             }
         }
-    __syncthreads();
-    for (int t = tid; t < kTH * kTW * 4; t += kBlock) {
-        const int px = t >> 2, cg = t & 3;
-        const int y = y0 + px / kTW, xx = x0 + px % kTW;
-        if (y >= H || xx >= W) continue;
-        *reinterpret_cast<uint4*>(out + (((int64_t)n * H + y) * W + xx) * 32 + cg * 8) =
-            *reinterpret_cast<const uint4*>(&s_out[px * 32 + cg * 8]);
+        // epilogue through LDS: the 8 x 16 x 32 tile as [row][column][channel], then 16-byte stores
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni) {
+                const int co = ni * 16 + r16;
+                const float b = bias ? bias[co] : 0.0f;
+                for (int j = 0; j < 4; ++j) {
+                    float v = acc[mi][ni][j] + b;
+                    if (relu) v = fmaxf(v, 0.0f);
+                    s_out[((wave * 2 + mi) * kTW + 4 * g + j) * 32 + co] = (bf16)v;
+                }
+            }
+        __syncthreads();
+        for (int t = tid; t < kTH * kTW * 4; t += kBlock) {
+            const int px = t >> 2, cg = t & 3;
+            const int y = y0 + px / kTW, xx = x0 + px % kTW;
+            if (y >= H || xx >= W) continue;
+            *reinterpret_cast<uint4*>(out + (((int64_t)n * H + y) * W + xx) * 32 + cg * 8) =
+                *reinterpret_cast<const uint4*>(&s_out[px * 32 + cg * 8]);
+        }
     }
 }
 
@@ -1958,12 +1976,22 @@ lgi_status lgi_unet_conv3x3_up_fwd(const void* lo, int32_t cu, const void* skip,
 static lgi_status first_fwd(bool log_in, const float* x, const void* w, int32_t cin_w, const float* bias, int32_t n,
                             int32_t h, int32_t wd, int32_t relu, void* out, void* stream) {
     if (!x || !w || !out || cin_w < 3 || n <= 0 || h <= 0 || wd <= 0) return LGI_ERR_INVALID_ARG;
-    const int tiles = cdiv(wd, kTW) * cdiv(h, kTH) * n;
+    const int64_t tiles = (int64_t)cdiv(wd, kTW) * cdiv(h, kTH) * n;
+    if (tiles > INT32_MAX) return LGI_ERR_INVALID_ARG;
+    // persistent: as many blocks as are resident at once, never more than tiles
+    int dev = 0, cus = 256, per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(
+                                                                  log_in ? k_conv_first<true> : k_conv_first<false>),
+                                                     kBlock, 0) != hipSuccess)
+        return LGI_ERR_HIP;
+    const int grid = (int)std::min<int64_t>(tiles, (int64_t)cus * std::max(1, per_cu));
     if (log_in)
-        hipLaunchKernelGGL(k_conv_first<true>, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
+        hipLaunchKernelGGL(k_conv_first<true>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
                            cin_w, bias, n, h, wd, relu, (bf16*)out);
     else
-        hipLaunchKernelGGL(k_conv_first<false>, dim3(tiles), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
+        hipLaunchKernelGGL(k_conv_first<false>, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, x, (const bf16*)w,
                            cin_w, bias, n, h, wd, relu, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }

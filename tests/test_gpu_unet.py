@@ -386,6 +386,27 @@ def test_first_layer_kernel_matches_packed_path_and_fp64(gpu):
     assert e1 < BF16_TOL and d < 1e-2
 
 
+def test_first_layer_kernel_many_tiles_per_block(gpu):
+    """A frame with more 8x16 tiles than the persistent first layer has blocks (each block walks
+    several tiles, loading the next one's halo ahead), ragged in both directions, against an
+    FP32 convolution of the same BF16-rounded operands."""
+    from lgi import unet
+    g = torch.Generator().manual_seed(22)
+    n, h, w = 2, 517, 1030  # 2 x 65 x 65 = 8450 tiles
+    x = torch.log1p(torch.rand(n, h, w, 3, generator=g) * 3.0)
+    wt = torch.zeros(32, 9, 32)
+    wt[:, :, :3] = torch.randn(32, 9, 3, generator=g) * (2.0 / 27) ** 0.5
+    wb = wt.to(torch.bfloat16)
+    b = torch.randn(32, generator=g) * 0.1
+    y = unet.conv3x3_first_fwd(x.cuda(), wb.cuda(), b.cuda(), True).float().cpu()
+    xin = x.to(torch.bfloat16).float().permute(0, 3, 1, 2)
+    wk = wb.float()[:, :, :3].reshape(32, 3, 3, 3).permute(0, 3, 1, 2)  # [co][c][ky][kx]
+    ref = torch.relu(torch.nn.functional.conv2d(xin, wk, b, padding=1)).permute(0, 2, 3, 1)
+    err = ((y - ref).abs().max() / ref.abs().max()).item()
+    print(f"first layer, {n}x{h}x{w}: rel err {err:.2e}", flush=True)
+    assert err < BF16_TOL
+
+
 POOL_CASES = [  # n, h, w, c0, c1, cout
     (2, 16, 32, 32, 0, 32),
     (1, 14, 46, 64, 0, 64),     # ragged tiles in both directions
