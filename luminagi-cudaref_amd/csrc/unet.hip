@@ -1130,23 +1130,59 @@ __device__ __forceinline__ void lin_src(int d, int in, int& i0, int& i1, float& 
     l1 = s - (float)i0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
-    const int Ho = 2 * H, Wo = 2 * W, C8 = C / 8;
-    int n, y, x, c;
-    if (!row_thread(Ho, Wo, C8, n, y, x, c)) return;
-    const int64_t p = ((int64_t)n * Ho + y) * Wo + x;
-    int y0, y1, x0, x1;
-    float ly, lx;
-    lin_src(y, H, y0, y1, ly);
-    lin_src(x, W, x0, x1, lx);
-    auto at = [&](int yy, int xx, float f[8]) {
-        unpack8(*reinterpret_cast<const uint4*>(in + (((int64_t)n * H + yy) * W + xx) * C + c), f);
-    };
-    float a[8], b[8], cc[8], d[8], v[8];
-    at(y0, x0, a); at(y0, x1, b); at(y1, x0, cc); at(y1, x1, d);
+// Output (y, x) = the bilinear blend of the input pixels lin_src picks, in the expression order
+// below; e8 = 8 channels at column offset c.
+__device__ __forceinline__ void up_blend(const float a[8], const float b[8], const float cc[8], const float d[8],
+                                         float ly, float lx, float v[8]) {
     for (int e = 0; e < 8; ++e)
         v[e] = (1.f - ly) * ((1.f - lx) * a[e] + lx * b[e]) + ly * ((1.f - lx) * cc[e] + lx * d[e]);
-    *reinterpret_cast<uint4*>(out + p * C + c) = pack8(v);
+}
+
+// One thread per input pixel (i, j) and 8 channels writes the 2 x 2 output pixels (2i + a,
+// 2j + b). Away from the border their sources are the 3 x 3 input neighbourhood of (i, j):
+// output row 2i reads rows i-1 and i, row 2i + 1 rows i and i + 1 (lin_src), so the 9 pixels
+// are loaded once for the 4 outputs (instead of 4 loads per output). Border threads evaluate
+// each output from lin_src's indices. Same arithmetic either way.
+__global__ __launch_bounds__(kBlock) void k_up_fwd(const bf16* in, int N, int H, int W, int C, bf16* out) {
+    const int Ho = 2 * H, Wo = 2 * W, C8 = C / 8;
+    int n, i, j, c;
+    if (!row_thread(H, W, C8, n, i, j, c)) return;
+    auto px = [&](int yy, int xx) {
+        return *reinterpret_cast<const uint4*>(in + (((int64_t)n * H + yy) * W + xx) * C + c);
+    };
+    if (i >= 1 && i + 1 < H && j >= 1 && j + 1 < W) {
+        uint4 q[3][3];
+#pragma unroll
+        for (int ry = 0; ry < 3; ++ry)
+#pragma unroll
+            for (int rx = 0; rx < 3; ++rx) q[ry][rx] = px(i - 1 + ry, j - 1 + rx);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            int y0, y1, x0, x1;
+            float ly, lx, v[8];
+            lin_src(2 * i + a, H, y0, y1, ly);  // y0 = i - 1 + a, y1 = i + a
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                lin_src(2 * j + b, W, x0, x1, lx);  // x0 = j - 1 + b, x1 = j + b
+                float fa[8], fb[8], fc[8], fd[8];
+                unpack8(q[a][b], fa); unpack8(q[a][b + 1], fb); unpack8(q[a + 1][b], fc); unpack8(q[a + 1][b + 1], fd);
+                up_blend(fa, fb, fc, fd, ly, lx, v);
+                *reinterpret_cast<uint4*>(out + (((int64_t)n * Ho + 2 * i + a) * Wo + 2 * j + b) * C + c) = pack8(v);
+            }
+        }
+        return;
+    }
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int y = 2 * i + a, x = 2 * j + b;
+            int y0, y1, x0, x1;
+            float ly, lx, fa[8], fb[8], fc[8], fd[8], v[8];
+            lin_src(y, H, y0, y1, ly);
+            lin_src(x, W, x0, x1, lx);
+            unpack8(px(y0, x0), fa); unpack8(px(y0, x1), fb); unpack8(px(y1, x0), fc); unpack8(px(y1, x1), fd);
+            up_blend(fa, fb, fc, fd, ly, lx, v);
+            *reinterpret_cast<uint4*>(out + (((int64_t)n * Ho + y) * Wo + x) * C + c) = pack8(v);
+        }
 }
 
 // Adjoint of k_up_fwd as a gather: each input pixel sums the weighted gradients of the
@@ -2116,7 +2152,7 @@ lgi_status lgi_unet_maxpool_bwd(const void* in, const void* dout, int32_t n, int
 
 lgi_status lgi_unet_upsample_fwd(const void* in, int32_t n, int32_t h, int32_t w, int32_t c, void* out, void* stream) {
     if (!in || !out || c % 8 || !row_grid_fits(n, 2 * (int64_t)h, 2 * (int64_t)w, c / 8)) return LGI_ERR_INVALID_ARG;
-    hipLaunchKernelGGL(k_up_fwd, row_grid(n, 2 * h, 2 * w, c / 8), dim3(kBlock), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_up_fwd, row_grid(n, h, w, c / 8), dim3(kBlock), 0, (hipStream_t)stream,
                        (const bf16*)in, n, h, w, c, (bf16*)out);
     return hipGetLastError() == hipSuccess ? LGI_OK : LGI_ERR_HIP;
 }

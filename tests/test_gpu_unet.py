@@ -133,6 +133,40 @@ def test_upsample_forward_backward(gpu):
     assert rel(xc.grad, xd.grad) < 1e-2
 
 
+def _up_exact(x):
+    """The upsample's arithmetic in FP32, one operation at a time (the kernel is built without
+    contraction): lin_src's source rows/columns and weights, then the blend, rounded to BF16."""
+    n, h, w, c = x.shape
+    xf = x.float().numpy()
+
+    def src(d, size):
+        s = (np.arange(d, dtype=np.float32) + np.float32(0.5)) * np.float32(0.5) - np.float32(0.5)
+        s = np.maximum(s, np.float32(0.0))
+        i0 = s.astype(np.int64)
+        return i0, np.minimum(i0 + 1, size - 1), (s - i0.astype(np.float32)).astype(np.float32)
+
+    y0, y1, ly = src(2 * h, h)
+    x0, x1, lx = src(2 * w, w)
+    ly = ly[None, :, None, None]
+    lx = lx[None, None, :, None]
+    one = np.float32(1.0)
+    a, b = xf[:, y0][:, :, x0], xf[:, y0][:, :, x1]
+    cc, d = xf[:, y1][:, :, x0], xf[:, y1][:, :, x1]
+    v = (one - ly) * ((one - lx) * a + lx * b) + ly * ((one - lx) * cc + lx * d)
+    return torch.from_numpy(v.astype(np.float32)).to(torch.bfloat16)
+
+
+@pytest.mark.parametrize("shape", [(2, 9, 13, 32), (1, 1, 1, 8), (1, 2, 3, 16), (1, 54, 108, 64)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_upsample_forward_bit_exact(gpu, shape):
+    """The upsample's output, bit for bit, against its arithmetic restated in FP32 on the CPU:
+    interior threads (which share one 3 x 3 neighbourhood among their 4 outputs) and the border."""
+    from lgi import unet
+    x = bf16_rand(*shape, seed=14)
+    y = unet.UpsampleFn.apply(x.cuda()).cpu()
+    assert torch.equal(y.view(torch.int16), _up_exact(x).view(torch.int16))
+
+
 UP_CASES = [  # n, h, w (upsample input), cu, cs, cout, relu
     (1, 8, 32, 64, 32, 32, True),      # one tile
     (2, 11, 45, 32, 32, 64, True),     # ragged tiles, two images, two output-channel blocks
