@@ -248,6 +248,10 @@ __device__ __forceinline__ void conv32_epilogue(const ConvArgs& a, const f32x16 
                 s_out[((wave * RW + mi) * kTW32 + 8 * (j >> 2) + 4 * kg + (j & 3)) * TN + c] = (bf16)v;
             }
         }
+    // the tail's 3 x 32 weights and 3 biases go to LDS first: read per pixel from memory, the
+    // compiler reloads them after every output store (the pointers may alias)
+    __shared__ float s_tail[TN == 32 ? 3 * 32 + 3 : 1];
+    if (TN == 32 && a.tail_out && tid < 3 * 32 + 3) s_tail[tid] = tid < 3 * 32 ? a.tail_w[tid] : a.tail_b[tid - 3 * 32];
     __syncthreads();
     if (TN == 32 && a.tail_out) {
         // the tail on the BF16 features of each pixel, in k_tail_fwd's order of operations
@@ -258,8 +262,8 @@ __device__ __forceinline__ void conv32_epilogue(const ConvArgs& a, const f32x16 
             for (int q = 0; q < 4; ++q) unpack8(*reinterpret_cast<const uint4*>(&s_out[px * TN + q * 8]), f + q * 8);
             const int64_t p = ((int64_t)n * a.H + y) * a.W + x;
             for (int o = 0; o < 3; ++o) {
-                float acc_t = a.tail_b[o];
-                for (int c = 0; c < 32; ++c) acc_t += a.tail_w[o * 32 + c] * f[c];
+                float acc_t = s_tail[3 * 32 + o];
+                for (int c = 0; c < 32; ++c) acc_t += s_tail[o * 32 + c] * f[c];
                 a.tail_out[p * 3 + o] = from_log(to_log(a.tail_x[p * 3 + o]) + acc_t);
             }
         }
